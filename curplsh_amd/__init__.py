@@ -1,0 +1,328 @@
+"""curplsh_amd: MI355X-native brute-force L2 kNN behind the cuRPLSH IndexBF API.
+
+Python mirror of the reference's C++ operator interface for this path
+(include/index.h, include/index_bf.h, include/device_resources.h,
+include/internal/{search,norm}.h), over the C ABI of libcurplsh_amd.so.
+All compute runs in the HIP library; nothing here computes distances.
+
+    res = DeviceResources()
+    idx = IndexBF(res, dim, IndexBFConfig(precision="bf16"))
+    idx.add(x)                      # float32 [n, dim] (host numpy or DeviceArray)
+    I, D = idx.search(q, k)         # int32 / float32 [nq, k], ascending
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import PREC_BF16, PREC_FP8, PREC_FP32, RplshError, call, device_count, load
+
+__all__ = [
+    "DeviceResources", "DeviceArray", "IndexBF", "IndexBFConfig", "RplshError",
+    "search_l2_distance", "compute_l2_norm", "encode_bf16", "encode_e4m3",
+    "gen_uniform", "gen_gaussian", "gen_mixture", "device_count", "Event", "load",
+    "PREC_FP32", "PREC_BF16", "PREC_FP8",
+]
+
+_PREC = {"fp32": PREC_FP32, "plain": PREC_FP32, "bf16": PREC_BF16, "fp8": PREC_FP8,
+         PREC_FP32: PREC_FP32, PREC_BF16: PREC_BF16, PREC_FP8: PREC_FP8}
+
+
+def _ptr(a) -> int:
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays must be C-contiguous")
+        return a.ctypes.data
+    raise TypeError(f"unsupported buffer type {type(a)}")
+
+
+class DeviceResources:
+    """DeviceResources (include/device_resources.h:60-263): per-device streams."""
+
+    def __init__(self, device: int | None = None):
+        load()
+        self._h = C.c_void_p()
+        call("rplsh_resources_create", C.byref(self._h))
+        if device is not None:
+            self.initialize_for_device(device)
+
+    def initialize_for_device(self, device: int) -> None:
+        call("rplsh_resources_init_device", self._h, device)
+
+    def default_stream(self, device: int = 0) -> int:
+        s = C.c_void_p()
+        call("rplsh_resources_default_stream", self._h, device, C.byref(s))
+        return s.value or 0
+
+    def alternate_streams(self, device: int = 0) -> list[int]:
+        n = C.c_int(8)
+        arr = (C.c_void_p * 8)()
+        call("rplsh_resources_alternate_streams", self._h, device, arr, C.byref(n))
+        return [arr[i] or 0 for i in range(n.value)]
+
+    def async_copy_stream(self, device: int = 0) -> int:
+        s = C.c_void_p()
+        call("rplsh_resources_async_copy_stream", self._h, device, C.byref(s))
+        return s.value or 0
+
+    def sync_default_stream(self, device: int = 0) -> None:
+        call("rplsh_resources_sync_default_stream", self._h, device)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            call("rplsh_resources_destroy", self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """hipMalloc-backed row-major buffer (the role of DeviceTensor,
+    include/device_tensor.h:8-186), owned unless wrapping a foreign pointer."""
+
+    def __init__(self, shape, dtype, device: int = 0, ptr: int | None = None):
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape)) if not isinstance(shape, tuple) else tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.device = device
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self._own = ptr is None
+        if ptr is None:
+            p = C.c_void_p()
+            call("rplsh_malloc", device, self.nbytes, 0, C.byref(p))
+            self.ptr = p.value or 0
+        else:
+            self.ptr = ptr
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, device: int = 0, stream: int = 0) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype, device)
+        d.copy_from(a, stream)
+        return d
+
+    def copy_from(self, a: np.ndarray, stream: int = 0) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, "size mismatch"
+        call("rplsh_memcpy_async", self.ptr, a.ctypes.data, self.nbytes, stream)
+        call("rplsh_stream_sync", stream)
+
+    def to_host(self, stream: int = 0) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        call("rplsh_memcpy_async", out.ctypes.data, self.ptr, self.nbytes, stream)
+        call("rplsh_stream_sync", stream)
+        return out
+
+    def free(self) -> None:
+        if self._own and self.ptr:
+            call("rplsh_free", self.ptr)
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    """hipEvent wrapper for timing on a given stream."""
+
+    def __init__(self):
+        load()
+        self._h = C.c_void_p()
+        call("rplsh_event_create", C.byref(self._h))
+
+    def record(self, stream: int = 0) -> None:
+        call("rplsh_event_record", self._h, stream)
+
+    def synchronize(self) -> None:
+        call("rplsh_event_sync", self._h)
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        call("rplsh_event_elapsed_ms", self._h, end._h, C.byref(ms))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self._h.value:
+                call("rplsh_event_destroy", self._h)
+        except Exception:
+            pass
+
+
+@dataclass
+class IndexBFConfig:
+    """IndexConfig + IndexBFConfig (include/index.h:8-16, include/index_bf.h:8-19).
+
+    precision: "fp32" (plain path), "bf16" or "fp8" (precision forms).
+    kprime_factor: over-fetch k' = kprime_factor * k for the precision forms.
+    """
+
+    device: int = 0
+    memory_space: int = 0
+    store_transposed: bool = False
+    compute_exact_distances: bool = False
+    precision: str | int = "fp32"
+    kprime_factor: int = 4
+    fp64_boundary: bool = True
+
+    def to_c(self) -> _lib.BfConfig:
+        c = _lib.BfConfig()
+        c.device = self.device
+        c.memory_space = self.memory_space
+        c.store_transposed = int(self.store_transposed)
+        c.compute_exact_distances = int(self.compute_exact_distances)
+        c.precision = _PREC[self.precision]
+        c.kprime_factor = self.kprime_factor
+        c.fp64_boundary = int(self.fp64_boundary)
+        return c
+
+
+class IndexBF:
+    """IndexBF (include/index_bf.h:21-46): exact brute-force L2 kNN.
+
+    search() returns exact squared L2 distances, ascending, ties broken by the
+    lower index; k > n pads with -1 / FLT_MAX (include/index.h:42-44).
+    """
+
+    def __init__(self, resources: DeviceResources, dim: int, config: IndexBFConfig | None = None):
+        self.config = config or IndexBFConfig()
+        self.resources = resources
+        self._h = C.c_void_p()
+        cfg = self.config.to_c()
+        call("rplsh_bf_create", resources.handle, int(dim), C.byref(cfg), C.byref(self._h))
+        self.dim = int(dim)
+
+    # Index::train / add / search / reset (include/index.h:29-55)
+    def train(self, x) -> None:
+        n = 0 if x is None else int(x.shape[0])
+        call("rplsh_bf_train", self._h, n, _ptr(x) if n else None)
+
+    def add(self, x) -> None:
+        if isinstance(x, np.ndarray):
+            x = np.ascontiguousarray(x, dtype=np.float32)
+        n = int(x.shape[0])
+        if n and x.shape[1] != self.dim:
+            raise ValueError(f"expected dim {self.dim}, got {x.shape[1]}")
+        call("rplsh_bf_add", self._h, n, _ptr(x) if n else None)
+
+    def search(self, q, k: int, indices=None, distances=None):
+        """Returns (indices int32 [nq,k], distances float32 [nq,k]).
+
+        q and the optional outputs may be numpy arrays (host) or DeviceArrays."""
+        if isinstance(q, np.ndarray):
+            q = np.ascontiguousarray(q, dtype=np.float32)
+        nq = int(q.shape[0])
+        if indices is None:
+            indices = np.empty((nq, k), np.int32)
+        if distances is None:
+            distances = np.empty((nq, k), np.float32)
+        if nq:
+            call("rplsh_bf_search", self._h, nq, _ptr(q), int(k), _ptr(indices), _ptr(distances))
+        return indices, distances
+
+    def reset(self) -> None:
+        call("rplsh_bf_reset", self._h)
+
+    def set_precision(self, precision, kprime_factor: int = 0) -> None:
+        call("rplsh_bf_set_precision", self._h, _PREC[precision], int(kprime_factor))
+
+    @property
+    def num_data(self) -> int:
+        n = C.c_int64()
+        call("rplsh_bf_num_data", self._h, C.byref(n))
+        return n.value
+
+    getNumData = num_data
+
+    def get_dimension(self) -> int:
+        return self.dim
+
+    def get_device(self) -> int:
+        return self.config.device
+
+    def data_pointers(self) -> tuple[int, int]:
+        x, n = C.c_void_p(), C.c_void_p()
+        call("rplsh_bf_data_dev", self._h, C.byref(x), C.byref(n))
+        return x.value or 0, n.value or 0
+
+    def last_stats(self) -> dict:
+        s = _lib.SearchStats()
+        call("rplsh_bf_last_stats", self._h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            call("rplsh_bf_destroy", self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def compute_l2_norm(x: DeviceArray, out: DeviceArray, squared: bool = True, stream: int = 0) -> None:
+    """computeL2Norm (include/internal/norm.h:7-8) on device arrays."""
+    n, d = x.shape
+    call("rplsh_l2norm_dev", x.ptr, n, d, out.ptr, int(squared), stream)
+    call("rplsh_stream_sync", stream)
+
+
+def search_l2_distance(bases: DeviceArray, bases_norm: DeviceArray | None, queries: DeviceArray,
+                       k: int, indices: DeviceArray, distances: DeviceArray,
+                       exact: bool = True, stream: int = 0) -> None:
+    """searchL2Distance (include/internal/search.h:8-11), plain FP32 path."""
+    n, d = bases.shape if bases is not None else (0, queries.shape[1])
+    nq = queries.shape[0]
+    call("rplsh_search_l2_dev", bases.ptr if bases is not None else None, n, d,
+         bases_norm.ptr if bases_norm is not None else None, queries.ptr, nq, int(k),
+         indices.ptr, distances.ptr, int(exact), stream)
+
+
+def encode_bf16(x: DeviceArray, ld: int, out: DeviceArray, stream: int = 0) -> None:
+    n, d = x.shape
+    call("rplsh_encode_bf16_dev", x.ptr, n, d, ld, out.ptr, stream)
+    call("rplsh_stream_sync", stream)
+
+
+def encode_e4m3(x: DeviceArray, ld: int, out: DeviceArray, scales: DeviceArray, stream: int = 0) -> None:
+    n, d = x.shape
+    call("rplsh_encode_e4m3_dev", x.ptr, n, d, ld, out.ptr, scales.ptr, stream)
+    call("rplsh_stream_sync", stream)
+
+
+# ----------------------------------------------------------- synthetic data
+def gen_uniform(n: int, d: int, seed: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+    out = np.empty((n, d), np.float32)
+    call("rplsh_gen_uniform", out.ctypes.data, n, d, seed, lo, hi)
+    return out
+
+
+def gen_gaussian(n: int, d: int, seed: int, unit_normalize: bool = False) -> np.ndarray:
+    out = np.empty((n, d), np.float32)
+    call("rplsh_gen_gaussian", out.ctypes.data, n, d, seed, int(unit_normalize))
+    return out
+
+
+def gen_mixture(n: int, d: int, centers: int, sigma: float, center_seed: int, point_seed: int,
+                dup_fraction: float = 0.0) -> np.ndarray:
+    out = np.empty((n, d), np.float32)
+    call("rplsh_gen_mixture", out.ctypes.data, n, d, centers, sigma, center_seed, point_seed,
+         dup_fraction)
+    return out

@@ -1,0 +1,168 @@
+"""ctypes binding of libcurplsh_amd.so (the C ABI declared in include/rplsh.h).
+
+The product path is the HIP library; this module only loads it and declares
+the signatures. There is no CPU fallback: if the library is missing the import
+of any compute entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcurplsh_amd.so")
+
+OK = 0
+STATUS_NAMES = {
+    0: "OK",
+    1: "INVALID_ARG",
+    2: "HIP",
+    3: "OUT_OF_MEMORY",
+    4: "STATE",
+    5: "UNSUPPORTED",
+    6: "BUDGET",
+    7: "NO_DEVICE",
+}
+
+PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
+
+
+class RplshError(RuntimeError):
+    """A C-ABI call returned a non-zero rplsh_status."""
+
+    def __init__(self, status: int, func: str, msg: str):
+        self.status = status
+        self.func = func
+        super().__init__(f"{func}: {STATUS_NAMES.get(status, status)}: {msg}")
+
+
+class BfConfig(C.Structure):
+    """rplsh_bf_config: IndexConfig + IndexBFConfig (include/index.h:8-16,
+    include/index_bf.h:8-19) plus the precision-form knobs."""
+
+    _fields_ = [
+        ("device", C.c_int),
+        ("memory_space", C.c_int),
+        ("store_transposed", C.c_int),
+        ("compute_exact_distances", C.c_int),
+        ("precision", C.c_int),
+        ("kprime_factor", C.c_int),
+        ("fp64_boundary", C.c_int),
+    ]
+
+
+class SearchStats(C.Structure):
+    _fields_ = [
+        ("queries", C.c_int64),
+        ("candidates", C.c_int64),
+        ("survivors", C.c_int64),
+        ("levels", C.c_int32),
+        ("retried_queries", C.c_int32),
+        ("boundary_fp64", C.c_int32),
+        ("kprime", C.c_int32),
+        ("ms_total", C.c_float),
+    ]
+
+
+_vp = C.c_void_p
+_i32 = C.c_int
+_i64 = C.c_int64
+_u64 = C.c_uint64
+_f32 = C.c_float
+_pf = C.POINTER(C.c_float)
+
+# name -> argtypes (restype is always c_int status unless listed in _RESTYPES)
+_SIGS = {
+    "rplsh_last_error": [],
+    "rplsh_version": [],
+    "rplsh_device_count": [C.POINTER(_i32)],
+    "rplsh_get_device": [C.POINTER(_i32)],
+    "rplsh_set_device": [_i32],
+    "rplsh_pointer_device": [_vp, C.POINTER(_i32)],
+    "rplsh_resources_create": [C.POINTER(_vp)],
+    "rplsh_resources_destroy": [_vp],
+    "rplsh_resources_init_device": [_vp, _i32],
+    "rplsh_resources_default_stream": [_vp, _i32, C.POINTER(_vp)],
+    "rplsh_resources_set_default_stream": [_vp, _i32, _vp],
+    "rplsh_resources_alternate_streams": [_vp, _i32, C.POINTER(_vp), C.POINTER(_i32)],
+    "rplsh_resources_async_copy_stream": [_vp, _i32, C.POINTER(_vp)],
+    "rplsh_resources_sync_default_stream": [_vp, _i32],
+    "rplsh_malloc": [_i32, C.c_size_t, _i32, C.POINTER(_vp)],
+    "rplsh_free": [_vp],
+    "rplsh_memcpy_async": [_vp, _vp, C.c_size_t, _vp],
+    "rplsh_memset_async": [_vp, _i32, C.c_size_t, _vp],
+    "rplsh_stream_sync": [_vp],
+    "rplsh_device_sync": [],
+    "rplsh_host_alloc": [C.c_size_t, C.POINTER(_vp)],
+    "rplsh_host_free": [_vp],
+    "rplsh_mem_info": [_i32, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+    "rplsh_event_create": [C.POINTER(_vp)],
+    "rplsh_event_destroy": [_vp],
+    "rplsh_event_record": [_vp, _vp],
+    "rplsh_event_elapsed_ms": [_vp, _vp, _pf],
+    "rplsh_event_sync": [_vp],
+    "rplsh_l2norm_dev": [_vp, _i64, _i32, _vp, _i32, _vp],
+    "rplsh_encode_bf16_dev": [_vp, _i64, _i32, _i32, _vp, _vp],
+    "rplsh_encode_e4m3_dev": [_vp, _i64, _i32, _i32, _vp, _vp, _vp],
+    "rplsh_search_l2_dev": [_vp, _i64, _i32, _vp, _vp, _i64, _i32, _vp, _vp, _i32, _vp],
+    "rplsh_bf_config_default": [C.POINTER(BfConfig)],
+    "rplsh_bf_create": [_vp, _i32, C.POINTER(BfConfig), C.POINTER(_vp)],
+    "rplsh_bf_destroy": [_vp],
+    "rplsh_bf_train": [_vp, _i64, _vp],
+    "rplsh_bf_add": [_vp, _i64, _vp],
+    "rplsh_bf_search": [_vp, _i64, _vp, _i32, _vp, _vp],
+    "rplsh_bf_reset": [_vp],
+    "rplsh_bf_num_data": [_vp, C.POINTER(_i64)],
+    "rplsh_bf_dimension": [_vp, C.POINTER(_i32)],
+    "rplsh_bf_device": [_vp, C.POINTER(_i32)],
+    "rplsh_bf_set_precision": [_vp, _i32, _i32],
+    "rplsh_bf_data_dev": [_vp, C.POINTER(_vp), C.POINTER(_vp)],
+    "rplsh_bf_last_stats": [_vp, C.POINTER(SearchStats)],
+    "rplsh_gen_uniform": [_vp, _i64, _i32, _u64, _f32, _f32],
+    "rplsh_gen_gaussian": [_vp, _i64, _i32, _u64, _i32],
+    "rplsh_gen_mixture": [_vp, _i64, _i32, _i32, _f32, _u64, _u64, C.c_double],
+}
+
+_RESTYPES = {
+    "rplsh_last_error": C.c_char_p,
+    "rplsh_version": C.c_char_p,
+    "rplsh_bf_config_default": None,
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load() -> C.CDLL:
+    """Load the HIP library (once). Raises if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `make -C curplsh_amd` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+        for name, args in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, C.c_int)
+        _lib = lib
+        return lib
+
+
+def call(name: str, *args) -> None:
+    """Call a status-returning entry point; raise RplshError on failure."""
+    lib = load()
+    st = getattr(lib, name)(*args)
+    if st != OK:
+        msg = lib.rplsh_last_error()
+        raise RplshError(st, name, msg.decode() if msg else "")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    call("rplsh_device_count", C.byref(n))
+    return n.value

@@ -1,0 +1,105 @@
+// Device-side shared definitions: candidate encoding, MFMA vector types and
+// the host-visible launch-argument structs of the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace rplsh {
+
+// A candidate is one 64-bit word: the order-preserving bit pattern of its
+// float key in the high half and its row index in the low half. Comparing
+// words as unsigned integers is the (key, index) lexicographic order the
+// search defines for ties ("lower index first").
+__host__ __device__ inline uint32_t key_bits(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__host__ __device__ inline float key_from_bits(uint32_t u) {
+  u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+__host__ __device__ inline uint64_t pack_cand(float key, int32_t idx) {
+  return ((uint64_t)key_bits(key) << 32) | (uint32_t)idx;
+}
+
+__host__ __device__ inline float cand_key(uint64_t c) { return key_from_bits((uint32_t)(c >> 32)); }
+__host__ __device__ inline int32_t cand_idx(uint64_t c) { return (int32_t)(uint32_t)(c & 0xffffffffu); }
+
+constexpr uint64_t kEmptyCand = ~0ull;
+
+// Per-query status bits written by the select / re-rank kernels.
+enum StatusBits : int {
+  kStatusOverflow = 1,  // more survivors than the candidate capacity
+  kStatusShort = 2,     // fewer survivors than the rank asked for
+  kStatusCertFail = 4,  // plain path: candidate margin not certified
+};
+
+// Arguments of the fused contraction + threshold filter.
+struct FilterArgs {
+  const void* x;          // database rows, element type of the kernel
+  int64_t ldx;            // row stride in elements
+  int dvalid_bytes;       // bytes of a row holding data; the rest of the row is zero
+  const float* xnorm;     // [n] squared norms (FP32, canonical)
+  const float* xscale;    // [n] per-row scale (FP8 only)
+  int64_t m;              // logical rows of this pass
+  int64_t stride;         // physical row = logical row * stride
+  int64_t rows_per_slice; // logical rows per workgroup slice (multiple of 64)
+  const void* q;          // encoded queries [nq_pad][row bytes], zero padded
+  const float* qscale;    // [nq] per-row scale (FP8 only)
+  const float* tau;       // [nq] threshold on the approximate relative distance
+  int nq;
+  int num_qtiles;
+  int num_slices;
+  int cap;                // candidate slots per query
+  int* counts;            // [nq] survivors (may exceed cap: overflow)
+  uint64_t* cand;         // [nq][cap]
+};
+
+struct SelectArgs {
+  const uint64_t* cand;  // [nq][cap]
+  const int* counts;     // [nq]
+  int cap;
+  int nq;
+  int K;                 // rank (threshold mode) or list length (top-K mode)
+  int topk_mode;         // 0: tau_out = key of K-th smallest; 1: sorted K smallest
+  const float* tau_in;   // [nq] threshold the candidates were filtered with
+  float* tau_out;        // [nq]
+  uint64_t* topk_out;    // [nq][K]
+  float* bound_out;      // [nq] key every non-selected row is >= to
+  int* status;           // [nq]
+};
+
+struct RerankArgs {
+  const float* x;        // FP32 database [n][ldx]
+  int64_t ldx;
+  const float* xnorm;    // [n]
+  int d;
+  const float* q;        // FP32 queries [nq][ldq]
+  int64_t ldq;
+  const float* qnorm;    // [nq]
+  const uint64_t* topk;  // [nq][kp] approximate candidates
+  int kp;
+  int k;
+  int nq;
+  int exact;             // add ||q||^2
+  const float* bound;    // [nq] certification bound (plain path) or nullptr
+  float eps_coeff;       // plain: certified margin = eps_coeff * (||q||^2 + 2 xn_max)
+  float band_coeff;      // FP64 band half-width = band_coeff * (||q||^2 + 2 xn_max) + ulp terms
+  float xn_max;
+  int fp64_boundary;
+  int32_t* out_idx;      // [nq][k]
+  float* out_dist;       // [nq][k]
+  int* status;           // [nq]
+  float* relk_out;       // [nq] k-th canonical relative distance (for a retry)
+  int* fp64_count;       // total boundary candidates re-ranked in FP64
+};
+
+}  // namespace rplsh

@@ -1,0 +1,391 @@
+// Fused candidate contraction + threshold filter (the hot kernel).
+//
+// Replaces, in one pass over the database, the reference's per-tile
+// cublasSgemm (src/internal/matrix_ops.cu:56-58 via search.cu:176-178), the
+// fused base-norm add + k-select (select.cu:189-224) and its global distance
+// tile: the Bq x Bn distance matrix never exists in HBM.
+//
+// Work decomposition
+//   workgroup  = 4 waves; 4*TN*32 queries (TN query tiles of 32 per wave) x one
+//                slice of database rows. Query fragments stay in registers for
+//                the whole slice; the slice streams through LDS in chunks of 64
+//                rows (double buffered, one barrier per chunk).
+//   MFMA       = 32x32 output tiles, database rows on M (A operand), queries on
+//                N (B operand): lane l owns query column l%32 and 16 rows.
+//   accumulate = the first MFMA of every tile takes C = -||x||^2/2 of its rows,
+//                so the accumulator ends as  q.x - ||x||^2/2  and the
+//                approximate relative distance is  key = -2 * acc.
+//   filter     = key <= tau_q  <=>  acc >= -tau_q/2: one max-reduction of the
+//                lane's 16 values per tile and a compare; survivors (rare) are
+//                appended to the query's candidate list with a global atomic.
+//   grid       = (query tiles x slices), renumbered so consecutive workgroups
+//                on one XCD (hardware dispatch is round-robin over the 8 XCDs)
+//                take all query tiles of the same slice: the slice is read
+//                from HBM once per XCD and re-served from that XCD's L2.
+//
+// Element kinds
+//   F32  : v_mfma_f32_32x32x2_f32, 4 per 32-byte chunk (lane half h supplies
+//          elements 8c+4h..8c+4h+3 of chunk c; both operands use the same
+//          assignment, so the contraction is unchanged).
+//   BF16 : v_mfma_f32_32x32x16_bf16, one per 32-byte chunk.
+//   FP8  : v_mfma_scale_f32_32x32x64_f8f6f4 (e4m3), one per 64-byte chunk; the
+//          per-row power-of-two scales go in as E8M0 scale operands, so the
+//          accumulator is already in real units.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace rplsh {
+
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kBlock = 256;
+constexpr int kRowsPerChunk = 64;
+
+template <Elem E>
+struct ElemTraits;
+template <>
+struct ElemTraits<Elem::F32> {
+  static constexpr int kBytes = 4;
+  static constexpr int kLaneBytes = 16;
+};
+template <>
+struct ElemTraits<Elem::BF16> {
+  static constexpr int kBytes = 2;
+  static constexpr int kLaneBytes = 16;
+};
+template <>
+struct ElemTraits<Elem::FP8> {
+  static constexpr int kBytes = 1;
+  static constexpr int kLaneBytes = 32;
+};
+
+template <int LB>
+struct Frag;
+template <>
+struct Frag<16> {
+  i32x4 v;
+};
+template <>
+struct Frag<32> {
+  i32x4 v0, v1;
+};
+
+template <int LB>
+__device__ inline Frag<LB> load_frag(const char* p);
+template <>
+__device__ inline Frag<16> load_frag<16>(const char* p) {
+  Frag<16> f;
+  f.v = *reinterpret_cast<const i32x4*>(p);
+  return f;
+}
+template <>
+__device__ inline Frag<32> load_frag<32>(const char* p) {
+  Frag<32> f;
+  f.v0 = *reinterpret_cast<const i32x4*>(p);
+  f.v1 = *reinterpret_cast<const i32x4*>(p + 16);
+  return f;
+}
+
+template <int LB>
+__device__ inline Frag<LB> zero_frag() {
+  Frag<LB> f;
+  __builtin_memset(&f, 0, sizeof(f));
+  return f;
+}
+
+// acc += A(rows) * B(queries) over one K-chunk.
+template <Elem E>
+__device__ inline f32x16 mfma_chunk(const Frag<ElemTraits<E>::kLaneBytes>& a,
+                                    const Frag<ElemTraits<E>::kLaneBytes>& b, f32x16 c, int sa,
+                                    int sb);
+
+template <>
+__device__ inline f32x16 mfma_chunk<Elem::F32>(const Frag<16>& a, const Frag<16>& b, f32x16 c,
+                                               int, int) {
+  const f32x4 av = __builtin_bit_cast(f32x4, a.v);
+  const f32x4 bv = __builtin_bit_cast(f32x4, b.v);
+  c = __builtin_amdgcn_mfma_f32_32x32x2f32(av[0], bv[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x2f32(av[1], bv[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x2f32(av[2], bv[2], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x2f32(av[3], bv[3], c, 0, 0, 0);
+  return c;
+}
+
+template <>
+__device__ inline f32x16 mfma_chunk<Elem::BF16>(const Frag<16>& a, const Frag<16>& b, f32x16 c,
+                                                int, int) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a.v),
+                                                  __builtin_bit_cast(bf16x8, b.v), c, 0, 0, 0);
+}
+
+template <>
+__device__ inline f32x16 mfma_chunk<Elem::FP8>(const Frag<32>& a, const Frag<32>& b, f32x16 c,
+                                               int sa, int sb) {
+  i32x8 av, bv;
+  av.lo = a.v0;
+  av.hi = a.v1;
+  bv.lo = b.v0;
+  bv.hi = b.v1;
+  // cbsz = blgp = 0: both operands e4m3; byte 0 of sa/sb is the E8M0 scale.
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, c, 0, 0, 0, sa, 0, sb);
+}
+
+// E8M0 exponent byte of a power-of-two scale.
+__device__ inline int e8m0_of(float s) { return (int)((__float_as_uint(s) >> 23) & 0xff); }
+
+template <Elem E, int DPAD, int TN>
+__global__ __launch_bounds__(kBlock, 2) void filter_kernel(FilterArgs a) {
+  using T = ElemTraits<E>;
+  constexpr int LB = T::kLaneBytes;
+  constexpr int CB = 2 * LB;                    // bytes per MFMA K-chunk of a row
+  constexpr int ROWB = DPAD * T::kBytes;        // bytes per row
+  constexpr int NC = ROWB / CB;                 // K-chunks per row
+  constexpr int RS = ROWB + 16;                 // LDS row stride (bank spread)
+  constexpr int BM = kRowsPerChunk;
+  constexpr int BQ = 4 * TN * 32;
+  constexpr int UNITS = BM * ROWB / 16;         // 16-byte units per chunk
+  constexpr int UPT = (UNITS + kBlock - 1) / kBlock;
+  constexpr int UROW = ROWB / 16;               // units per row
+  static_assert(ROWB % CB == 0, "row must be whole K-chunks");
+
+  __shared__ __attribute__((aligned(16))) char sx[2][BM * RS];
+  __shared__ __attribute__((aligned(16))) float sn[2][BM];
+  __shared__ int se[2][BM];
+
+  // XCD-aware renumbering (see header).
+  const int per_xcd = (int)(gridDim.x >> 3);
+  const int logical = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (logical >= a.num_qtiles * a.num_slices) return;
+  const int qtile = logical % a.num_qtiles;
+  const int slice = logical / a.num_qtiles;
+
+  const int64_t r_begin = (int64_t)slice * a.rows_per_slice;
+  int64_t r_end = r_begin + a.rows_per_slice;
+  if (r_end > a.m) r_end = a.m;
+  if (r_begin >= r_end) return;  // whole workgroup leaves together
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int h = lane >> 5, col = lane & 31;
+
+  // Query fragments and thresholds of this lane's TN query columns.
+  Frag<LB> qf[TN][NC];
+  float thr[TN];
+  int qsc[TN];
+  int qrow[TN];
+#pragma unroll
+  for (int nt = 0; nt < TN; ++nt) {
+    const int qr = qtile * BQ + (wave * TN + nt) * 32 + col;
+    qrow[nt] = qr;
+    if (qr < a.nq) {
+      const char* qp = reinterpret_cast<const char*>(a.q) + (int64_t)qr * ROWB;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) qf[nt][c] = load_frag<LB>(qp + CB * c + LB * h);
+      thr[nt] = fmaxf(-0.5f * a.tau[qr], -FLT_MAX);
+      qsc[nt] = E == Elem::FP8 ? e8m0_of(a.qscale[qr]) : 127;
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) qf[nt][c] = zero_frag<LB>();
+      thr[nt] = INFINITY;
+      qsc[nt] = 127;
+    }
+  }
+
+  const char* xb = reinterpret_cast<const char*>(a.x);
+  const int64_t row_bytes_g = a.ldx * T::kBytes;
+  i32x4 pre[UPT];
+  float pn = 0.0f;
+  int pe = 127;
+
+  auto gload = [&](int64_t c0) {
+#pragma unroll
+    for (int t = 0; t < UPT; ++t) {
+      const int u = tid + kBlock * t;
+      i32x4 v = {0, 0, 0, 0};
+      if (UNITS % kBlock == 0 || u < UNITS) {
+        const int r = u / UROW;
+        const int cb = (u - r * UROW) * 16;
+        const int64_t j = c0 + r;
+        if (j < r_end && cb < a.dvalid_bytes)
+          v = *reinterpret_cast<const i32x4*>(xb + j * a.stride * row_bytes_g + cb);
+      }
+      pre[t] = v;
+    }
+    if (tid < BM) {
+      const int64_t j = c0 + tid;
+      if (j < r_end) {
+        pn = -0.5f * a.xnorm[j * a.stride];
+        if (E == Elem::FP8) pe = e8m0_of(a.xscale[j * a.stride]);
+      } else {
+        pn = -INFINITY;
+        pe = 127;
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < UPT; ++t) {
+      const int u = tid + kBlock * t;
+      if (UNITS % kBlock == 0 || u < UNITS) {
+        const int r = u / UROW;
+        const int cb = (u - r * UROW) * 16;
+        *reinterpret_cast<i32x4*>(&sx[buf][r * RS + cb]) = pre[t];
+      }
+    }
+    if (tid < BM) {
+      sn[buf][tid] = pn;
+      se[buf][tid] = pe;
+    }
+  };
+
+  int64_t c0 = r_begin;
+  gload(c0);
+  sstore(0);
+  __syncthreads();
+  int buf = 0;
+
+  for (; c0 < r_end; c0 += BM) {
+    const bool has_next = c0 + BM < r_end;
+    if (has_next) gload(c0 + BM);
+
+    f32x16 acc[2][TN];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      // C-init rows of this lane: mt*32 + 8*(i/4) + 4*h + (i%4), i = 0..15.
+      f32x16 ci;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(&sn[buf][mt * 32 + 8 * j + 4 * h]);
+        ci[4 * j + 0] = v[0];
+        ci[4 * j + 1] = v[1];
+        ci[4 * j + 2] = v[2];
+        ci[4 * j + 3] = v[3];
+      }
+      const int xsc = E == Elem::FP8 ? se[buf][mt * 32 + col] : 127;
+      const char* ap = &sx[buf][(mt * 32 + col) * RS + LB * h];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const Frag<LB> af = load_frag<LB>(ap + CB * c);
+#pragma unroll
+        for (int nt = 0; nt < TN; ++nt)
+          acc[mt][nt] = mfma_chunk<E>(af, qf[nt][c], c == 0 ? ci : acc[mt][nt], xsc, qsc[nt]);
+      }
+    }
+
+    // Filter: survivors are rows with acc >= thr (key = -2 acc <= tau).
+    float mx[2][TN];
+    bool hit = false;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < TN; ++nt) {
+        float m = acc[mt][nt][0];
+#pragma unroll
+        for (int i = 1; i < 16; ++i) m = fmaxf(m, acc[mt][nt][i]);
+        mx[mt][nt] = m;
+        hit |= m >= thr[nt];
+      }
+    if (__any(hit)) {
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < TN; ++nt) {
+          if (mx[mt][nt] >= thr[nt]) {
+            const int qr = qrow[nt];
+            int* cnt = a.counts + qr;
+            uint64_t* dst = a.cand + (int64_t)qr * a.cap;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+              const float v = acc[mt][nt][i];
+              if (v >= thr[nt]) {
+                const int64_t row = c0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+                const int pos = atomicAdd(cnt, 1);
+                if (pos < a.cap) dst[pos] = pack_cand(-2.0f * v, (int32_t)(row * a.stride));
+              }
+            }
+          }
+        }
+    }
+
+    if (has_next) sstore(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+}
+
+template <Elem E, int DPAD>
+constexpr int tn_for() {
+  return DPAD * ElemTraits<E>::kBytes <= 384 ? 2 : 1;
+}
+
+template <Elem E, int DPAD>
+void launch_one(const FilterArgs& a, hipStream_t st) {
+  constexpr int TN = tn_for<E, DPAD>();
+  const int total = a.num_qtiles * a.num_slices;
+  const unsigned grid = (unsigned)round_up(total, 8);
+  filter_kernel<E, DPAD, TN><<<grid, kBlock, 0, st>>>(a);
+  RPLSH_HIP(hipGetLastError());
+}
+
+template <Elem E>
+void launch_elem(int dpad, const FilterArgs& a, hipStream_t st) {
+  switch (dpad) {
+    case 64: return launch_one<E, 64>(a, st);
+    case 128: return launch_one<E, 128>(a, st);
+    case 256: return launch_one<E, 256>(a, st);
+    default: break;
+  }
+  if constexpr (E != Elem::FP8) {
+    switch (dpad) {
+      case 32: return launch_one<E, 32>(a, st);
+      case 96: return launch_one<E, 96>(a, st);
+      case 160: return launch_one<E, 160>(a, st);
+      case 192: return launch_one<E, 192>(a, st);
+      default: break;
+    }
+  }
+  fail(RPLSH_ERR_UNSUPPORTED, "filter: padded dimension %d not instantiated", dpad);
+}
+
+}  // namespace
+
+int filter_dpad(Elem e, int d) {
+  if (d <= 0 || d > 256) return 0;
+  if (e == Elem::FP8) {
+    // 64-element K-chunks.
+    if (d <= 64) return 64;
+    if (d <= 128) return 128;
+    return 256;
+  }
+  static const int sizes[] = {32, 64, 96, 128, 160, 192, 256};
+  for (int s : sizes)
+    if (d <= s) return s;
+  return 0;
+}
+
+int filter_block_queries(Elem e, int dpad) {
+  const int bytes = e == Elem::F32 ? 4 : (e == Elem::BF16 ? 2 : 1);
+  return 4 * 32 * (dpad * bytes <= 384 ? 2 : 1);
+}
+
+void launch_filter(Elem e, int dpad, const FilterArgs& a, hipStream_t st) {
+  switch (e) {
+    case Elem::F32: return launch_elem<Elem::F32>(dpad, a, st);
+    case Elem::BF16: return launch_elem<Elem::BF16>(dpad, a, st);
+    case Elem::FP8: return launch_elem<Elem::FP8>(dpad, a, st);
+  }
+}
+
+}  // namespace rplsh

@@ -1,0 +1,50 @@
+// Host launchers of the device kernels (internal C++ interface between the
+// translation units of libcurplsh_amd; nothing here crosses the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device.hpp"
+
+namespace rplsh {
+
+// norm.hip
+void launch_l2norm(const float* x, int64_t n, int d, int64_t ldx, float* out, int squared,
+                   hipStream_t st);
+void launch_encode_bf16(const float* x, int64_t n, int d, int64_t ldx, int ld, uint16_t* out,
+                        hipStream_t st);
+void launch_encode_e4m3(const float* x, int64_t n, int d, int64_t ldx, int ld, uint8_t* out,
+                        float* scales, hipStream_t st);
+void launch_fill_f32(float* p, int64_t n, float v, hipStream_t st);
+void launch_fill_i32(int* p, int64_t n, int v, hipStream_t st);
+void launch_pad_rows(const float* x, int64_t n, int d, int64_t ldx, int ld, float* out,
+                     hipStream_t st);
+// out must be zero (or a smaller non-negative value) before the call.
+void launch_max_reduce(const float* v, int64_t n, float* out, hipStream_t st);
+
+// filter.hip: element kinds of the candidate contraction.
+enum class Elem { F32 = 0, BF16 = 1, FP8 = 2 };
+
+// Padded row length (elements) the filter kernels use for dimension d, or 0
+// if d is not supported.
+int filter_dpad(Elem e, int d);
+// Queries per workgroup of the filter kernel for (e, dpad).
+int filter_block_queries(Elem e, int dpad);
+void launch_filter(Elem e, int dpad, const FilterArgs& a, hipStream_t st);
+
+// select.hip
+void launch_select(const SelectArgs& a, hipStream_t st);
+void launch_rerank(const RerankArgs& a, hipStream_t st);
+// Compacts the ids of queries with a non-zero status into list, count in *n.
+void launch_collect_failed(const int* status, int nq, int* list, int* n, hipStream_t st);
+// Sums counts[0..nq) into *out (int64).
+void launch_sum_counts(const int* counts, int nq, int cap, long long* out, hipStream_t st);
+// Row gather / scatter for re-running a subset of queries.
+void launch_gather_rows(const float* src, int64_t ld, const int* rows, int nrows, int d,
+                        float* dst, hipStream_t st);
+void launch_scatter_results(const int32_t* si, const float* sd, const int* rows, int nrows, int k,
+                            int32_t* di, float* dd, hipStream_t st);
+
+}  // namespace rplsh

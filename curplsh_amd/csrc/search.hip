@@ -1,0 +1,734 @@
+// Search orchestration and the IndexBF / searchL2Distance entry points.
+//
+// Reference shape (src/internal/search.cu:79-216): query norms, tiled
+// cuBLAS GEMMs into a global distance buffer, a select per tile, +||q||^2,
+// index offsets and a cross-tile merge -- about 260 launches and ~1 GiB of
+// temporaries per call at 1M x 128, 10k queries.
+//
+// Here a search is a short, fixed pipeline per query batch:
+//   1. canonical FP32 query norms; queries encoded for the candidate
+//      contraction (FP32 padded copy, BF16, or FP8 e4m3 + power-of-two scale);
+//   2. filter levels: the fused MFMA filter runs over strided samples of the
+//      database with growing density and then over all rows; after each
+//      sample level a radix select turns the rho-th smallest survivor into the
+//      next level's per-query threshold tau. The last level keeps every row
+//      with approximate relative distance <= tau (about 8 k' rows per query)
+//      and selects the k' best by (approximate key, index). This is exact
+//      whenever the last level keeps between k' and the capacity rows: a row
+//      the filter drops has key > tau >= every kept key;
+//   3. re-rank: canonical FP32 distances of the k' candidates, top k;
+//      precision forms re-order the boundary band in FP64; the plain path
+//      certifies that the MFMA error margin cannot hide a better row;
+//   4. the rare queries whose last level kept too few / too many rows, or
+//      whose certificate failed, are gathered and re-run with a wider margin.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "kernels.hpp"
+#include "rplsh.h"
+
+namespace rplsh {
+
+namespace {
+
+constexpr float kUnitRoundoff = 5.9604645e-08f;  // 2^-24
+
+// Growable device buffer (hipMalloc only when a call needs more than before).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void* ensure(size_t b) {
+    if (b <= bytes) return p;
+    if (p) RPLSH_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    size_t nb = std::max(b, (size_t)256);
+    RPLSH_HIP(hipMalloc(&p, nb));
+    bytes = nb;
+    return p;
+  }
+  template <typename T>
+  T* as(size_t count) {
+    return static_cast<T*>(ensure(count * sizeof(T)));
+  }
+};
+
+struct Workspace {
+  DevBuf qnorm, qenc, qscale, qpad, tau, tau2, counts, cand, topk, bound, status, relk;
+  DevBuf flist, fcount, fp64, surv;
+  DevBuf qsub, isub, dsub;  // a retried subset of queries
+};
+
+}  // namespace
+
+// The database as the search kernels see it.
+struct DbView {
+  const float* x = nullptr;  // FP32 rows [n][ldx]
+  int64_t ldx = 0;
+  int64_t n = 0;
+  int d = 0;
+  const float* xnorm = nullptr;
+  float xn_max = 0.0f;
+  Elem elem = Elem::F32;
+  int dpad = 0;
+  const void* xenc = nullptr;  // candidate rows [n][ldenc] of elem
+  int64_t ldenc = 0;
+  int dvalid_bytes = 0;
+  const float* xscale = nullptr;  // FP8 per-row scales
+};
+
+struct SearchParams {
+  int k = 0;
+  int kp = 0;          // candidates re-ranked per query
+  int exact = 1;
+  int fp64_boundary = 0;
+  bool certify = false;
+  int c_factor = 8;    // last-level target survivors = c_factor * kp
+};
+
+struct Level {
+  int64_t stride;
+  int64_t m;
+  int rank_next;  // rho of the threshold for the next level; 0 on the last
+};
+
+// Filter levels for n rows and k' candidates (see header).
+static std::vector<Level> plan_levels(int64_t n, int kp, int c_factor) {
+  const int64_t kM0 = 1024;   // rows of the first (unfiltered) sample
+  const int kRhoFinal = 8;    // rank of the last threshold
+  const int kRhoMid = 16;     // rank of intermediate thresholds
+  const double kGrowth = 16;  // density step between sample levels
+  std::vector<Level> L;
+  const int64_t s0 = std::max<int64_t>(1, n / kM0);
+  if (s0 == 1) {
+    L.push_back({1, n, 0});
+    return L;
+  }
+  const int64_t target = (int64_t)c_factor * kp;
+  const int64_t s_pf = std::max<int64_t>(1, target / kRhoFinal);
+  std::vector<int64_t> strides;
+  if (s0 <= s_pf) {
+    strides.push_back(s0);
+  } else {
+    const int nl = (int)std::ceil(std::log((double)s0 / (double)s_pf) / std::log(kGrowth));
+    for (int i = 0; i <= nl; ++i) {
+      double s = (double)s0 * std::pow((double)s_pf / (double)s0, (double)i / nl);
+      int64_t si = std::max<int64_t>(1, (int64_t)std::llround(s));
+      if (strides.empty() || si < strides.back()) strides.push_back(si);
+    }
+  }
+  for (size_t i = 0; i < strides.size(); ++i) {
+    const int64_t s = strides[i];
+    if (s == 1) break;
+    const int64_t m = div_up(n, s);
+    int rank;
+    if (i + 1 < strides.size() && strides[i + 1] > 1) {
+      rank = kRhoMid;
+    } else {
+      rank = (int)std::min<int64_t>(m, div_up(target, s));
+    }
+    L.push_back({s, m, std::max(1, rank)});
+  }
+  L.push_back({1, n, 0});
+  return L;
+}
+
+static int64_t level_cap(const std::vector<Level>& L, int kp, int c_factor) {
+  int64_t cap = 4096;
+  cap = std::max<int64_t>(cap, L.front().stride > 1 ? L.front().m : 0);
+  cap = std::max<int64_t>(cap, 4 * (int64_t)c_factor * kp);
+  if (L.size() == 1) cap = std::max<int64_t>(cap, L.front().m);
+  return cap;
+}
+
+static void run_filter(const DbView& db, const Level& lv, const void* qenc, const float* qscale,
+                       const float* tau, int nq, int cap, int* counts, uint64_t* cand,
+                       hipStream_t st) {
+  FilterArgs fa{};
+  fa.x = db.xenc;
+  fa.ldx = db.ldenc;
+  fa.dvalid_bytes = db.dvalid_bytes;
+  fa.xnorm = db.xnorm;
+  fa.xscale = db.xscale;
+  fa.m = lv.m;
+  fa.stride = lv.stride;
+  fa.q = qenc;
+  fa.qscale = qscale;
+  fa.tau = tau;
+  fa.nq = nq;
+  const int bq = filter_block_queries(db.elem, db.dpad);
+  fa.num_qtiles = (int)div_up(nq, bq);
+  const int64_t target_wgs = 4096;
+  int64_t slices = std::max<int64_t>(1, div_up(target_wgs, fa.num_qtiles));
+  slices = std::min<int64_t>(slices, div_up(lv.m, 256));
+  slices = std::max<int64_t>(slices, 1);
+  fa.rows_per_slice = round_up(div_up(lv.m, slices), 64);
+  fa.num_slices = (int)div_up(lv.m, fa.rows_per_slice);
+  fa.cap = cap;
+  fa.counts = counts;
+  fa.cand = cand;
+  launch_fill_i32(counts, nq, 0, st);
+  launch_filter(db.elem, db.dpad, fa, st);
+}
+
+// One batch of nq device-resident FP32 queries [nq][d] -> device outputs.
+static void search_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
+                         int32_t* out_i, float* out_d, std::vector<Workspace>& wss, int depth,
+                         hipStream_t st, rplsh_search_stats& stats) {
+  if (nq <= 0) return;
+  Workspace& ws = wss.at(depth);
+  const int d = db.d;
+  const int kp = (int)std::min<int64_t>(p.kp, db.n);
+  const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor);
+  const int64_t cap64 = level_cap(L, kp, p.c_factor);
+  RPLSH_REQUIRE(cap64 < (1ll << 31), "candidate capacity overflow");
+  const int cap = (int)cap64;
+
+  // 1. query norms and encoding
+  float* qn = ws.qnorm.as<float>(nq);
+  launch_l2norm(q, nq, d, d, qn, 1, st);
+  const int eb = db.elem == Elem::F32 ? 4 : (db.elem == Elem::BF16 ? 2 : 1);
+  void* qenc = ws.qenc.ensure((size_t)nq * db.dpad * eb);
+  float* qscale = nullptr;
+  switch (db.elem) {
+    case Elem::F32: launch_pad_rows(q, nq, d, d, db.dpad, static_cast<float*>(qenc), st); break;
+    case Elem::BF16: launch_encode_bf16(q, nq, d, d, db.dpad, static_cast<uint16_t*>(qenc), st); break;
+    case Elem::FP8:
+      qscale = ws.qscale.as<float>(nq);
+      launch_encode_e4m3(q, nq, d, d, db.dpad, static_cast<uint8_t*>(qenc), qscale, st);
+      break;
+  }
+
+  // 2. filter levels
+  float* tau = ws.tau.as<float>(nq);
+  float* tau2 = ws.tau2.as<float>(nq);
+  int* counts = ws.counts.as<int>(nq);
+  uint64_t* cand = ws.cand.as<uint64_t>((size_t)nq * cap);
+  uint64_t* topk = ws.topk.as<uint64_t>((size_t)nq * kp);
+  float* bound = ws.bound.as<float>(nq);
+  int* status = ws.status.as<int>(nq);
+  float* relk = ws.relk.as<float>(nq);
+  launch_fill_f32(tau, nq, INFINITY, st);
+  launch_fill_i32(status, nq, 0, st);
+  for (size_t li = 0; li < L.size(); ++li) {
+    const Level& lv = L[li];
+    run_filter(db, lv, qenc, qscale, tau, nq, cap, counts, cand, st);
+    SelectArgs sa{};
+    sa.cand = cand;
+    sa.counts = counts;
+    sa.cap = cap;
+    sa.nq = nq;
+    sa.tau_in = tau;
+    if (lv.rank_next > 0) {
+      sa.K = lv.rank_next;
+      sa.topk_mode = 0;
+      sa.tau_out = tau2;
+      sa.status = nullptr;  // a loose intermediate threshold only costs time
+      launch_select(sa, st);
+      std::swap(tau, tau2);
+    } else {
+      sa.K = kp;
+      sa.topk_mode = 1;
+      sa.topk_out = topk;
+      sa.bound_out = bound;
+      sa.status = status;
+      launch_select(sa, st);
+    }
+  }
+  stats.levels = (int32_t)L.size();
+  {
+    long long* sv = ws.surv.as<long long>(1);
+    RPLSH_HIP(hipMemsetAsync(sv, 0, sizeof(long long), st));
+    launch_sum_counts(counts, nq, cap, sv, st);
+  }
+
+  // 3. re-rank
+  int* fp64c = ws.fp64.as<int>(1);
+  RPLSH_HIP(hipMemsetAsync(fp64c, 0, sizeof(int), st));
+  RerankArgs ra{};
+  ra.x = db.x;
+  ra.ldx = db.ldx;
+  ra.xnorm = db.xnorm;
+  ra.d = d;
+  ra.q = q;
+  ra.ldq = d;
+  ra.qnorm = qn;
+  ra.topk = topk;
+  ra.kp = kp;
+  ra.k = p.k;
+  ra.nq = nq;
+  ra.exact = p.exact;
+  ra.bound = p.certify ? bound : nullptr;
+  ra.eps_coeff = 4.0f * (float)(d + 4) * kUnitRoundoff;
+  ra.band_coeff = 2.0f * (float)(d + 2) * kUnitRoundoff;
+  ra.xn_max = db.xn_max;
+  ra.fp64_boundary = p.fp64_boundary;
+  ra.out_idx = out_i;
+  ra.out_dist = out_d;
+  ra.status = status;
+  ra.relk_out = relk;
+  ra.fp64_count = fp64c;
+  launch_rerank(ra, st);
+
+  // 4. failed queries
+  int* flist = ws.flist.as<int>(nq);
+  int* fcount = ws.fcount.as<int>(1);
+  RPLSH_HIP(hipMemsetAsync(fcount, 0, sizeof(int), st));
+  launch_collect_failed(status, nq, flist, fcount, st);
+  int nfail = 0, nfp64 = 0;
+  long long nsurv = 0;
+  RPLSH_HIP(hipMemcpyAsync(&nfail, fcount, sizeof(int), hipMemcpyDeviceToHost, st));
+  RPLSH_HIP(hipMemcpyAsync(&nfp64, fp64c, sizeof(int), hipMemcpyDeviceToHost, st));
+  RPLSH_HIP(hipMemcpyAsync(&nsurv, ws.surv.p, sizeof(long long), hipMemcpyDeviceToHost, st));
+  RPLSH_HIP(hipStreamSynchronize(st));
+  stats.boundary_fp64 += nfp64;
+  if (depth == 0) {
+    stats.survivors = nsurv;
+    stats.candidates = (int64_t)nq * kp;
+    stats.kprime = kp;
+  }
+  if (nfail == 0) return;
+  if (depth + 1 >= (int)wss.size())
+    fail(RPLSH_ERR_STATE, "search: %d queries still unresolved after %d retries", nfail, depth);
+  stats.retried_queries += nfail;
+
+  // Re-run the failed subset with four times the survivor margin and, for
+  // certification failures, four times the candidates.
+  Workspace& sub = wss.at(depth + 1);
+  float* qs = sub.qsub.as<float>((size_t)nfail * d);
+  int32_t* is = sub.isub.as<int32_t>((size_t)nfail * p.k);
+  float* ds = sub.dsub.as<float>((size_t)nfail * p.k);
+  launch_gather_rows(q, d, flist, nfail, d, qs, st);
+  SearchParams p2 = p;
+  p2.c_factor = p.c_factor * 4;
+  p2.kp = (int)std::min<int64_t>(std::min<int64_t>(db.n, 4096), (int64_t)p.kp * 4);
+  search_batch(db, qs, nfail, p2, is, ds, wss, depth + 1, st, stats);
+  launch_scatter_results(is, ds, flist, nfail, p.k, out_i, out_d, st);
+  RPLSH_HIP(hipStreamSynchronize(st));
+}
+
+static int plain_kprime(int k) { return std::max(2 * k, k + 32); }
+
+}  // namespace rplsh
+
+using namespace rplsh;
+
+// ------------------------------------------------------------------ IndexBF
+
+struct rplsh_bf {
+  rplsh_resources* res = nullptr;
+  int dim = 0;
+  rplsh_bf_config cfg{};
+  int64_t n = 0;
+  int64_t capacity = 0;
+  int64_t ldx = 0;
+  float* x = nullptr;
+  float* xn = nullptr;
+  float* xn_max_dev = nullptr;
+  float xn_max = 0.0f;
+  void* xenc = nullptr;
+  float* xscale = nullptr;
+  int enc_precision = -1;  // precision xenc currently holds
+  int dpad = 0;
+  std::vector<Workspace> ws;
+  DevBuf qdev, idev, ddev;
+  rplsh_search_stats last{};
+};
+
+static hipStream_t bf_stream(rplsh_bf* b) {
+  void* s = nullptr;
+  if (rplsh_resources_default_stream(b->res, b->cfg.device, &s) != RPLSH_OK)
+    fail(RPLSH_ERR_STATE, "default stream: %s", rplsh_last_error());
+  return (hipStream_t)s;
+}
+
+static Elem elem_of(int precision) {
+  switch (precision) {
+    case RPLSH_PREC_FP32: return Elem::F32;
+    case RPLSH_PREC_BF16: return Elem::BF16;
+    case RPLSH_PREC_FP8: return Elem::FP8;
+  }
+  fail(RPLSH_ERR_INVALID_ARG, "unknown precision %d", precision);
+}
+
+static int elem_bytes(Elem e) { return e == Elem::F32 ? 4 : (e == Elem::BF16 ? 2 : 1); }
+
+// (Re-)encode rows [from, n) of the candidate copy for the configured precision.
+static void bf_encode(rplsh_bf* b, int64_t from, hipStream_t st) {
+  const Elem e = elem_of(b->cfg.precision);
+  if (e == Elem::F32) {
+    b->enc_precision = b->cfg.precision;
+    b->dpad = filter_dpad(e, b->dim);
+    return;
+  }
+  const int dpad = filter_dpad(e, b->dim);
+  RPLSH_REQUIRE(dpad > 0, "dimension %d not supported by precision %d", b->dim, b->cfg.precision);
+  if (b->enc_precision != b->cfg.precision || !b->xenc) {
+    if (b->xenc) RPLSH_HIP(hipFree(b->xenc));
+    if (b->xscale) RPLSH_HIP(hipFree(b->xscale));
+    b->xenc = nullptr;
+    b->xscale = nullptr;
+    if (b->capacity > 0) {
+      RPLSH_HIP(hipMalloc(&b->xenc, (size_t)b->capacity * dpad * elem_bytes(e)));
+      if (e == Elem::FP8) RPLSH_HIP(hipMalloc((void**)&b->xscale, (size_t)b->capacity * sizeof(float)));
+    }
+    b->enc_precision = b->cfg.precision;
+    b->dpad = dpad;
+    from = 0;
+  }
+  const int64_t cnt = b->n - from;
+  if (cnt <= 0) return;
+  const float* src = b->x + from * b->ldx;
+  if (e == Elem::BF16) {
+    launch_encode_bf16(src, cnt, b->dim, b->ldx, dpad, static_cast<uint16_t*>(b->xenc) + from * dpad, st);
+  } else {
+    launch_encode_e4m3(src, cnt, b->dim, b->ldx, dpad, static_cast<uint8_t*>(b->xenc) + from * dpad,
+                       b->xscale + from, st);
+  }
+}
+
+static DbView bf_view(rplsh_bf* b) {
+  DbView v;
+  v.x = b->x;
+  v.ldx = b->ldx;
+  v.n = b->n;
+  v.d = b->dim;
+  v.xnorm = b->xn;
+  v.xn_max = b->xn_max;
+  v.elem = elem_of(b->cfg.precision);
+  v.dpad = b->dpad;
+  if (v.elem == Elem::F32) {
+    v.xenc = b->x;
+    v.ldenc = b->ldx;
+    v.dvalid_bytes = (int)(b->ldx * 4);
+  } else {
+    v.xenc = b->xenc;
+    v.ldenc = b->dpad;
+    v.dvalid_bytes = b->dpad * elem_bytes(v.elem);
+    v.xscale = b->xscale;
+  }
+  return v;
+}
+
+static void bf_free_data(rplsh_bf* b) {
+  if (b->x) (void)hipFree(b->x);
+  if (b->xn) (void)hipFree(b->xn);
+  if (b->xenc) (void)hipFree(b->xenc);
+  if (b->xscale) (void)hipFree(b->xscale);
+  b->x = nullptr;
+  b->xn = nullptr;
+  b->xenc = nullptr;
+  b->xscale = nullptr;
+  b->n = b->capacity = 0;
+  b->xn_max = 0.0f;
+  b->enc_precision = -1;
+}
+
+extern "C" {
+
+void rplsh_bf_config_default(rplsh_bf_config* cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->device = 0;
+  cfg->memory_space = 0;
+  cfg->precision = RPLSH_PREC_FP32;
+  cfg->kprime_factor = 4;
+  cfg->fp64_boundary = 1;
+}
+
+rplsh_status rplsh_bf_create(rplsh_resources* res, int dim, const rplsh_bf_config* cfg,
+                             rplsh_bf** out) {
+  return guard([&] {
+    RPLSH_REQUIRE(out, "out is NULL");
+    RPLSH_REQUIRE(res, "resources is NULL");
+    RPLSH_REQUIRE(dim > 0, "dimension must be > 0 (got %d)", dim);
+    rplsh_bf_config c;
+    rplsh_bf_config_default(&c);
+    if (cfg) c = *cfg;
+    int count = 0;
+    rplsh_device_count(&count);
+    RPLSH_REQUIRE(c.device >= 0 && c.device < count, "device %d out of range [0, %d)", c.device, count);
+    RPLSH_REQUIRE(c.memory_space == 0, "only MemorySpace::Device is supported");
+    RPLSH_REQUIRE(dim <= 256, "dimension %d > 256 is not supported", dim);
+    (void)elem_of(c.precision);
+    if (c.kprime_factor <= 0) c.kprime_factor = 4;
+    if (rplsh_resources_init_device(res, c.device) != RPLSH_OK)
+      fail(RPLSH_ERR_STATE, "%s", rplsh_last_error());
+    auto* b = new rplsh_bf();
+    b->res = res;
+    b->dim = dim;
+    b->cfg = c;
+    b->ldx = round_up(dim, 4);
+    b->ws.resize(4);
+    b->dpad = filter_dpad(elem_of(c.precision), dim);
+    *out = b;
+  });
+}
+
+rplsh_status rplsh_bf_destroy(rplsh_bf* b) {
+  return guard([&] {
+    if (!b) return;
+    DeviceScope scope(b->cfg.device);
+    (void)hipDeviceSynchronize();
+    bf_free_data(b);
+    delete b;
+  });
+}
+
+rplsh_status rplsh_bf_train(rplsh_bf* b, int64_t, const float*) {
+  return guard([&] { RPLSH_REQUIRE(b, "index is NULL"); });
+}
+
+rplsh_status rplsh_bf_reset(rplsh_bf* b) {
+  return guard([&] {
+    RPLSH_REQUIRE(b, "index is NULL");
+    DeviceScope scope(b->cfg.device);
+    RPLSH_HIP(hipStreamSynchronize(bf_stream(b)));
+    bf_free_data(b);
+  });
+}
+
+rplsh_status rplsh_bf_add(rplsh_bf* b, int64_t num, const float* data) {
+  return guard([&] {
+    RPLSH_REQUIRE(b, "index is NULL");
+    if (num <= 0) return;
+    RPLSH_REQUIRE(data, "data is NULL");
+    RPLSH_REQUIRE(b->n + num < (1ll << 31), "index size limited to 2^31 rows");
+    DeviceScope scope(b->cfg.device);
+    hipStream_t st = bf_stream(b);
+    const int64_t need = b->n + num;
+    if (need > b->capacity) {
+      // Grow: new buffers, copy the resident rows, re-encode everything.
+      const int64_t cap = b->capacity == 0 ? need : std::max(need, b->capacity + b->capacity / 2);
+      float *nx = nullptr, *nn = nullptr;
+      RPLSH_HIP(hipMalloc(&nx, (size_t)cap * b->ldx * sizeof(float)));
+      RPLSH_HIP(hipMalloc(&nn, (size_t)cap * sizeof(float)));
+      if (b->n > 0) {
+        RPLSH_HIP(hipMemcpyAsync(nx, b->x, (size_t)b->n * b->ldx * sizeof(float), hipMemcpyDeviceToDevice, st));
+        RPLSH_HIP(hipMemcpyAsync(nn, b->xn, (size_t)b->n * sizeof(float), hipMemcpyDeviceToDevice, st));
+      }
+      RPLSH_HIP(hipStreamSynchronize(st));
+      if (b->x) RPLSH_HIP(hipFree(b->x));
+      if (b->xn) RPLSH_HIP(hipFree(b->xn));
+      if (b->xenc) RPLSH_HIP(hipFree(b->xenc));
+      if (b->xscale) RPLSH_HIP(hipFree(b->xscale));
+      b->xenc = nullptr;
+      b->xscale = nullptr;
+      b->enc_precision = -1;
+      b->x = nx;
+      b->xn = nn;
+      b->capacity = cap;
+    }
+    float* dst = b->x + b->n * b->ldx;
+    if (b->ldx == b->dim) {
+      RPLSH_HIP(hipMemcpyAsync(dst, data, (size_t)num * b->dim * sizeof(float), hipMemcpyDefault, st));
+    } else {
+      RPLSH_HIP(hipMemsetAsync(dst, 0, (size_t)num * b->ldx * sizeof(float), st));
+      RPLSH_HIP(hipMemcpy2DAsync(dst, b->ldx * sizeof(float), data, b->dim * sizeof(float),
+                                 b->dim * sizeof(float), num, hipMemcpyDefault, st));
+    }
+    const int64_t from = b->n;
+    b->n = need;
+    launch_l2norm(dst, num, b->dim, b->ldx, b->xn + from, 1, st);
+    if (!b->xn_max_dev) RPLSH_HIP(hipMalloc(&b->xn_max_dev, sizeof(float)));
+    RPLSH_HIP(hipMemcpyAsync(b->xn_max_dev, &b->xn_max, sizeof(float), hipMemcpyHostToDevice, st));
+    launch_max_reduce(b->xn + from, num, b->xn_max_dev, st);
+    bf_encode(b, b->enc_precision == b->cfg.precision ? from : 0, st);
+    RPLSH_HIP(hipMemcpyAsync(&b->xn_max, b->xn_max_dev, sizeof(float), hipMemcpyDeviceToHost, st));
+    RPLSH_HIP(hipStreamSynchronize(st));
+  });
+}
+
+rplsh_status rplsh_bf_set_precision(rplsh_bf* b, int precision, int kprime_factor) {
+  return guard([&] {
+    RPLSH_REQUIRE(b, "index is NULL");
+    (void)elem_of(precision);
+    RPLSH_REQUIRE(filter_dpad(elem_of(precision), b->dim) > 0, "dimension not supported");
+    DeviceScope scope(b->cfg.device);
+    b->cfg.precision = precision;
+    if (kprime_factor > 0) b->cfg.kprime_factor = kprime_factor;
+    hipStream_t st = bf_stream(b);
+    bf_encode(b, 0, st);
+    RPLSH_HIP(hipStreamSynchronize(st));
+  });
+}
+
+rplsh_status rplsh_bf_search(rplsh_bf* b, int64_t nq, const float* queries, int k,
+                             int32_t* indices, float* distances) {
+  return guard([&] {
+    RPLSH_REQUIRE(b, "index is NULL");
+    if (nq <= 0) return;
+    RPLSH_REQUIRE(queries && indices && distances, "NULL pointer");
+    RPLSH_REQUIRE(k > 0, "k must be > 0 (got %d)", k);
+    RPLSH_REQUIRE(nq < (1ll << 31) && (int64_t)nq * k < (1ll << 40), "batch too large");
+    DeviceScope scope(b->cfg.device);
+    hipStream_t st = bf_stream(b);
+    const int dev = b->cfg.device;
+    rplsh_search_stats stats{};
+    stats.queries = nq;
+    hipEvent_t e0, e1;
+    RPLSH_HIP(hipEventCreate(&e0));
+    RPLSH_HIP(hipEventCreate(&e1));
+    RPLSH_HIP(hipEventRecord(e0, st));
+
+    const float* qd = queries;
+    if (pointer_device(queries) != dev) {
+      float* t = b->qdev.as<float>((size_t)nq * b->dim);
+      RPLSH_HIP(hipMemcpyAsync(t, queries, (size_t)nq * b->dim * sizeof(float), hipMemcpyDefault, st));
+      qd = t;
+    }
+    const bool i_dev = pointer_device(indices) == dev;
+    const bool d_dev = pointer_device(distances) == dev;
+    int32_t* id = i_dev ? indices : b->idev.as<int32_t>((size_t)nq * k);
+    float* dd = d_dev ? distances : b->ddev.as<float>((size_t)nq * k);
+
+    if (b->n == 0) {
+      // Empty index: FLT_MAX / -1 (search.cu:93-100).
+      launch_fill_i32(id, nq * k, -1, st);
+      launch_fill_f32(dd, nq * k, FLT_MAX, st);
+    } else {
+      if (b->enc_precision != b->cfg.precision) bf_encode(b, 0, st);
+      const DbView v = bf_view(b);
+      SearchParams p;
+      p.k = k;
+      p.exact = 1;  // IndexBF::search always asks for exact distances (index_bf.cpp:68-69)
+      if (v.elem == Elem::F32) {
+        p.kp = plain_kprime(k);
+        p.certify = true;
+        p.fp64_boundary = 0;
+      } else {
+        p.kp = k * b->cfg.kprime_factor;
+        p.certify = false;
+        p.fp64_boundary = b->cfg.fp64_boundary;
+      }
+      RPLSH_REQUIRE(std::min<int64_t>(p.kp, b->n) <= 4096, "k' = %d exceeds 4096", p.kp);
+      search_batch(v, qd, (int)nq, p, id, dd, b->ws, 0, st, stats);
+    }
+    if (!i_dev) RPLSH_HIP(hipMemcpyAsync(indices, id, (size_t)nq * k * sizeof(int32_t), hipMemcpyDefault, st));
+    if (!d_dev) RPLSH_HIP(hipMemcpyAsync(distances, dd, (size_t)nq * k * sizeof(float), hipMemcpyDefault, st));
+    RPLSH_HIP(hipEventRecord(e1, st));
+    RPLSH_HIP(hipEventSynchronize(e1));
+    RPLSH_HIP(hipEventElapsedTime(&stats.ms_total, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->last = stats;
+  });
+}
+
+rplsh_status rplsh_bf_num_data(const rplsh_bf* b, int64_t* n) {
+  return guard([&] {
+    RPLSH_REQUIRE(b && n, "NULL argument");
+    *n = b->n;
+  });
+}
+
+rplsh_status rplsh_bf_dimension(const rplsh_bf* b, int* d) {
+  return guard([&] {
+    RPLSH_REQUIRE(b && d, "NULL argument");
+    *d = b->dim;
+  });
+}
+
+rplsh_status rplsh_bf_device(const rplsh_bf* b, int* device) {
+  return guard([&] {
+    RPLSH_REQUIRE(b && device, "NULL argument");
+    *device = b->cfg.device;
+  });
+}
+
+rplsh_status rplsh_bf_data_dev(const rplsh_bf* b, const float** data, const float** norms) {
+  return guard([&] {
+    RPLSH_REQUIRE(b, "index is NULL");
+    if (data) *data = b->x;
+    if (norms) *norms = b->xn;
+  });
+}
+
+rplsh_status rplsh_bf_last_stats(const rplsh_bf* b, rplsh_search_stats* out) {
+  return guard([&] {
+    RPLSH_REQUIRE(b && out, "NULL argument");
+    *out = b->last;
+  });
+}
+
+// searchL2Distance on caller device buffers, plain FP32 path.
+rplsh_status rplsh_search_l2_dev(const float* x, int64_t n, int d, const float* xnorms,
+                                 const float* q, int64_t nq, int k, int32_t* indices,
+                                 float* distances, int exact, void* stream) {
+  return guard([&] {
+    RPLSH_REQUIRE(d > 0 && d <= 256, "dimension %d unsupported", d);
+    RPLSH_REQUIRE(n >= 0 && nq >= 0, "bad sizes");
+    RPLSH_REQUIRE(k > 0, "k must be > 0");
+    if (nq == 0) return;
+    RPLSH_REQUIRE(q && indices && distances && (n == 0 || x), "NULL pointer");
+    hipStream_t st = (hipStream_t)stream;
+    if (n == 0) {
+      launch_fill_i32(indices, nq * k, -1, st);
+      launch_fill_f32(distances, nq * k, FLT_MAX, st);
+      RPLSH_HIP(hipStreamSynchronize(st));
+      return;
+    }
+    RPLSH_REQUIRE(plain_kprime(k) <= 4096 || n <= 4096, "k too large");
+    DevBuf xpad, xn, xmax;
+    DbView v;
+    v.n = n;
+    v.d = d;
+    v.elem = Elem::F32;
+    v.dpad = filter_dpad(Elem::F32, d);
+    const bool aligned = (d % 4 == 0) && ((uintptr_t)x % 16 == 0);
+    if (aligned) {
+      v.x = x;
+      v.ldx = d;
+    } else {
+      const int ld = (int)round_up(d, 4);
+      float* t = xpad.as<float>((size_t)n * ld);
+      launch_pad_rows(x, n, d, d, ld, t, st);
+      v.x = t;
+      v.ldx = ld;
+    }
+    if (xnorms) {
+      v.xnorm = xnorms;
+    } else {
+      float* t = xn.as<float>(n);
+      launch_l2norm(v.x, n, d, v.ldx, t, 1, st);
+      v.xnorm = t;
+    }
+    float* mx = xmax.as<float>(1);
+    RPLSH_HIP(hipMemsetAsync(mx, 0, sizeof(float), st));
+    launch_max_reduce(v.xnorm, n, mx, st);
+    RPLSH_HIP(hipMemcpyAsync(&v.xn_max, mx, sizeof(float), hipMemcpyDeviceToHost, st));
+    RPLSH_HIP(hipStreamSynchronize(st));
+    v.xenc = v.x;
+    v.ldenc = v.ldx;
+    v.dvalid_bytes = (int)(v.ldx * 4);
+    SearchParams p;
+    p.k = k;
+    p.kp = plain_kprime(k);
+    p.exact = exact;
+    p.certify = true;
+    std::vector<Workspace> wss(4);
+    rplsh_search_stats stats{};
+    search_batch(v, q, (int)nq, p, indices, distances, wss, 0, st, stats);
+    RPLSH_HIP(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,408 @@
+// Candidate selection and exact re-ranking.
+//
+// select_kernel: one workgroup per query. Finds the K-th smallest candidate
+// word (key, index) by an 8-pass radix select over the 64-bit words (8 bits
+// per pass, histogram in LDS), then either reports its key (the threshold of
+// the next filter level) or gathers the K smallest words and bitonic-sorts
+// them. Because words are unique (indices are), the K smallest words are
+// exactly the K smallest candidates in (key, index) order -- the order the
+// reference's BlockSelect leaves unspecified for ties (merge_network.cuh:89-92)
+// and the search defines as "lower index first". Role of the reference's
+// kernelL2SelectK / kernelBlockSelect (select.cu:55-96,189-224) and of the
+// cross-tile merge (search.cu:203-208).
+//
+// rerank_kernel: one workgroup per query. Recomputes every candidate's
+// relative distance in the canonical FP32 order
+//     dot = fmaf chain over j,  rel = fmaf(-2, dot, ||x||^2)
+// sorts by (rel, index), keeps k, and writes  D = rel + ||q||^2
+// (broadcast.cu:15-38 folded into the write; index offsets of
+// search.cu:21-42 are not needed: candidates carry global indices).
+// Plain path: certifies that no non-candidate can enter the top k. Precision
+// forms: re-orders the boundary band around the k-th distance by FP64
+// distance.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace rplsh {
+
+namespace {
+
+constexpr int kSelBlock = 256;
+constexpr int kSelLds = 4096;     // candidates staged in LDS when they fit
+constexpr int kMaxSortK = 4096;   // longest sorted list
+
+// Bitonic sort of s[0..P) ascending (P a power of two), whole block.
+__device__ inline void bitonic_sort_u64(uint64_t* s, int P) {
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < (P >> 1); i += blockDim.x) {
+        const int lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+        const int hi = lo + j;
+        const bool up = (lo & k) == 0;
+        const uint64_t x = s[lo], y = s[hi];
+        if ((x > y) == up) {
+          s[lo] = y;
+          s[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__device__ inline int pow2_at_least(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// K-th smallest (1-based) of the n unique words at src, whole block.
+__device__ uint64_t radix_kth(const uint64_t* src, int n, int K, int* hist, int* sh) {
+  uint64_t prefix = 0;
+  int kr = K;
+  for (int pass = 7; pass >= 0; --pass) {
+    const int shift = 8 * pass;
+    const uint64_t mask = pass == 7 ? 0ull : (~0ull << (shift + 8));
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const uint64_t v = src[i];
+      if ((v & mask) == prefix) atomicAdd(&hist[(v >> shift) & 255], 1);
+    }
+    __syncthreads();
+    // Inclusive scan over the 256 bins (Hillis-Steele in place).
+    for (int off = 1; off < 256; off <<= 1) {
+      int v = 0;
+      if (threadIdx.x < 256 && threadIdx.x >= off) v = hist[threadIdx.x - off];
+      __syncthreads();
+      if (threadIdx.x < 256) hist[threadIdx.x] += v;
+      __syncthreads();
+    }
+    if (threadIdx.x < 256) {
+      const int before = threadIdx.x == 0 ? 0 : hist[threadIdx.x - 1];
+      if (before < kr && kr <= hist[threadIdx.x]) {
+        sh[0] = threadIdx.x;
+        sh[1] = before;
+      }
+    }
+    __syncthreads();
+    prefix |= (uint64_t)sh[0] << shift;
+    kr -= sh[1];
+    __syncthreads();
+  }
+  return prefix;
+}
+
+__global__ __launch_bounds__(kSelBlock) void select_kernel(SelectArgs a) {
+  __shared__ uint64_t stage[kSelLds];
+  __shared__ uint64_t outl[kMaxSortK];
+  __shared__ int hist[256];
+  __shared__ int sh[4];
+  const int q = blockIdx.x;
+  if (q >= a.nq) return;
+  const int count = a.counts[q];
+  const int n = count < a.cap ? count : a.cap;
+  int st = 0;
+  if (count > a.cap) st |= kStatusOverflow;
+  const uint64_t* gsrc = a.cand + (int64_t)q * a.cap;
+  const uint64_t* src = gsrc;
+  if (n <= kSelLds) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) stage[i] = gsrc[i];
+    __syncthreads();
+    src = stage;
+  }
+  const int K = a.K;
+  if (n < K) st |= kStatusShort;
+
+  if (!a.topk_mode) {
+    float tau = INFINITY;
+    if (n >= K && K > 0) tau = cand_key(radix_kth(src, n, K, hist, sh));
+    if (threadIdx.x == 0) {
+      a.tau_out[q] = tau;
+      if (st && a.status) atomicOr(a.status + q, st);
+    }
+    return;
+  }
+
+  // Top-K mode: the K smallest words, sorted.
+  const int take = n < K ? n : K;
+  uint64_t thr = ~0ull;
+  if (n > K) thr = radix_kth(src, n, K, hist, sh);
+  if (threadIdx.x == 0) sh[2] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint64_t v = src[i];
+    if (n <= K || v <= thr) {
+      const int p = atomicAdd(&sh[2], 1);
+      if (p < kMaxSortK) outl[p] = v;
+    }
+  }
+  __syncthreads();
+  const int P = pow2_at_least(take > 0 ? take : 1);
+  for (int i = take + threadIdx.x; i < P; i += blockDim.x) outl[i] = kEmptyCand;
+  __syncthreads();
+  bitonic_sort_u64(outl, P);
+  uint64_t* dst = a.topk_out + (int64_t)q * K;
+  for (int i = threadIdx.x; i < K; i += blockDim.x) dst[i] = i < take ? outl[i] : kEmptyCand;
+  if (threadIdx.x == 0) {
+    // Every row outside the list has key >= bound: either a survivor that lost
+    // the select, or a row the filter rejected (key > tau_in).
+    a.bound_out[q] = n >= K && K > 0 ? cand_key(outl[K - 1]) : a.tau_in[q];
+    if (st && a.status) atomicOr(a.status + q, st);
+  }
+}
+
+// Canonical FP32 relative distance of query sq (in LDS) to row idx.
+__device__ inline float canonical_rel(const float* __restrict__ x, int64_t ldx,
+                                      const float* __restrict__ xnorm, int d, const float* sq,
+                                      int32_t idx, bool vec4) {
+  const float* xr = x + (int64_t)idx * ldx;
+  float s = 0.0f;
+  if (vec4) {
+    for (int j = 0; j < d; j += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(xr + j);
+      s = fmaf(sq[j + 0], v.x, s);
+      s = fmaf(sq[j + 1], v.y, s);
+      s = fmaf(sq[j + 2], v.z, s);
+      s = fmaf(sq[j + 3], v.w, s);
+    }
+  } else {
+    for (int j = 0; j < d; ++j) s = fmaf(sq[j], xr[j], s);
+  }
+  return fmaf(-2.0f, s, xnorm[idx]);
+}
+
+// sum_j (q_j - x_j)^2 in double, sequential j, no contraction: the same
+// arithmetic as the FP64 oracle.
+__device__ inline double fp64_dist(const float* __restrict__ x, int64_t ldx, int d,
+                                   const float* sq, int32_t idx) {
+  const float* xr = x + (int64_t)idx * ldx;
+  double s = 0.0;
+  for (int j = 0; j < d; ++j) {
+    const double t = __dsub_rn((double)sq[j], (double)xr[j]);
+    s = __dadd_rn(s, __dmul_rn(t, t));
+  }
+  return s;
+}
+
+constexpr int kMaxBand = 512;
+
+__global__ __launch_bounds__(kSelBlock) void rerank_kernel(RerankArgs a) {
+  __shared__ uint64_t s[kMaxSortK];
+  __shared__ float sq[256];
+  __shared__ double bd[kMaxBand];
+  __shared__ int bi[kMaxBand];
+  __shared__ uint64_t bw[kMaxBand];
+  __shared__ int sh[4];
+  const int q = blockIdx.x;
+  if (q >= a.nq) return;
+  const int d = a.d;
+  for (int j = threadIdx.x; j < d; j += blockDim.x) sq[j] = a.q[(int64_t)q * a.ldq + j];
+  __syncthreads();
+  const bool vec4 = (d % 4 == 0) && (a.ldx % 4 == 0);
+  const int kp = a.kp;
+  const int P = pow2_at_least(kp);
+  const uint64_t* src = a.topk + (int64_t)q * kp;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    uint64_t w = kEmptyCand;
+    if (i < kp) {
+      const uint64_t c = src[i];
+      if (c != kEmptyCand) {
+        const int32_t idx = cand_idx(c);
+        w = pack_cand(canonical_rel(a.x, a.ldx, a.xnorm, d, sq, idx, vec4), idx);
+      }
+    }
+    s[i] = w;
+  }
+  __syncthreads();
+  bitonic_sort_u64(s, P);
+
+  if (threadIdx.x == 0) {
+    int valid = 0;
+    // Empty words sort last; count the real ones.
+    int lo = 0, hi = kp;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s[mid] == kEmptyCand) hi = mid; else lo = mid + 1;
+    }
+    valid = lo;
+    sh[0] = valid;
+  }
+  __syncthreads();
+  const int valid = sh[0];
+  const int k = a.k;
+  const float qn = a.qnorm[q];
+
+  // Plain path: certify the candidate margin.
+  if (a.bound && threadIdx.x == 0 && valid >= k && k > 0) {
+    const float relk = cand_key(s[k - 1]);
+    const float eps = a.eps_coeff * (qn + 2.0f * a.xn_max);
+    const float b = a.bound[q];
+    if (!(relk < b - eps)) {
+      atomicOr(a.status + q, kStatusCertFail);
+      a.relk_out[q] = relk;
+    }
+  }
+
+  // Precision forms: FP64 order inside the band around the k-th distance.
+  if (a.fp64_boundary && valid > k && k > 0) {
+    if (threadIdx.x == 0) {
+      const float relk = cand_key(s[k - 1]);
+      const float E = a.band_coeff * (qn + 2.0f * a.xn_max) + 4.0f * 5.96e-8f * (fabsf(relk) + qn);
+      int b0 = k - 1, b1 = k - 1;
+      while (b0 > 0 && cand_key(s[b0 - 1]) >= relk - 2.0f * E && (k - 1) - (b0 - 1) < kMaxBand / 2) --b0;
+      while (b1 + 1 < valid && cand_key(s[b1 + 1]) <= relk + 2.0f * E && (b1 + 1) - b0 < kMaxBand) ++b1;
+      sh[1] = b0;
+      sh[2] = b1;
+    }
+    __syncthreads();
+    const int b0 = sh[1], b1 = sh[2];
+    const int nb = b1 - b0 + 1;
+    if (nb > 1 && b1 >= k) {
+      for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        const int32_t idx = cand_idx(s[b0 + i]);
+        bd[i] = fp64_dist(a.x, a.ldx, d, sq, idx);
+        bi[i] = b0 + i;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        // Insertion sort of the band by (fp64 distance, index); nb is small.
+        for (int i = 1; i < nb; ++i) {
+          const double kd = bd[i];
+          const int kb = bi[i];
+          const int32_t kx = cand_idx(s[kb]);
+          int j = i - 1;
+          while (j >= 0 && (bd[j] > kd || (bd[j] == kd && cand_idx(s[bi[j]]) > kx))) {
+            bd[j + 1] = bd[j];
+            bi[j + 1] = bi[j];
+            --j;
+          }
+          bd[j + 1] = kd;
+          bi[j + 1] = kb;
+        }
+        atomicAdd(a.fp64_count, nb);
+      }
+      __syncthreads();
+      // Rewrite the band positions in FP64 order (words keep their FP32 key).
+      for (int i = threadIdx.x; i < nb; i += blockDim.x) bw[i] = s[bi[i]];
+      __syncthreads();
+      for (int i = threadIdx.x; i < nb; i += blockDim.x) s[b0 + i] = bw[i];
+      __syncthreads();
+    }
+  }
+
+  for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    const int64_t o = (int64_t)q * k + j;
+    if (j < valid) {
+      const uint64_t w = s[j];
+      const float rel = cand_key(w);
+      a.out_idx[o] = cand_idx(w);
+      a.out_dist[o] = a.exact ? rel + qn : rel;
+    } else {
+      a.out_idx[o] = -1;
+      a.out_dist[o] = FLT_MAX;
+    }
+  }
+}
+
+__global__ void collect_failed_kernel(const int* __restrict__ status, int nq, int* __restrict__ list,
+                                      int* __restrict__ n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nq && status[i] != 0) {
+    const int p = atomicAdd(n, 1);
+    list[p] = i;
+  }
+}
+
+__global__ void sum_counts_kernel(const int* __restrict__ counts, int nq, int cap,
+                                  unsigned long long* __restrict__ out) {
+  unsigned long long s = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x)
+    s += (unsigned long long)counts[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
+}
+
+__global__ void gather_rows_kernel(const float* __restrict__ src, int64_t ld, const int* __restrict__ rows,
+                                   int nrows, int d, float* __restrict__ dst) {
+  const int64_t total = (int64_t)nrows * d;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / d);
+    const int j = (int)(e - (int64_t)r * d);
+    dst[e] = src[(int64_t)rows[r] * ld + j];
+  }
+}
+
+__global__ void scatter_results_kernel(const int32_t* __restrict__ si, const float* __restrict__ sd,
+                                       const int* __restrict__ rows, int nrows, int k,
+                                       int32_t* __restrict__ di, float* __restrict__ dd) {
+  const int64_t total = (int64_t)nrows * k;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / k);
+    const int j = (int)(e - (int64_t)r * k);
+    di[(int64_t)rows[r] * k + j] = si[e];
+    dd[(int64_t)rows[r] * k + j] = sd[e];
+  }
+}
+
+inline unsigned grid_for(int64_t work) {
+  int64_t g = (work + 255) / 256;
+  if (g > 65536) g = 65536;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+}  // namespace
+
+void launch_select(const SelectArgs& a, hipStream_t st) {
+  if (a.nq <= 0) return;
+  if (a.topk_mode && a.K > kMaxSortK) fail(RPLSH_ERR_UNSUPPORTED, "top-K select limited to %d", kMaxSortK);
+  select_kernel<<<a.nq, kSelBlock, 0, st>>>(a);
+  RPLSH_HIP(hipGetLastError());
+}
+
+void launch_rerank(const RerankArgs& a, hipStream_t st) {
+  if (a.nq <= 0) return;
+  if (a.kp > kMaxSortK) fail(RPLSH_ERR_UNSUPPORTED, "re-rank list limited to %d", kMaxSortK);
+  if (a.d > 256) fail(RPLSH_ERR_UNSUPPORTED, "re-rank dimension limited to 256");
+  rerank_kernel<<<a.nq, kSelBlock, 0, st>>>(a);
+  RPLSH_HIP(hipGetLastError());
+}
+
+void launch_collect_failed(const int* status, int nq, int* list, int* n, hipStream_t st) {
+  if (nq <= 0) return;
+  collect_failed_kernel<<<(unsigned)div_up(nq, 256), 256, 0, st>>>(status, nq, list, n);
+  RPLSH_HIP(hipGetLastError());
+}
+
+void launch_sum_counts(const int* counts, int nq, int cap, long long* out, hipStream_t st) {
+  if (nq <= 0) return;
+  unsigned g = grid_for(nq);
+  if (g > 256) g = 256;
+  sum_counts_kernel<<<g, 256, 0, st>>>(counts, nq, cap, reinterpret_cast<unsigned long long*>(out));
+  RPLSH_HIP(hipGetLastError());
+}
+
+void launch_gather_rows(const float* src, int64_t ld, const int* rows, int nrows, int d, float* dst,
+                        hipStream_t st) {
+  if (nrows <= 0) return;
+  gather_rows_kernel<<<grid_for((int64_t)nrows * d), 256, 0, st>>>(src, ld, rows, nrows, d, dst);
+  RPLSH_HIP(hipGetLastError());
+}
+
+void launch_scatter_results(const int32_t* si, const float* sd, const int* rows, int nrows, int k,
+                            int32_t* di, float* dd, hipStream_t st) {
+  if (nrows <= 0 || k <= 0) return;
+  scatter_results_kernel<<<grid_for((int64_t)nrows * k), 256, 0, st>>>(si, sd, rows, nrows, k, di, dd);
+  RPLSH_HIP(hipGetLastError());
+}
+
+}  // namespace rplsh

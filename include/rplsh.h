@@ -1,0 +1,214 @@
+/*
+ * rplsh.h -- C ABI of the MI355X brute-force L2 kNN library (libcurplsh_amd.so).
+ *
+ * This is the boundary the C++ Index/IndexBF host classes (include/curplsh/)
+ * call, and what any other binding (ctypes, cgo, JNI) binds. Plain pointers,
+ * sizes and status codes only; no HIP or C++ types cross it. Streams are
+ * passed as opaque `void*` (a hipStream_t; NULL = the null stream).
+ *
+ * Conventions
+ *  - Every entry point returns rplsh_status; RPLSH_OK == 0. On failure a
+ *    thread-local message is available from rplsh_last_error().
+ *  - Pointers are caller-owned unless a function says it copies.
+ *  - Functions named *_dev take device pointers only. rplsh_bf_add and
+ *    rplsh_bf_search accept host or device pointers, like the reference
+ *    IndexBF (src/index_bf.cpp:26-74).
+ *  - Indices are int32, distances float32, row-major [rows][cols].
+ *  - Not thread-safe per handle: one rplsh_resources / rplsh_bf per thread, or
+ *    external locking (device_resources.h:113-165 is unlocked in the reference
+ *    too).
+ */
+#ifndef RPLSH_H
+#define RPLSH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rplsh_status {
+  RPLSH_OK = 0,
+  RPLSH_ERR_INVALID_ARG = 1,  /* precondition violated (reference: host_assert -> abort) */
+  RPLSH_ERR_HIP = 2,          /* HIP runtime error (reference: checkCudaErrors -> exit) */
+  RPLSH_ERR_OUT_OF_MEMORY = 3,
+  RPLSH_ERR_STATE = 4,        /* call not valid in the object's current state */
+  RPLSH_ERR_UNSUPPORTED = 5,
+  RPLSH_ERR_BUDGET = 6,       /* HBM budget of a serving loop exceeded */
+  RPLSH_ERR_NO_DEVICE = 7
+} rplsh_status;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* rplsh_last_error(void);
+const char* rplsh_version(void);
+
+/* ------------------------------------------------------------ devices -- */
+/* getNumDevices / getCurrentDevice / setCurrentDevice
+ * (include/internal/cuda_utils.h:1230-1300, used by index.cpp:14). */
+rplsh_status rplsh_device_count(int* count);
+rplsh_status rplsh_get_device(int* device);
+rplsh_status rplsh_set_device(int device);
+/* getDeviceForAddress (cuda_utils.h:1369-1388): device of a pointer, -1 for
+ * host memory. */
+rplsh_status rplsh_pointer_device(const void* ptr, int* device);
+
+/* ---------------------------------------------------------- resources -- */
+/* DeviceResources (include/device_resources.h:60-263): per-device default
+ * stream, async-copy stream and two alternate streams, created lazily. There
+ * is no BLAS or RNG handle: the contraction is this library's own kernel. */
+typedef struct rplsh_resources rplsh_resources;
+
+rplsh_status rplsh_resources_create(rplsh_resources** out);
+rplsh_status rplsh_resources_destroy(rplsh_resources* res);
+/* initializeForDevice (device_resources.h:113-165). */
+rplsh_status rplsh_resources_init_device(rplsh_resources* res, int device);
+/* getDefaultStream (device_resources.h:181-184). */
+rplsh_status rplsh_resources_default_stream(rplsh_resources* res, int device, void** stream);
+/* setDefaultStream: a user stream replaces the created one
+ * (userDefaultStreams_, device_resources.h:248). NULL restores it. */
+rplsh_status rplsh_resources_set_default_stream(rplsh_resources* res, int device, void* stream);
+/* getAlternateStreams (device_resources.h:187-190); writes up to *count. */
+rplsh_status rplsh_resources_alternate_streams(rplsh_resources* res, int device,
+                                               void** streams, int* count);
+/* getAsyncCopyStream (device_resources.h:199-202). */
+rplsh_status rplsh_resources_async_copy_stream(rplsh_resources* res, int device, void** stream);
+/* syncDefaultStream (device_resources.h:220-223). */
+rplsh_status rplsh_resources_sync_default_stream(rplsh_resources* res, int device);
+
+/* ------------------------------------------------------------- memory -- */
+/* allocMemory / copyMemory (include/memory_space.h:26-41). memory_space:
+ * 0 = Device (hipMalloc), 1 = Unified (hipMallocManaged). */
+rplsh_status rplsh_malloc(int device, size_t bytes, int memory_space, void** out);
+rplsh_status rplsh_free(void* ptr);
+/* Stream-ordered copy in any direction (hipMemcpyDefault semantics). */
+rplsh_status rplsh_memcpy_async(void* dst, const void* src, size_t bytes, void* stream);
+rplsh_status rplsh_memset_async(void* dst, int value, size_t bytes, void* stream);
+rplsh_status rplsh_stream_sync(void* stream);
+rplsh_status rplsh_device_sync(void);
+/* Pinned host memory (hipHostMalloc) for fast staging. */
+rplsh_status rplsh_host_alloc(size_t bytes, void** out);
+rplsh_status rplsh_host_free(void* ptr);
+rplsh_status rplsh_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
+
+/* Events for timing on a given stream. */
+typedef struct rplsh_event rplsh_event;
+rplsh_status rplsh_event_create(rplsh_event** out);
+rplsh_status rplsh_event_destroy(rplsh_event* ev);
+rplsh_status rplsh_event_record(rplsh_event* ev, void* stream);
+rplsh_status rplsh_event_elapsed_ms(rplsh_event* start, rplsh_event* stop, float* ms);
+rplsh_status rplsh_event_sync(rplsh_event* ev);
+
+/* ------------------------------------------- internal kernels (device) -- */
+/* computeL2Norm (include/internal/norm.h:7-8, src/internal/norm.cu:191-202):
+ * out[i] = sum_j x[i][j]^2 (squared != 0) or its sqrt, in the canonical
+ * order n = fmaf(x[j], x[j], n), j = 0..d-1. Device pointers. */
+rplsh_status rplsh_l2norm_dev(const float* x, int64_t n, int d, float* out, int squared,
+                              void* stream);
+
+/* Encoders used by the precision forms. Device pointers.
+ * bf16: round-to-nearest-even, row-major [n][ld] with zero padding d..ld-1.
+ * e4m3: per-row FP32 scale s = the smallest power of two >= max_j|x_j| / 448
+ * (1 for an all-zero row; a power of two so the MFMA applies it exactly as an
+ * E8M0 exponent), code = e4m3_rne_sat(x_j / s); scales[i] = s. */
+rplsh_status rplsh_encode_bf16_dev(const float* x, int64_t n, int d, int ld, uint16_t* out,
+                                   void* stream);
+rplsh_status rplsh_encode_e4m3_dev(const float* x, int64_t n, int d, int ld, uint8_t* out,
+                                   float* scales, void* stream);
+
+/* Precision of the candidate contraction. */
+typedef enum rplsh_precision {
+  RPLSH_PREC_FP32 = 0, /* plain path: FP32 MFMA candidates + canonical FP32 re-rank */
+  RPLSH_PREC_BF16 = 1, /* BF16 MFMA candidates, over-fetch, FP32 + FP64-boundary re-rank */
+  RPLSH_PREC_FP8 = 2   /* FP8 e4m3 (per-row scale) candidates, same re-rank */
+} rplsh_precision;
+
+/* searchL2Distance (include/internal/search.h:8-11, search.cu:79-216) on
+ * device pointers, plain FP32 path: indices/distances [nq][k] ascending by
+ * (distance, index). xnorms may be NULL (computed, as search.cu:102-108).
+ * exact != 0 adds ||q||^2 (broadcast.cu:15-38). k > n pads with -1/FLT_MAX;
+ * n == 0 fills -1/FLT_MAX (search.cu:93-100). Synchronises `stream`. */
+rplsh_status rplsh_search_l2_dev(const float* x, int64_t n, int d, const float* xnorms,
+                                 const float* q, int64_t nq, int k, int32_t* indices,
+                                 float* distances, int exact, void* stream);
+
+/* ------------------------------------------------------------ IndexBF -- */
+/* IndexBFConfig + IndexConfig (include/index_bf.h:8-19, include/index.h:8-16)
+ * plus the precision-form knobs of this library. */
+typedef struct rplsh_bf_config {
+  int device;                  /* IndexConfig::device */
+  int memory_space;            /* IndexConfig::memorySpace: 0 Device, 1 Unified */
+  int store_transposed;        /* IndexBFConfig::storeTransposed (accepted; layout is
+                                  always the row-major + packed copies below) */
+  int compute_exact_distances; /* IndexBFConfig::computeExactDistances; IndexBF::search
+                                  always returns exact distances (index_bf.cpp:68-69) */
+  int precision;               /* rplsh_precision */
+  int kprime_factor;           /* over-fetch k' = kprime_factor * k for BF16/FP8 (0 -> 4) */
+  int fp64_boundary;           /* FP64 re-rank of the boundary band (BF16/FP8), default 1 */
+} rplsh_bf_config;
+
+/* Fills the defaults: device 0, Device memory, FP32, k' factor 4, boundary on. */
+void rplsh_bf_config_default(rplsh_bf_config* cfg);
+
+typedef struct rplsh_bf rplsh_bf;
+
+/* IndexBF::IndexBF (index_bf.cpp:8-11) / Index::Index (index.cpp:8-27):
+ * dim > 0 and device < count, else RPLSH_ERR_INVALID_ARG. `res` is not
+ * owned and must outlive the index (index.h:75). */
+rplsh_status rplsh_bf_create(rplsh_resources* res, int dim, const rplsh_bf_config* cfg,
+                             rplsh_bf** out);
+rplsh_status rplsh_bf_destroy(rplsh_bf* index);
+/* IndexBF::train: a no-op, the index needs no training (index_bf.cpp:13-15). */
+rplsh_status rplsh_bf_train(rplsh_bf* index, int64_t n, const float* x);
+/* IndexBF::add (index_bf.cpp:26-44): copies x (host or device) into an owned
+ * device buffer, computes the squared norms and the low-precision copy of the
+ * configured precision. Unlike the reference's one-shot add, repeated adds
+ * append (labels continue sequentially). n <= 0 is a no-op. */
+rplsh_status rplsh_bf_add(rplsh_bf* index, int64_t n, const float* x);
+/* IndexBF::search (index_bf.cpp:46-74): k nearest by exact squared L2,
+ * ascending, ties by lower index; host or device pointers for queries and
+ * outputs; k > n pads with -1 / FLT_MAX. nq <= 0 is a no-op. Returns when the
+ * results are in place. */
+rplsh_status rplsh_bf_search(rplsh_bf* index, int64_t nq, const float* queries, int k,
+                             int32_t* indices, float* distances);
+/* IndexBF::reset (index_bf.cpp:17-24). */
+rplsh_status rplsh_bf_reset(rplsh_bf* index);
+rplsh_status rplsh_bf_num_data(const rplsh_bf* index, int64_t* n);
+rplsh_status rplsh_bf_dimension(const rplsh_bf* index, int* d);
+rplsh_status rplsh_bf_device(const rplsh_bf* index, int* device);
+rplsh_status rplsh_bf_set_precision(rplsh_bf* index, int precision, int kprime_factor);
+/* Device pointers of the resident data (row-major [n][d] f32, norms [n]). */
+rplsh_status rplsh_bf_data_dev(const rplsh_bf* index, const float** data, const float** norms);
+
+/* Counters of the last search on this index. */
+typedef struct rplsh_search_stats {
+  int64_t queries;          /* queries in the call */
+  int64_t candidates;       /* total candidates kept for re-rank */
+  int64_t survivors;        /* total survivors of the final filter level */
+  int32_t levels;           /* filter levels run (sample passes + full pass) */
+  int32_t retried_queries;  /* queries re-run because a threshold was too tight or
+                               too loose, or (FP32) the candidate margin was not
+                               certified */
+  int32_t boundary_fp64;    /* candidates re-ranked in FP64 */
+  int32_t kprime;           /* candidates per query re-ranked */
+  float ms_total;           /* device time of the call */
+} rplsh_search_stats;
+rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out);
+
+/* --------------------------------------------------- synthetic data -- */
+/* Counter-based, thread-count independent generators (splitmix64 hashing +
+ * Box-Muller in double); identical output on every x86-64 host running the
+ * same libm. Host pointers, row-major [n][d]. */
+rplsh_status rplsh_gen_uniform(float* out, int64_t n, int d, uint64_t seed, float lo, float hi);
+rplsh_status rplsh_gen_gaussian(float* out, int64_t n, int d, uint64_t seed, int unit_normalize);
+/* Gaussian mixture: centres U[-1,1)^d from center_seed; point i = centre[c_i]
+ * + sigma * N(0, I), c_i and noise from point_seed. dup_fraction of the rows
+ * (Bernoulli per row) are overwritten with exact copies of non-duplicate
+ * rows. */
+rplsh_status rplsh_gen_mixture(float* out, int64_t n, int d, int centers, float sigma,
+                               uint64_t center_seed, uint64_t point_seed, double dup_fraction);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RPLSH_H */
