@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Headline benchmark: brute-force L2 kNN QPS on one MI355X.
+
+Workload (the metric's configuration): database 10,000,000 x 96 unit-normalised
+standard-Gaussian vectors (generator seed 4), 10,000 queries from the same
+distribution (seed 5), k = 100. A step is one IndexBF.search of the whole
+10,000-query batch with queries and outputs resident in HBM.
+
+Forms measured on the same resident index:
+  * bf16  (headline `value`): BF16 MFMA candidates, over-fetch k' = 4k = 400,
+    exact FP32 re-rank, FP64 re-rank of the boundary band;
+  * fp32  (plain path, the drop-in): FP32 MFMA candidates + canonical FP32
+    re-rank (bit-exact to the canonical restatement);
+  * fp8   : e4m3 + per-row power-of-two scale candidates, same re-rank.
+recall@100 and the max relative distance error are measured against the FP64
+oracle on a query subset; the plain path is checked bit-exact against the
+canonical FP32 restatement on that subset.
+
+roofline: the dominant kernel is the full-database filter launch (the fused
+contraction). Its algorithmic work is 2 * nq * n * d flops per launch; its
+duration is measured live with HIP events recorded on the search stream around
+that launch. cpu_baseline: the canonical FP32 restatement of the reference's
+searchL2Distance, run on the host cores over a bounded query sample.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+MI355X_PEAK = {"bf16": 2500.0, "fp8": 5000.0, "fp32": 157.3}  # dense TFLOP/s
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--d", type=int, default=96)
+    ap.add_argument("--k", type=int, default=100)
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp8", "fp32"])
+    ap.add_argument("--forms", default="bf16,fp32,fp8",
+                    help="forms to time (the --precision form is the headline value)")
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--recall-queries", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build has no "
+              "multi-GPU path; the per-GPU value is the N=1 line.", file=sys.stderr)
+        return 2
+
+    import curplsh_amd as ca
+
+    if ca.device_count() < 1:
+        log("no GPU visible")
+        return 1
+    n, nq, d, k = args.n, args.nq, args.d, args.k
+    log(f"[bench] generating {n}x{d} database and {nq} queries")
+    t0 = time.time()
+    x = ca.gen_gaussian(n, d, 4, True)
+    q = ca.gen_gaussian(nq, d, 5, True)
+    log(f"[bench] generated in {time.time() - t0:.1f}s")
+
+    res = ca.DeviceResources(0)
+    forms = [f for f in args.forms.split(",") if f]
+    if args.precision not in forms:
+        forms.insert(0, args.precision)
+    forms.sort(key=lambda f: f != args.precision)
+
+    idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=args.precision, kprime_factor=4))
+    t0 = time.time()
+    idx.add(x)
+    log(f"[bench] index built (add incl. H2D, norms, encode) in {time.time() - t0:.1f}s")
+    dq = ca.DeviceArray.from_host(q)
+    dI = ca.DeviceArray((nq, k), np.int32)
+    dD = ca.DeviceArray((nq, k), np.float32)
+
+    results = {}
+    for form in forms:
+        idx.set_precision(form, 4)
+        steps = args.steps if form == args.precision else max(1, args.plain_steps)
+        for _ in range(args.warmup):
+            idx.search(dq, k, dI, dD)
+        ca.call("rplsh_device_sync")
+        filt, tot, sel, rr, samp = [], [], [], [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            idx.search(dq, k, dI, dD)
+            s = idx.last_stats()
+            filt.append(s["ms_filter_full"])
+            tot.append(s["ms_total"])
+            sel.append(s["ms_select"])
+            rr.append(s["ms_rerank"])
+            samp.append(s["ms_filter_samples"])
+        ca.call("rplsh_device_sync")
+        wall = time.perf_counter() - t0
+        ms = wall * 1000.0 / steps
+        flops = 2.0 * nq * n * d
+        fms = float(np.mean(filt))
+        st = idx.last_stats()
+        results[form] = {
+            "qps": nq / (ms / 1000.0),
+            "ms_per_step": ms,
+            "ms_filter_full": fms,
+            "ms_filter_samples": float(np.mean(samp)),
+            "ms_select": float(np.mean(sel)),
+            "ms_rerank": float(np.mean(rr)),
+            "filter_tflops": flops / (fms / 1000.0) / 1e12,
+            "steps": steps,
+            "stats": st,
+            "I": dI.to_host(),
+            "D": dD.to_host(),
+        }
+        log(f"[bench] {form}: {ms:.2f} ms/step, QPS {results[form]['qps']:.0f}, filter {fms:.2f} ms "
+            f"({results[form]['filter_tflops']:.1f} TFLOP/s), samples {np.mean(samp):.2f} ms, "
+            f"select {np.mean(sel):.2f} ms, rerank {np.mean(rr):.2f} ms, stats {st}")
+
+    head = results[args.precision]
+    extra = {}
+    # Accuracy against the FP64 oracle / the canonical restatement on a subset.
+    R = min(args.recall_queries, nq)
+    oracle_mod = None
+    if R > 0:
+        import oracle as oracle_mod  # the checker, not the measured path
+
+        t0 = time.time()
+        I64, _ = oracle_mod.search_fp64(x, q[:R], k)
+        log(f"[bench] FP64 oracle on {R} queries in {time.time() - t0:.1f}s")
+        for form, r in results.items():
+            rec = oracle_mod.recall(r["I"][:R], I64)
+            dp = oracle_mod.pair_dist_fp64(x, q[:R], r["I"][:R])
+            err = float(np.max(np.abs(r["D"][:R].astype(np.float64) - dp) / np.maximum(dp, 1e-30)))
+            extra[f"{form}_recall_at_{k}"] = rec
+            extra[f"{form}_max_rel_dist_err"] = err
+        if "fp32" in results:
+            t0 = time.time()
+            Ic, Dc = oracle_mod.search_canonical(x, q[:R], k)
+            extra["fp32_bitexact_vs_canonical"] = bool(
+                np.array_equal(Ic, results["fp32"]["I"][:R])
+                and np.array_equal(Dc.view(np.uint32), results["fp32"]["D"][:R].view(np.uint32)))
+            log(f"[bench] canonical check on {R} queries in {time.time() - t0:.1f}s")
+
+    # CPU baseline: canonical FP32 restatement (port) on a bounded query sample.
+    cpu = None
+    if args.cpu_seconds > 0:
+        if oracle_mod is None:
+            import oracle as oracle_mod
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cal = min(nq, threads)
+        t0 = time.perf_counter()
+        oracle_mod.search_canonical(x, q[:cal], k, threads=threads)
+        tc = time.perf_counter() - t0
+        S = int(min(nq, max(cal, cal * args.cpu_seconds / max(tc, 1e-6))))
+        S = max(threads, (S // threads) * threads)
+        t0 = time.perf_counter()
+        oracle_mod.search_canonical(x, q[:S], k, threads=threads)
+        tc = time.perf_counter() - t0
+        cpu = {"value": S / tc, "unit": "QPS", "cores": threads, "kind": "port",
+               "sample": f"first {S} of the {nq} queries against the full {n}x{d} database, k={k}, "
+                         f"canonical FP32 restatement of searchL2Distance (oracle/oracle.c), "
+                         f"{threads} OpenMP threads, {tc:.1f}s"}
+        log(f"[bench] CPU baseline {cpu['value']:.1f} QPS on {threads} threads ({S} queries, {tc:.1f}s)")
+
+    form = args.precision
+    peak = MI355X_PEAK[form]
+    ach = head["filter_tflops"]
+    line = {
+        "metric": "QPS at 10M x 96, k=100: BF16/FP8+rerank vs FP32 plain, recall@100 >= 0.999",
+        "value": round(head["qps"], 1),
+        "unit": "QPS",
+        "n_gpus": 1,
+        "steps": head["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": form,
+        "data": "synthetic: seeded unit-normalised Gaussian vectors (counter-based generator)",
+        "config": {
+            "workload": f"{n}x{d} database, {nq} queries, k={k}, {form} candidates k'=4k + FP32 re-rank"
+                        + (" + FP64 boundary" if form != "fp32" else ""),
+            "n": n, "d": d, "nq": nq, "k": k, "precision": form, "kprime": 4 * k if form != "fp32" else None,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": round(ach, 2),
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": round(ach / peak, 4),
+            "traffic": None,
+            "kernel": f"filter_kernel<{form}> full-database launch",
+            "algorithmic_flops_per_launch": 2.0 * nq * n * d,
+            "ms_per_launch": round(head["ms_filter_full"], 4),
+        },
+        "cpu_baseline": cpu,
+    }
+    for f, r in results.items():
+        line[f"{f}_qps"] = round(r["qps"], 1)
+        line[f"{f}_ms_per_step"] = round(r["ms_per_step"], 3)
+        line[f"{f}_filter_tflops"] = round(r["filter_tflops"], 2)
+        line[f"{f}_filter_frac_of_peak"] = round(r["filter_tflops"] / MI355X_PEAK[f], 4)
+        line[f"{f}_phase_ms"] = {"filter_full": round(r["ms_filter_full"], 3),
+                                 "filter_samples": round(r["ms_filter_samples"], 3),
+                                 "select": round(r["ms_select"], 3), "rerank": round(r["ms_rerank"], 3)}
+        line[f"{f}_retried_queries"] = r["stats"]["retried_queries"]
+    if "fp32" in results and form != "fp32":
+        line["speedup_vs_plain_fp32"] = round(head["qps"] / results["fp32"]["qps"], 3)
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

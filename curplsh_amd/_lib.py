@@ -62,6 +62,11 @@ class SearchStats(C.Structure):
         ("boundary_fp64", C.c_int32),
         ("kprime", C.c_int32),
         ("ms_total", C.c_float),
+        ("ms_filter_full", C.c_float),
+        ("ms_filter_samples", C.c_float),
+        ("ms_select", C.c_float),
+        ("ms_rerank", C.c_float),
+        ("rows_filter_full", C.c_int64),
     ]
 
 

@@ -71,6 +71,28 @@ struct DevBuf {
   }
 };
 
+// Events around the phases of a batch (depth 0 only), read after the sync.
+struct PhaseEvents {
+  hipEvent_t ev[8] = {};
+  bool on = false;
+  void create() {
+    for (auto& e : ev) RPLSH_HIP(hipEventCreate(&e));
+    on = true;
+  }
+  ~PhaseEvents() {
+    if (on)
+      for (auto e : ev) (void)hipEventDestroy(e);
+  }
+  void rec(int i, hipStream_t st) {
+    if (on) RPLSH_HIP(hipEventRecord(ev[i], st));
+  }
+  float ms(int a, int b) const {
+    float t = 0.0f;
+    if (on) RPLSH_HIP(hipEventElapsedTime(&t, ev[a], ev[b]));
+    return t;
+  }
+};
+
 struct Workspace {
   DevBuf qnorm, qenc, qscale, qpad, tau, tau2, counts, cand, topk, bound, status, relk;
   DevBuf flist, fcount, fp64, surv;
@@ -228,9 +250,15 @@ static void search_batch(const DbView& db, const float* q, int nq, const SearchP
   float* relk = ws.relk.as<float>(nq);
   launch_fill_f32(tau, nq, INFINITY, st);
   launch_fill_i32(status, nq, 0, st);
+  PhaseEvents pe;
+  if (depth == 0) pe.create();
+  pe.rec(0, st);
   for (size_t li = 0; li < L.size(); ++li) {
     const Level& lv = L[li];
+    const bool last = li + 1 == L.size();
+    if (last) pe.rec(1, st);
     run_filter(db, lv, qenc, qscale, tau, nq, cap, counts, cand, st);
+    if (last) pe.rec(2, st);
     SelectArgs sa{};
     sa.cand = cand;
     sa.counts = counts;
@@ -251,9 +279,13 @@ static void search_batch(const DbView& db, const float* q, int nq, const SearchP
       sa.bound_out = bound;
       sa.status = status;
       launch_select(sa, st);
+      pe.rec(3, st);
     }
   }
-  stats.levels = (int32_t)L.size();
+  if (depth == 0) {
+    stats.levels = (int32_t)L.size();
+    stats.rows_filter_full = L.back().m;
+  }
   {
     long long* sv = ws.surv.as<long long>(1);
     RPLSH_HIP(hipMemsetAsync(sv, 0, sizeof(long long), st));
@@ -286,7 +318,9 @@ static void search_batch(const DbView& db, const float* q, int nq, const SearchP
   ra.status = status;
   ra.relk_out = relk;
   ra.fp64_count = fp64c;
+  pe.rec(4, st);
   launch_rerank(ra, st);
+  pe.rec(5, st);
 
   // 4. failed queries
   int* flist = ws.flist.as<int>(nq);
@@ -301,6 +335,10 @@ static void search_batch(const DbView& db, const float* q, int nq, const SearchP
   RPLSH_HIP(hipStreamSynchronize(st));
   stats.boundary_fp64 += nfp64;
   if (depth == 0) {
+    stats.ms_filter_samples = pe.ms(0, 1);
+    stats.ms_filter_full = pe.ms(1, 2);
+    stats.ms_select = pe.ms(2, 3);
+    stats.ms_rerank = pe.ms(4, 5);
     stats.survivors = nsurv;
     stats.candidates = (int64_t)nq * kp;
     stats.kprime = kp;

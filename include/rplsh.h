@@ -192,6 +192,12 @@ typedef struct rplsh_search_stats {
   int32_t boundary_fp64;    /* candidates re-ranked in FP64 */
   int32_t kprime;           /* candidates per query re-ranked */
   float ms_total;           /* device time of the call */
+  float ms_filter_full;     /* device time of the full-database filter launch
+                               (HIP events on the search stream) */
+  float ms_filter_samples;  /* device time of the sample-level filter launches */
+  float ms_select;          /* device time of the last-level top-k' select */
+  float ms_rerank;          /* device time of the re-rank */
+  int64_t rows_filter_full; /* database rows of the full filter launch */
 } rplsh_search_stats;
 rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out);
 
