@@ -1,0 +1,133 @@
+// C++ drop-in test of curplsh::IndexBF / searchL2Distance / computeL2Norm,
+// written like the reference's drivers (test/test_search.cpp,
+// test/test_norm.cpp) but asserting. Needs a GPU. Exit status 0 = pass.
+//
+// Checks:
+//  1. test_norm fixture (test/test_norm.cpp:15-22): all-ones rows -> norm = dim.
+//  2. test_search fixture (test/test_search.cpp:17-36) through the internal
+//     searchL2Distance with relative distances: top-10 [1,0,2,...,9].
+//  3. IndexBF with host pointers vs the canonical CPU oracle (bit-exact I, D).
+//  4. IndexBF with device pointers gives the same results; repeated add
+//     equals one add; reset empties the index.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "curplsh/index_bf.h"
+#include "curplsh/internal/norm.h"
+#include "curplsh/internal/search.h"
+#include "../../oracle/oracle.h"
+
+#define CHECK(c)                                                          \
+  do {                                                                    \
+    if (!(c)) {                                                           \
+      std::fprintf(stderr, "FAIL %s at %s:%d\n", #c, __FILE__, __LINE__); \
+      return 1;                                                           \
+    }                                                                     \
+  } while (0)
+
+using namespace curplsh;
+
+static int test_norm(DeviceResources& res) {
+  for (int dim : {32, 64, 96, 100, 128, 256, 1500}) {
+    const int num = 1001;
+    std::vector<float> h((size_t)num * dim, 1.0f);
+    DeviceTensor<float, 2> v({num, dim});
+    DeviceTensor<float, 1> n({num});
+    Stream s = res.getDefaultStream(0);
+    v.copyFrom(h.data(), s);
+    computeL2Norm(v, n, true, s);
+    std::vector<float> out(num);
+    n.copyTo(out.data(), s);
+    streamSynchronize(s);
+    for (int i = 0; i < num; ++i) CHECK(out[i] == (float)dim);
+  }
+  std::printf("norm fixture ok\n");
+  return 0;
+}
+
+static int test_search_fixture(DeviceResources& res) {
+  const int dim = 128, nb = 1000, nq = 100, k = 10;
+  std::vector<float> b((size_t)nb * dim), q((size_t)nq * dim, 1.0f);
+  for (int i = 0; i < nb; ++i)
+    for (int d = 0; d < dim; ++d) b[(size_t)i * dim + d] = std::fmin((float)i, 10.0f);
+  Stream s = res.getDefaultStream(0);
+  DeviceTensor<float, 2> bases({nb, dim}), queries({nq, dim}), dist({nq, k});
+  DeviceTensor<int, 2> idx({nq, k});
+  bases.copyFrom(b.data(), s);
+  queries.copyFrom(q.data(), s);
+  searchL2Distance(&res, bases, nullptr, queries, k, idx, dist, false);
+  std::vector<int> I((size_t)nq * k);
+  std::vector<float> D((size_t)nq * k);
+  idx.copyTo(I.data(), s);
+  dist.copyTo(D.data(), s);
+  streamSynchronize(s);
+  const int eI[10] = {1, 0, 2, 3, 4, 5, 6, 7, 8, 9};
+  const float eD[10] = {-128, 0, 0, 384, 1024, 1920, 3072, 4480, 6144, 8064};
+  for (int r = 0; r < nq; ++r)
+    for (int j = 0; j < k; ++j) {
+      CHECK(I[(size_t)r * k + j] == eI[j]);
+      CHECK(D[(size_t)r * k + j] == eD[j]);
+    }
+  std::printf("search fixture ok\n");
+  return 0;
+}
+
+static int test_index(DeviceResources& res) {
+  const int dim = 96, n = 50000, nq = 300, k = 20;
+  std::vector<float> x((size_t)n * dim), q((size_t)nq * dim);
+  CHECK(rplsh_gen_uniform(x.data(), n, dim, 11, -1.0f, 1.0f) == RPLSH_OK);
+  CHECK(rplsh_gen_uniform(q.data(), nq, dim, 12, -1.0f, 1.0f) == RPLSH_OK);
+  std::vector<int32_t> Io((size_t)nq * k);
+  std::vector<float> Do((size_t)nq * k);
+  oracle_search_canonical(x.data(), n, dim, nullptr, q.data(), nq, k, 1, Io.data(), Do.data(), 0);
+
+  IndexBF index(&res, dim);
+  index.train(n, x.data());
+  index.add(n / 2, x.data());
+  index.add(n - n / 2, x.data() + (size_t)(n / 2) * dim);
+  CHECK(index.getNumData() == n);
+  std::vector<int> I((size_t)nq * k);
+  std::vector<float> D((size_t)nq * k);
+  index.search(nq, q.data(), k, I.data(), D.data());
+  CHECK(std::memcmp(I.data(), Io.data(), I.size() * 4) == 0);
+  CHECK(std::memcmp(D.data(), Do.data(), D.size() * 4) == 0);
+
+  // Device-resident queries and outputs.
+  DeviceTensor<float, 2> dq({nq, dim}), dd({nq, k});
+  DeviceTensor<int, 2> di({nq, k});
+  Stream s = res.getDefaultStream(0);
+  dq.copyFrom(q.data(), s);
+  streamSynchronize(s);
+  index.search(nq, dq.data(), k, di.data(), dd.data());
+  std::vector<int> I2((size_t)nq * k);
+  std::vector<float> D2((size_t)nq * k);
+  di.copyTo(I2.data(), s);
+  dd.copyTo(D2.data(), s);
+  streamSynchronize(s);
+  CHECK(I2 == I);
+  CHECK(std::memcmp(D2.data(), D.data(), D.size() * 4) == 0);
+
+  index.reset();
+  CHECK(index.getNumData() == 0);
+  index.search(nq, q.data(), k, I.data(), D.data());
+  for (size_t i = 0; i < I.size(); ++i) CHECK(I[i] == -1);
+  std::printf("IndexBF ok\n");
+  return 0;
+}
+
+int main() {
+  if (getNumDevices() < 1) {
+    std::fprintf(stderr, "no GPU\n");
+    return 2;
+  }
+  DeviceScope scope(0);
+  DeviceResources res(0);
+  if (test_norm(res)) return 1;
+  if (test_search_fixture(res)) return 1;
+  if (test_index(res)) return 1;
+  std::printf("all C++ tests passed\n");
+  return 0;
+}
