@@ -152,8 +152,9 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(SelectArgs a) {
   for (int i = threadIdx.x; i < K; i += blockDim.x) dst[i] = i < take ? outl[i] : kEmptyCand;
   if (threadIdx.x == 0) {
     // Every row outside the list has key >= bound: either a survivor that lost
-    // the select, or a row the filter rejected (key > tau_in).
-    a.bound_out[q] = n >= K && K > 0 ? cand_key(outl[K - 1]) : a.tau_in[q];
+    // the select (only when there were more than K), or a row the filter
+    // rejected (key > tau_in; +inf when the level kept every row).
+    a.bound_out[q] = count > K && K > 0 ? cand_key(outl[K - 1]) : a.tau_in[q];
     if (st && a.status) atomicOr(a.status + q, st);
   }
 }
