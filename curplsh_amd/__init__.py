@@ -24,7 +24,7 @@ __all__ = [
     "DeviceResources", "DeviceArray", "IndexBF", "IndexBFConfig", "RplshError",
     "search_l2_distance", "compute_l2_norm", "encode_bf16", "encode_e4m3",
     "gen_uniform", "gen_gaussian", "gen_mixture", "device_count", "Event", "load",
-    "PREC_FP32", "PREC_BF16", "PREC_FP8",
+    "PREC_FP32", "PREC_BF16", "PREC_FP8", "Server",
 ]
 
 _PREC = {"fp32": PREC_FP32, "plain": PREC_FP32, "bf16": PREC_BF16, "fp8": PREC_FP8,
@@ -275,6 +275,87 @@ class IndexBF:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Continuous-batching request loop over resident indexes (rplsh_server_*).
+
+    submit() enqueues a request (host arrays) and returns a handle; wait()
+    returns (indices, distances, info). Requests for the same index and k that
+    queue behind each other run as one batch; batches on different slots
+    overlap on the GPU; admission is bounded by an HBM budget."""
+
+    def __init__(self, device: int = 0, slots: int = 4, hbm_budget_bytes: int = 0,
+                 max_batch_queries: int = 4096):
+        load()
+        cfg = _lib.ServerConfig()
+        _lib.load().rplsh_server_config_default(C.byref(cfg))
+        cfg.device, cfg.slots = device, slots
+        cfg.hbm_budget_bytes, cfg.max_batch_queries = hbm_budget_bytes, max_batch_queries
+        self._h = C.c_void_p()
+        call("rplsh_server_create", C.byref(cfg), C.byref(self._h))
+        self._pending = {}
+        self._indexes = []
+
+    def add_index(self, index: "IndexBF") -> int:
+        i = C.c_int()
+        call("rplsh_server_add_index", self._h, index._h, C.byref(i))
+        self._indexes.append(index)
+        return i.value
+
+    def submit(self, index_id: int, q: np.ndarray, k: int) -> int:
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        I = np.empty((q.shape[0], k), np.int32)
+        D = np.empty((q.shape[0], k), np.float32)
+        rid = C.c_uint64()
+        call("rplsh_server_submit", self._h, int(index_id), q.shape[0], q.ctypes.data, int(k),
+             I.ctypes.data, D.ctypes.data, C.byref(rid))
+        self._pending[rid.value] = (q, I, D)
+        return rid.value
+
+    def done(self, rid: int) -> bool:
+        d = C.c_int()
+        call("rplsh_server_poll", self._h, C.c_uint64(rid), C.byref(d))
+        return bool(d.value)
+
+    def wait(self, rid: int, timeout_s: float = -1.0):
+        info = _lib.RequestInfo()
+        try:
+            call("rplsh_server_wait", self._h, C.c_uint64(rid), float(timeout_s), C.byref(info))
+        except RplshError as e:
+            if e.status != 4:  # a finished request (failed) releases its buffers
+                self._pending.pop(rid, None)
+            raise
+        _, I, D = self._pending.pop(rid)
+        return I, D, {f: getattr(info, f) for f, _ in info._fields_}
+
+    def drain(self) -> None:
+        call("rplsh_server_drain", self._h)
+
+    def stats(self) -> dict:
+        s = _lib.ServerStats()
+        call("rplsh_server_get_stats", self._h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def reset_stats(self) -> None:
+        call("rplsh_server_reset_stats", self._h)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            call("rplsh_server_destroy", self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_bytes(index: "IndexBF", nq: int, k: int) -> int:
+    b = C.c_size_t()
+    call("rplsh_bf_batch_bytes", index._h, int(nq), int(k), C.byref(b))
+    return b.value
 
 
 def compute_l2_norm(x: DeviceArray, out: DeviceArray, squared: bool = True, stream: int = 0) -> None:

@@ -70,6 +70,43 @@ class SearchStats(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("device", C.c_int),
+        ("slots", C.c_int),
+        ("hbm_budget_bytes", C.c_size_t),
+        ("max_batch_queries", C.c_int),
+    ]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [
+        ("status", C.c_int),
+        ("latency_ms", C.c_double),
+        ("queue_ms", C.c_double),
+        ("batch_requests", C.c_int),
+        ("batch_queries", C.c_int64),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64),
+        ("completed", C.c_int64),
+        ("rejected", C.c_int64),
+        ("queries", C.c_int64),
+        ("batches", C.c_int64),
+        ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("mean_ms", C.c_double),
+        ("max_ms", C.c_double),
+        ("wall_s", C.c_double),
+        ("qps", C.c_double),
+        ("peak_reserved_bytes", C.c_size_t),
+        ("peak_inflight", C.c_int),
+    ]
+
+
 _vp = C.c_void_p
 _i32 = C.c_int
 _i64 = C.c_int64
@@ -124,6 +161,17 @@ _SIGS = {
     "rplsh_bf_set_precision": [_vp, _i32, _i32],
     "rplsh_bf_data_dev": [_vp, C.POINTER(_vp), C.POINTER(_vp)],
     "rplsh_bf_last_stats": [_vp, C.POINTER(SearchStats)],
+    "rplsh_server_config_default": [C.POINTER(ServerConfig)],
+    "rplsh_server_create": [C.POINTER(ServerConfig), C.POINTER(_vp)],
+    "rplsh_server_destroy": [_vp],
+    "rplsh_server_add_index": [_vp, _vp, C.POINTER(_i32)],
+    "rplsh_server_submit": [_vp, _i32, _i64, _vp, _i32, _vp, _vp, C.POINTER(C.c_uint64)],
+    "rplsh_server_poll": [_vp, C.c_uint64, C.POINTER(_i32)],
+    "rplsh_server_wait": [_vp, C.c_uint64, C.c_double, C.POINTER(RequestInfo)],
+    "rplsh_server_drain": [_vp],
+    "rplsh_server_get_stats": [_vp, C.POINTER(ServerStats)],
+    "rplsh_server_reset_stats": [_vp],
+    "rplsh_bf_batch_bytes": [_vp, _i64, _i32, C.POINTER(C.c_size_t)],
     "rplsh_gen_uniform": [_vp, _i64, _i32, _u64, _f32, _f32],
     "rplsh_gen_gaussian": [_vp, _i64, _i32, _u64, _i32],
     "rplsh_gen_mixture": [_vp, _i64, _i32, _i32, _f32, _u64, _u64, C.c_double],
@@ -133,6 +181,7 @@ _RESTYPES = {
     "rplsh_last_error": C.c_char_p,
     "rplsh_version": C.c_char_p,
     "rplsh_bf_config_default": None,
+    "rplsh_server_config_default": None,
 }
 
 _lib = None

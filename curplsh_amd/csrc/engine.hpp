@@ -1,0 +1,130 @@
+// Internal engine interface shared by the search entry points (search.hip)
+// and the serving loop (server.hip). Nothing here crosses the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "rplsh.h"
+
+namespace rplsh {
+
+// Growable device buffer (hipMalloc only when a call needs more than before).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void* ensure(size_t b) {
+    if (b <= bytes) return p;
+    if (p) RPLSH_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    size_t nb = std::max(b, (size_t)256);
+    RPLSH_HIP(hipMalloc(&p, nb));
+    bytes = nb;
+    return p;
+  }
+  template <typename T>
+  T* as(size_t count) {
+    return static_cast<T*>(ensure(count * sizeof(T)));
+  }
+};
+
+// Events around the phases of a batch (depth 0 only), read after the sync.
+struct PhaseEvents {
+  hipEvent_t ev[8] = {};
+  bool on = false;
+  void create() {
+    for (auto& e : ev) RPLSH_HIP(hipEventCreate(&e));
+    on = true;
+  }
+  ~PhaseEvents() {
+    if (on)
+      for (auto e : ev) (void)hipEventDestroy(e);
+  }
+  void rec(int i, hipStream_t st) {
+    if (on) RPLSH_HIP(hipEventRecord(ev[i], st));
+  }
+  float ms(int a, int b) const {
+    float t = 0.0f;
+    if (on) RPLSH_HIP(hipEventElapsedTime(&t, ev[a], ev[b]));
+    return t;
+  }
+};
+
+struct Workspace {
+  DevBuf qnorm, qenc, qscale, qpad, tau, tau2, counts, cand, topk, bound, status, relk;
+  DevBuf flist, fcount, fp64, surv;
+  DevBuf qsub, isub, dsub;  // a retried subset of queries
+};
+
+// The database as the search kernels see it.
+struct DbView {
+  const float* x = nullptr;  // FP32 rows [n][ldx]
+  int64_t ldx = 0;
+  int64_t n = 0;
+  int d = 0;
+  const float* xnorm = nullptr;
+  float xn_max = 0.0f;
+  Elem elem = Elem::F32;
+  int dpad = 0;
+  const void* xenc = nullptr;  // candidate rows [n][ldenc] of elem
+  int64_t ldenc = 0;
+  int dvalid_bytes = 0;
+  const float* xscale = nullptr;  // FP8 per-row scales
+  bool packed = false;            // xenc is in 32-row MFMA tiles (filter_packed.hip)
+};
+
+struct SearchParams {
+  int k = 0;
+  int kp = 0;          // candidates re-ranked per query
+  int exact = 1;
+  int fp64_boundary = 0;
+  bool certify = false;
+  int c_factor = 8;    // last-level target survivors = c_factor * kp
+};
+
+
+// One batch of nq device-resident FP32 queries [nq][d] -> device outputs
+// [nq][k], on stream st, with the workspaces wss (one per retry depth).
+// Synchronises st.
+void search_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
+                  int32_t* out_i, float* out_d, std::vector<Workspace>& wss, int depth,
+                  hipStream_t st, rplsh_search_stats& stats);
+
+// Device bytes search_batch allocates for one batch (depth-0 workspace).
+size_t search_workspace_bytes(const DbView& db, int nq, const SearchParams& p);
+
+// The view and the parameters IndexBF::search uses for k on an index; the
+// candidate copy must already match the configured precision.
+DbView bf_db_view(const rplsh_bf* b);
+SearchParams bf_search_params(const rplsh_bf* b, int k);
+int bf_device_of(const rplsh_bf* b);
+// Brings the candidate copy up to date with the configured precision.
+void bf_prepare(rplsh_bf* b);
+
+}  // namespace rplsh

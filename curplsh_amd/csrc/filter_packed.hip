@@ -1,0 +1,456 @@
+// Fused contraction + threshold filter over an MFMA-tile-packed database
+// (BF16 and FP8 candidate copies). Same arithmetic and filter as
+// filter_kernel (filter.hip); different data movement:
+//
+//   * The index stores its candidate copy packed by 32-row tiles: within a
+//     tile, K-chunk c is a contiguous block of 64 lane-pieces, lane l's piece
+//     holding row (l % 32), elements [c*KC + (l/32)*KC/2, +KC/2) -- exactly
+//     the A operand of v_mfma_f32_32x32x16_bf16 (KC = 16) or of
+//     v_mfma_scale_f32_32x32x64_f8f6f4 (KC = 64). A wave loads an operand
+//     with one fully coalesced 1 KB (BF16) / 2 KB (FP8) read, straight into
+//     registers: no LDS staging and no workgroup barrier.
+//   * One wave per SIMD, up to TN = 8 query tiles (256 queries) per wave in
+//     registers, so every A operand feeds TN MFMAs: the register file, not
+//     LDS or L1 bandwidth, carries the reuse. The four waves of a workgroup
+//     take different queries and the same database rows, so a tile fetched
+//     by one wave is an L1 hit for the others.
+//   * Chunk c of the next tile is loaded into the operand register of chunk c
+//     as soon as the current tile's MFMAs on it are issued; -||x||^2/2 of the
+//     next tile is prefetched at the top of the iteration.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace rplsh {
+
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kBlock = 256;
+
+template <Elem E>
+struct PackTraits;
+template <>
+struct PackTraits<Elem::BF16> {
+  static constexpr int kBytes = 2;
+  static constexpr int kLaneBytes = 16;  // 8 bf16
+  static constexpr int kChunkElems = 16;
+};
+template <>
+struct PackTraits<Elem::FP8> {
+  static constexpr int kBytes = 1;
+  static constexpr int kLaneBytes = 32;  // 32 e4m3
+  static constexpr int kChunkElems = 64;
+};
+
+template <int LB>
+struct Frag {
+  i32x4 v[LB / 16];
+};
+
+template <int LB>
+__device__ inline Frag<LB> load_frag(const char* p) {
+  Frag<LB> f;
+#pragma unroll
+  for (int i = 0; i < LB / 16; ++i) f.v[i] = *reinterpret_cast<const i32x4*>(p + 16 * i);
+  return f;
+}
+
+template <int LB>
+__device__ inline Frag<LB> zero_frag() {
+  Frag<LB> f;
+#pragma unroll
+  for (int i = 0; i < LB / 16; ++i) f.v[i] = i32x4{0, 0, 0, 0};
+  return f;
+}
+
+template <Elem E>
+__device__ inline f32x16 mfma_chunk(const Frag<PackTraits<E>::kLaneBytes>& a,
+                                    const Frag<PackTraits<E>::kLaneBytes>& b, f32x16 c, int sa,
+                                    int sb);
+
+template <>
+__device__ inline f32x16 mfma_chunk<Elem::BF16>(const Frag<16>& a, const Frag<16>& b, f32x16 c,
+                                                int, int) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a.v[0]),
+                                                  __builtin_bit_cast(bf16x8, b.v[0]), c, 0, 0, 0);
+}
+
+template <>
+__device__ inline f32x16 mfma_chunk<Elem::FP8>(const Frag<32>& a, const Frag<32>& b, f32x16 c,
+                                               int sa, int sb) {
+  i32x8 av, bv;
+  av.lo = a.v[0];
+  av.hi = a.v[1];
+  bv.lo = b.v[0];
+  bv.hi = b.v[1];
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, c, 0, 0, 0, sa, 0, sb);
+}
+
+__device__ inline int e8m0_of(float s) { return (int)((__float_as_uint(s) >> 23) & 0xff); }
+
+template <Elem E, int DPAD>
+struct Geometry {
+  static constexpr int LB = PackTraits<E>::kLaneBytes;
+  static constexpr int CHUNK = 64 * LB;                       // bytes of one K-chunk of a tile
+  static constexpr int NC = DPAD / PackTraits<E>::kChunkElems;  // K-chunks per row
+  static constexpr int TILE = NC * CHUNK;                     // bytes of a 32-row tile
+  static constexpr int ROWB = DPAD * PackTraits<E>::kBytes;   // bytes of a query row
+};
+
+template <Elem E, int DPAD, int TN>
+__global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) {
+  using G = Geometry<E, DPAD>;
+  constexpr int LB = G::LB;
+  constexpr int NC = G::NC;
+  constexpr int BQ = 4 * TN * 32;
+
+  const int per_xcd = (int)(gridDim.x >> 3);
+  const int logical = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (logical >= a.num_qtiles * a.num_slices) return;
+  const int qtile = logical % a.num_qtiles;
+  const int slice = logical / a.num_qtiles;
+  const int64_t r_begin = (int64_t)slice * a.rows_per_slice;
+  int64_t r_end = r_begin + a.rows_per_slice;
+  if (r_end > a.m) r_end = a.m;
+  if (r_begin >= r_end) return;
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5, col = lane & 31;
+
+  Frag<LB> qf[TN][NC];
+  float thr[TN];
+  int qsc[TN];
+  int qrow[TN];
+#pragma unroll
+  for (int nt = 0; nt < TN; ++nt) {
+    const int qr = qtile * BQ + (wave * TN + nt) * 32 + col;
+    qrow[nt] = qr;
+    if (qr < a.nq) {
+      const char* qp = reinterpret_cast<const char*>(a.q) + (int64_t)qr * G::ROWB;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) qf[nt][c] = load_frag<LB>(qp + 2 * LB * c + LB * h);
+      thr[nt] = fmaxf(-0.5f * a.tau[qr], -FLT_MAX);
+      qsc[nt] = E == Elem::FP8 ? e8m0_of(a.qscale[qr]) : 127;
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) qf[nt][c] = zero_frag<LB>();
+      thr[nt] = INFINITY;
+      qsc[nt] = 127;
+    }
+  }
+
+  const char* xb = reinterpret_cast<const char*>(a.x);
+  const int64_t stride = a.stride;
+  const int64_t m = a.m;
+
+  // Address of this lane's A operand piece of the 32-row tile starting at
+  // logical row j0 (lane row j0 + col), or nullptr past the end.
+  auto tile_ptr = [&](int64_t j0) -> const char* {
+    const int64_t j = j0 + col;
+    if (j >= m) return nullptr;
+    const int64_t p = j * stride;
+    return xb + (p >> 5) * G::TILE + ((p & 31) + 32 * h) * LB;
+  };
+  auto tile_scale = [&](int64_t j0) -> int {
+    const int64_t j = j0 + col;
+    return (E == Elem::FP8 && j < m) ? e8m0_of(a.xscale[j * stride]) : 127;
+  };
+  // -||x||^2/2 of this lane's output rows j0 + 8*(i/4) + 4*h + (i%4).
+  auto load_cinit = [&](int64_t j0) -> f32x16 {
+    f32x16 ci;
+    if (stride == 1 && j0 + 32 <= m) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(a.xnorm + j0 + 8 * g + 4 * h);
+        ci[4 * g + 0] = -0.5f * v[0];
+        ci[4 * g + 1] = -0.5f * v[1];
+        ci[4 * g + 2] = -0.5f * v[2];
+        ci[4 * g + 3] = -0.5f * v[3];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int64_t r = j0 + 8 * (i >> 2) + 4 * h + (i & 3);
+        ci[i] = r < m ? -0.5f * a.xnorm[r * stride] : -INFINITY;
+      }
+    }
+    return ci;
+  };
+
+  // Single operand buffer: chunk c of the next tile is loaded into af[c] as
+  // soon as the current tile's MFMAs on af[c] have been issued.
+  Frag<LB> af[NC];
+  {
+    const char* p0 = tile_ptr(r_begin);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) af[c] = p0 ? load_frag<LB>(p0 + c * G::CHUNK) : zero_frag<LB>();
+  }
+  f32x16 ci = load_cinit(r_begin);
+  int xs = tile_scale(r_begin);
+
+  for (int64_t j0 = r_begin; j0 < r_end; j0 += 32) {
+    const bool has_next = j0 + 32 < r_end;
+    const char* pn = has_next ? tile_ptr(j0 + 32) : nullptr;
+    f32x16 nci = ci;
+    int nxs = xs;
+    if (has_next) {
+      nci = load_cinit(j0 + 32);
+      nxs = tile_scale(j0 + 32);
+    }
+
+    f32x16 acc[TN];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+      for (int nt = 0; nt < TN; ++nt)
+        acc[nt] = mfma_chunk<E>(af[c], qf[nt][c], c == 0 ? ci : acc[nt], xs, qsc[nt]);
+      if (has_next) af[c] = pn ? load_frag<LB>(pn + c * G::CHUNK) : zero_frag<LB>();
+    }
+
+    float mx[TN];
+    bool hit = false;
+#pragma unroll
+    for (int nt = 0; nt < TN; ++nt) {
+      float v = acc[nt][0];
+#pragma unroll
+      for (int i = 1; i < 16; ++i) v = fmaxf(v, acc[nt][i]);
+      mx[nt] = v;
+      hit |= v >= thr[nt];
+    }
+    if (__any(hit)) {
+#pragma unroll
+      for (int nt = 0; nt < TN; ++nt) {
+        if (mx[nt] >= thr[nt]) {
+          const int qr = qrow[nt];
+          int* cnt = a.counts + qr;
+          uint64_t* dst = a.cand + (int64_t)qr * a.cap;
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const float v = acc[nt][i];
+            if (v >= thr[nt]) {
+              const int64_t row = j0 + 8 * (i >> 2) + 4 * h + (i & 3);
+              const int pos = atomicAdd(cnt, 1);
+              if (pos < a.cap) dst[pos] = pack_cand(-2.0f * v, (int32_t)(row * stride));
+            }
+          }
+        }
+      }
+    }
+
+    ci = nci;
+    xs = nxs;
+  }
+}
+
+// Packed encoders: one thread per 16-byte output piece.
+__device__ inline uint16_t bf16_rne(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ inline uint8_t e4m3_rne_sat(float v) {
+  if (isnan(v)) return 0x7f;
+  const uint8_t sign = signbit(v) ? 0x80 : 0x00;
+  const float a = fabsf(v);
+  if (a >= 448.0f) return sign | 0x7e;
+  if (a < 0.015625f) return sign | (uint8_t)(int)rintf(a * 512.0f);
+  int e2;
+  frexpf(a, &e2);
+  int eb = e2 - 1;
+  int r = (int)rintf(ldexpf(a, 3 - eb));
+  if (r == 16) {
+    r = 8;
+    ++eb;
+  }
+  int code = ((eb + 7) << 3) | (r - 8);
+  if (code > 0x7e) code = 0x7e;
+  return sign | (uint8_t)code;
+}
+
+template <Elem E, int DPAD>
+__global__ void encode_packed_kernel(const float* __restrict__ x, int64_t n, int d, int64_t ldx,
+                                     int64_t tile_from, int64_t tile_to, char* __restrict__ out,
+                                     const float* __restrict__ scales) {
+  using G = Geometry<E, DPAD>;
+  constexpr int LB = G::LB;
+  constexpr int PIECES = G::TILE / 16;  // 16-byte pieces per tile
+  const int64_t total = (tile_to - tile_from) * PIECES;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = tile_from + e / PIECES;
+    const int w = (int)(e % PIECES);               // piece within the tile
+    const int c = w / (G::CHUNK / 16);             // K-chunk
+    const int l = (w % (G::CHUNK / 16)) / (LB / 16);  // lane
+    const int sub = w % (LB / 16);                 // 16-byte part of the lane piece
+    const int64_t row = t * 32 + (l & 31);
+    const int k0 = c * PackTraits<E>::kChunkElems + (l >> 5) * (PackTraits<E>::kChunkElems / 2) +
+                   sub * (16 / PackTraits<E>::kBytes);
+    i32x4 piece = {0, 0, 0, 0};
+    if (row < n) {
+      if (E == Elem::BF16) {
+        uint16_t v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = k0 + i < d ? bf16_rne(x[row * ldx + k0 + i]) : 0;
+        __builtin_memcpy(&piece, v, 16);
+      } else {
+        const float s = scales[row];
+        uint8_t v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = k0 + i < d ? e4m3_rne_sat(x[row * ldx + k0 + i] / s) : 0;
+        __builtin_memcpy(&piece, v, 16);
+      }
+    }
+    *reinterpret_cast<i32x4*>(out + t * G::TILE + (int64_t)w * 16) = piece;
+  }
+}
+
+// Per-row power-of-two FP8 scale (same rule as the row-major encoder).
+__global__ __launch_bounds__(256) void e4m3_scales_kernel(const float* __restrict__ x, int64_t n,
+                                                          int d, int64_t ldx,
+                                                          float* __restrict__ scales) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + wave;
+  if (row >= n) return;
+  float mx = 0.0f;
+  for (int j = lane; j < d; j += 64) mx = fmaxf(mx, fabsf(x[row * ldx + j]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  float s = 1.0f;
+  if (mx > 0.0f) {
+    const float s0 = mx / 448.0f;
+    int e;
+    const float f = frexpf(s0, &e);
+    s = f == 0.5f ? s0 : ldexpf(1.0f, e);
+  }
+  if (lane == 0) scales[row] = s;
+}
+
+// Query tiles per wave: as many as keep the query operands within ~160
+// registers (the accumulators take 16 per tile, the rest holds the current and
+// the prefetched database operands), at most 7.
+template <Elem E, int DPAD>
+constexpr int packed_tn() {
+  using G = Geometry<E, DPAD>;
+  constexpr int per_tile = G::NC * G::LB / 4;
+  constexpr int tn = 160 / per_tile;
+  return tn > 7 ? 7 : (tn < 1 ? 1 : tn);
+}
+
+template <Elem E, int DPAD>
+void launch_packed_one(const FilterArgs& a, hipStream_t st) {
+  constexpr int TN = packed_tn<E, DPAD>();
+  const int total = a.num_qtiles * a.num_slices;
+  filter_packed_kernel<E, DPAD, TN><<<(unsigned)round_up(total, 8), kBlock, 0, st>>>(a);
+  RPLSH_HIP(hipGetLastError());
+}
+
+template <Elem E, int DPAD>
+void encode_one(const float* x, int64_t n, int d, int64_t ldx, int64_t row_from, char* out,
+                float* scales, hipStream_t st) {
+  using G = Geometry<E, DPAD>;
+  if (E == Elem::FP8 && n > row_from) {
+    e4m3_scales_kernel<<<(unsigned)div_up(n - row_from, 4), 256, 0, st>>>(
+        x + row_from * ldx, n - row_from, d, ldx, scales + row_from);
+    RPLSH_HIP(hipGetLastError());
+  }
+  const int64_t t0 = row_from / 32, t1 = div_up(n, 32);
+  if (t1 <= t0) return;
+  const int64_t work = (t1 - t0) * (G::TILE / 16);
+  int64_t g = div_up(work, 256);
+  if (g > 65536) g = 65536;
+  encode_packed_kernel<E, DPAD><<<(unsigned)g, 256, 0, st>>>(x, n, d, ldx, t0, t1, out, scales);
+  RPLSH_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+int64_t packed_tile_bytes(Elem e, int dpad) {
+  return e == Elem::BF16 ? (int64_t)32 * dpad * 2 : (int64_t)32 * dpad;
+}
+
+int filter_packed_block_queries(Elem e, int dpad) {
+  int tn = 1;
+  if (e == Elem::BF16) {
+    switch (dpad) {
+      case 32: tn = packed_tn<Elem::BF16, 32>(); break;
+      case 64: tn = packed_tn<Elem::BF16, 64>(); break;
+      case 96: tn = packed_tn<Elem::BF16, 96>(); break;
+      case 128: tn = packed_tn<Elem::BF16, 128>(); break;
+      case 160: tn = packed_tn<Elem::BF16, 160>(); break;
+      case 192: tn = packed_tn<Elem::BF16, 192>(); break;
+      case 256: tn = packed_tn<Elem::BF16, 256>(); break;
+      default: break;
+    }
+  } else if (e == Elem::FP8) {
+    switch (dpad) {
+      case 64: tn = packed_tn<Elem::FP8, 64>(); break;
+      case 128: tn = packed_tn<Elem::FP8, 128>(); break;
+      case 256: tn = packed_tn<Elem::FP8, 256>(); break;
+      default: break;
+    }
+  }
+  return 4 * 32 * tn;
+}
+
+void launch_filter_packed(Elem e, int dpad, const FilterArgs& a, hipStream_t st) {
+  if (e == Elem::BF16) {
+    switch (dpad) {
+      case 32: return launch_packed_one<Elem::BF16, 32>(a, st);
+      case 64: return launch_packed_one<Elem::BF16, 64>(a, st);
+      case 96: return launch_packed_one<Elem::BF16, 96>(a, st);
+      case 128: return launch_packed_one<Elem::BF16, 128>(a, st);
+      case 160: return launch_packed_one<Elem::BF16, 160>(a, st);
+      case 192: return launch_packed_one<Elem::BF16, 192>(a, st);
+      case 256: return launch_packed_one<Elem::BF16, 256>(a, st);
+      default: break;
+    }
+  } else if (e == Elem::FP8) {
+    switch (dpad) {
+      case 64: return launch_packed_one<Elem::FP8, 64>(a, st);
+      case 128: return launch_packed_one<Elem::FP8, 128>(a, st);
+      case 256: return launch_packed_one<Elem::FP8, 256>(a, st);
+      default: break;
+    }
+  }
+  fail(RPLSH_ERR_UNSUPPORTED, "packed filter: element kind / dimension %d not instantiated", dpad);
+}
+
+void launch_encode_packed(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx,
+                          int64_t row_from, void* out, float* scales, hipStream_t st) {
+  char* o = static_cast<char*>(out);
+  if (e == Elem::BF16) {
+    switch (dpad) {
+      case 32: return encode_one<Elem::BF16, 32>(x, n, d, ldx, row_from, o, scales, st);
+      case 64: return encode_one<Elem::BF16, 64>(x, n, d, ldx, row_from, o, scales, st);
+      case 96: return encode_one<Elem::BF16, 96>(x, n, d, ldx, row_from, o, scales, st);
+      case 128: return encode_one<Elem::BF16, 128>(x, n, d, ldx, row_from, o, scales, st);
+      case 160: return encode_one<Elem::BF16, 160>(x, n, d, ldx, row_from, o, scales, st);
+      case 192: return encode_one<Elem::BF16, 192>(x, n, d, ldx, row_from, o, scales, st);
+      case 256: return encode_one<Elem::BF16, 256>(x, n, d, ldx, row_from, o, scales, st);
+      default: break;
+    }
+  } else if (e == Elem::FP8) {
+    switch (dpad) {
+      case 64: return encode_one<Elem::FP8, 64>(x, n, d, ldx, row_from, o, scales, st);
+      case 128: return encode_one<Elem::FP8, 128>(x, n, d, ldx, row_from, o, scales, st);
+      case 256: return encode_one<Elem::FP8, 256>(x, n, d, ldx, row_from, o, scales, st);
+      default: break;
+    }
+  }
+  fail(RPLSH_ERR_UNSUPPORTED, "packed encoder: dimension %d not instantiated", dpad);
+}
+
+}  // namespace rplsh

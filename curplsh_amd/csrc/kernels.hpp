@@ -34,6 +34,15 @@ int filter_dpad(Elem e, int d);
 int filter_block_queries(Elem e, int dpad);
 void launch_filter(Elem e, int dpad, const FilterArgs& a, hipStream_t st);
 
+// filter_packed.hip: the BF16 / FP8 candidate copies in 32-row MFMA tiles.
+int64_t packed_tile_bytes(Elem e, int dpad);
+int filter_packed_block_queries(Elem e, int dpad);
+void launch_filter_packed(Elem e, int dpad, const FilterArgs& a, hipStream_t st);
+// Encodes rows [row_from, n) (re-encoding the partial tile before row_from);
+// FP8 also writes the per-row power-of-two scales of rows [row_from, n).
+void launch_encode_packed(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx,
+                          int64_t row_from, void* out, float* scales, hipStream_t st);
+
 // select.hip
 void launch_select(const SelectArgs& a, hipStream_t st);
 void launch_rerank(const RerankArgs& a, hipStream_t st);

@@ -32,6 +32,7 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "engine.hpp"
 #include "kernels.hpp"
 #include "rplsh.h"
 
@@ -41,90 +42,7 @@ namespace {
 
 constexpr float kUnitRoundoff = 5.9604645e-08f;  // 2^-24
 
-// Growable device buffer (hipMalloc only when a call needs more than before).
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
-    o.p = nullptr;
-    o.bytes = 0;
-  }
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  void* ensure(size_t b) {
-    if (b <= bytes) return p;
-    if (p) RPLSH_HIP(hipFree(p));
-    p = nullptr;
-    bytes = 0;
-    size_t nb = std::max(b, (size_t)256);
-    RPLSH_HIP(hipMalloc(&p, nb));
-    bytes = nb;
-    return p;
-  }
-  template <typename T>
-  T* as(size_t count) {
-    return static_cast<T*>(ensure(count * sizeof(T)));
-  }
-};
-
-// Events around the phases of a batch (depth 0 only), read after the sync.
-struct PhaseEvents {
-  hipEvent_t ev[8] = {};
-  bool on = false;
-  void create() {
-    for (auto& e : ev) RPLSH_HIP(hipEventCreate(&e));
-    on = true;
-  }
-  ~PhaseEvents() {
-    if (on)
-      for (auto e : ev) (void)hipEventDestroy(e);
-  }
-  void rec(int i, hipStream_t st) {
-    if (on) RPLSH_HIP(hipEventRecord(ev[i], st));
-  }
-  float ms(int a, int b) const {
-    float t = 0.0f;
-    if (on) RPLSH_HIP(hipEventElapsedTime(&t, ev[a], ev[b]));
-    return t;
-  }
-};
-
-struct Workspace {
-  DevBuf qnorm, qenc, qscale, qpad, tau, tau2, counts, cand, topk, bound, status, relk;
-  DevBuf flist, fcount, fp64, surv;
-  DevBuf qsub, isub, dsub;  // a retried subset of queries
-};
-
 }  // namespace
-
-// The database as the search kernels see it.
-struct DbView {
-  const float* x = nullptr;  // FP32 rows [n][ldx]
-  int64_t ldx = 0;
-  int64_t n = 0;
-  int d = 0;
-  const float* xnorm = nullptr;
-  float xn_max = 0.0f;
-  Elem elem = Elem::F32;
-  int dpad = 0;
-  const void* xenc = nullptr;  // candidate rows [n][ldenc] of elem
-  int64_t ldenc = 0;
-  int dvalid_bytes = 0;
-  const float* xscale = nullptr;  // FP8 per-row scales
-};
-
-struct SearchParams {
-  int k = 0;
-  int kp = 0;          // candidates re-ranked per query
-  int exact = 1;
-  int fp64_boundary = 0;
-  bool certify = false;
-  int c_factor = 8;    // last-level target survivors = c_factor * kp
-};
 
 struct Level {
   int64_t stride;
@@ -196,7 +114,8 @@ static void run_filter(const DbView& db, const Level& lv, const void* qenc, cons
   fa.qscale = qscale;
   fa.tau = tau;
   fa.nq = nq;
-  const int bq = filter_block_queries(db.elem, db.dpad);
+  const int bq = db.packed ? filter_packed_block_queries(db.elem, db.dpad)
+                           : filter_block_queries(db.elem, db.dpad);
   fa.num_qtiles = (int)div_up(nq, bq);
   const int64_t target_wgs = 4096;
   int64_t slices = std::max<int64_t>(1, div_up(target_wgs, fa.num_qtiles));
@@ -208,13 +127,17 @@ static void run_filter(const DbView& db, const Level& lv, const void* qenc, cons
   fa.counts = counts;
   fa.cand = cand;
   launch_fill_i32(counts, nq, 0, st);
-  launch_filter(db.elem, db.dpad, fa, st);
+  if (db.packed) {
+    launch_filter_packed(db.elem, db.dpad, fa, st);
+  } else {
+    launch_filter(db.elem, db.dpad, fa, st);
+  }
 }
 
 // One batch of nq device-resident FP32 queries [nq][d] -> device outputs.
-static void search_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
-                         int32_t* out_i, float* out_d, std::vector<Workspace>& wss, int depth,
-                         hipStream_t st, rplsh_search_stats& stats) {
+void search_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
+                  int32_t* out_i, float* out_d, std::vector<Workspace>& wss, int depth,
+                  hipStream_t st, rplsh_search_stats& stats) {
   if (nq <= 0) return;
   Workspace& ws = wss.at(depth);
   const int d = db.d;
@@ -288,13 +211,13 @@ static void search_batch(const DbView& db, const float* q, int nq, const SearchP
   }
   {
     long long* sv = ws.surv.as<long long>(1);
-    RPLSH_HIP(hipMemsetAsync(sv, 0, sizeof(long long), st));
+    launch_fill_i32(reinterpret_cast<int*>(sv), 2, 0, st);
     launch_sum_counts(counts, nq, cap, sv, st);
   }
 
   // 3. re-rank
   int* fp64c = ws.fp64.as<int>(1);
-  RPLSH_HIP(hipMemsetAsync(fp64c, 0, sizeof(int), st));
+  launch_fill_i32(fp64c, 1, 0, st);
   RerankArgs ra{};
   ra.x = db.x;
   ra.ldx = db.ldx;
@@ -325,7 +248,7 @@ static void search_batch(const DbView& db, const float* q, int nq, const SearchP
   // 4. failed queries
   int* flist = ws.flist.as<int>(nq);
   int* fcount = ws.fcount.as<int>(1);
-  RPLSH_HIP(hipMemsetAsync(fcount, 0, sizeof(int), st));
+  launch_fill_i32(fcount, 1, 0, st);
   launch_collect_failed(status, nq, flist, fcount, st);
   int nfail = 0, nfp64 = 0;
   long long nsurv = 0;
@@ -364,6 +287,19 @@ static void search_batch(const DbView& db, const float* q, int nq, const SearchP
 }
 
 static int plain_kprime(int k) { return std::max(2 * k, k + 32); }
+
+size_t search_workspace_bytes(const DbView& db, int nq, const SearchParams& p) {
+  const int kp = (int)std::min<int64_t>(p.kp, db.n);
+  const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor);
+  const size_t cap = (size_t)level_cap(L, kp, p.c_factor);
+  const size_t eb = db.elem == Elem::F32 ? 4 : (db.elem == Elem::BF16 ? 2 : 1);
+  size_t b = 0;
+  b += (size_t)nq * 4 * 9;                 // norms, scales, tau x2, counts, bound, status, relk, list
+  b += (size_t)nq * db.dpad * eb;          // encoded queries
+  b += (size_t)nq * cap * 8;               // candidate words
+  b += (size_t)nq * kp * 8;                // top-k' words
+  return b + 4096;
+}
 
 }  // namespace rplsh
 
@@ -425,25 +361,19 @@ static void bf_encode(rplsh_bf* b, int64_t from, hipStream_t st) {
     b->xenc = nullptr;
     b->xscale = nullptr;
     if (b->capacity > 0) {
-      RPLSH_HIP(hipMalloc(&b->xenc, (size_t)b->capacity * dpad * elem_bytes(e)));
-      if (e == Elem::FP8) RPLSH_HIP(hipMalloc((void**)&b->xscale, (size_t)b->capacity * sizeof(float)));
+      const int64_t tiles = div_up(b->capacity, 32);
+      RPLSH_HIP(hipMalloc(&b->xenc, (size_t)(tiles * packed_tile_bytes(e, dpad))));
+      if (e == Elem::FP8) RPLSH_HIP(hipMalloc((void**)&b->xscale, (size_t)tiles * 32 * sizeof(float)));
     }
     b->enc_precision = b->cfg.precision;
     b->dpad = dpad;
     from = 0;
   }
-  const int64_t cnt = b->n - from;
-  if (cnt <= 0) return;
-  const float* src = b->x + from * b->ldx;
-  if (e == Elem::BF16) {
-    launch_encode_bf16(src, cnt, b->dim, b->ldx, dpad, static_cast<uint16_t*>(b->xenc) + from * dpad, st);
-  } else {
-    launch_encode_e4m3(src, cnt, b->dim, b->ldx, dpad, static_cast<uint8_t*>(b->xenc) + from * dpad,
-                       b->xscale + from, st);
-  }
+  if (b->n - from <= 0) return;
+  launch_encode_packed(e, dpad, b->x, b->n, b->dim, b->ldx, from, b->xenc, b->xscale, st);
 }
 
-static DbView bf_view(rplsh_bf* b) {
+static DbView bf_view(const rplsh_bf* b) {
   DbView v;
   v.x = b->x;
   v.ldx = b->ldx;
@@ -462,6 +392,7 @@ static DbView bf_view(rplsh_bf* b) {
     v.ldenc = b->dpad;
     v.dvalid_bytes = b->dpad * elem_bytes(v.elem);
     v.xscale = b->xscale;
+    v.packed = true;
   }
   return v;
 }
@@ -479,6 +410,38 @@ static void bf_free_data(rplsh_bf* b) {
   b->xn_max = 0.0f;
   b->enc_precision = -1;
 }
+
+namespace rplsh {
+
+DbView bf_db_view(const rplsh_bf* b) { return bf_view(b); }
+
+SearchParams bf_search_params(const rplsh_bf* b, int k) {
+  SearchParams p;
+  p.k = k;
+  p.exact = 1;  // IndexBF::search always asks for exact distances (index_bf.cpp:68-69)
+  if (elem_of(b->cfg.precision) == Elem::F32) {
+    p.kp = plain_kprime(k);
+    p.certify = true;
+    p.fp64_boundary = 0;
+  } else {
+    p.kp = k * b->cfg.kprime_factor;
+    p.certify = false;
+    p.fp64_boundary = b->cfg.fp64_boundary;
+  }
+  return p;
+}
+
+int bf_device_of(const rplsh_bf* b) { return b->cfg.device; }
+
+void bf_prepare(rplsh_bf* b) {
+  if (b->enc_precision == b->cfg.precision || b->n == 0) return;
+  DeviceScope scope(b->cfg.device);
+  hipStream_t st = bf_stream(b);
+  bf_encode(b, 0, st);
+  RPLSH_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace rplsh
 
 extern "C" {
 
@@ -645,18 +608,7 @@ rplsh_status rplsh_bf_search(rplsh_bf* b, int64_t nq, const float* queries, int 
     } else {
       if (b->enc_precision != b->cfg.precision) bf_encode(b, 0, st);
       const DbView v = bf_view(b);
-      SearchParams p;
-      p.k = k;
-      p.exact = 1;  // IndexBF::search always asks for exact distances (index_bf.cpp:68-69)
-      if (v.elem == Elem::F32) {
-        p.kp = plain_kprime(k);
-        p.certify = true;
-        p.fp64_boundary = 0;
-      } else {
-        p.kp = k * b->cfg.kprime_factor;
-        p.certify = false;
-        p.fp64_boundary = b->cfg.fp64_boundary;
-      }
+      const SearchParams p = bf_search_params(b, k);
       RPLSH_REQUIRE(std::min<int64_t>(p.kp, b->n) <= 4096, "k' = %d exceeds 4096", p.kp);
       search_batch(v, qd, (int)nq, p, id, dd, b->ws, 0, st, stats);
     }

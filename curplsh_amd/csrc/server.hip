@@ -1,0 +1,448 @@
+// Serving form: continuous batching over resident indexes.
+//
+// Each of the `slots` workers owns a HIP stream (default priority) and a
+// search workspace. An idle worker takes the oldest queued request, adds the
+// queued requests for the same (index, k) behind it while the batch stays
+// within max_batch_queries, and admits the batch when its workspace fits in
+// the HBM budget left by the batches in flight; otherwise it waits until a
+// finishing batch releases memory. Batches on different slots run
+// concurrently on the device. Because every query is filtered, selected and
+// re-ranked independently of the others in its batch, a request's result is
+// the same alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cfloat>
+#include <string>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+#include "engine.hpp"
+#include "rplsh.h"
+
+using namespace rplsh;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Request {
+  uint64_t id = 0;
+  int index = 0;
+  int64_t nq = 0;
+  const float* q = nullptr;
+  int k = 0;
+  int32_t* I = nullptr;
+  float* D = nullptr;
+  Clock::time_point t_submit, t_admit, t_done;
+  bool done = false;
+  rplsh_request_info info{};
+  std::string err;
+};
+
+struct Slot {
+  hipStream_t st = nullptr;
+  std::vector<Workspace> wss;
+  DevBuf qbuf, ibuf, dbuf;
+  std::thread th;
+};
+
+}  // namespace
+
+struct rplsh_server {
+  rplsh_server_config cfg{};
+  std::vector<rplsh_bf*> indexes;
+  std::mutex mu;
+  std::condition_variable cv;       // workers: queue / budget changes
+  std::condition_variable done_cv;  // waiters: completions
+  std::deque<Request*> queue;
+  std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+  uint64_t next_id = 1;
+  size_t reserved = 0;
+  int inflight = 0;
+  bool stop = false;
+  std::vector<std::unique_ptr<Slot>> slots;
+  // statistics
+  std::vector<double> lat;
+  int64_t submitted = 0, completed = 0, rejected = 0, queries = 0, batches = 0;
+  size_t peak_reserved = 0;
+  int peak_inflight = 0;
+  bool have_first = false;
+  Clock::time_point first_submit, last_done;
+};
+
+namespace {
+
+size_t batch_bytes(rplsh_bf* b, int64_t nq, int k) {
+  const DbView v = bf_db_view(b);
+  const SearchParams p = bf_search_params(b, k);
+  return search_workspace_bytes(v, (int)nq, p) + (size_t)nq * (v.d * 4 + (size_t)k * 8);
+}
+
+void finish(rplsh_server* s, Request* r, int status) {
+  r->t_done = Clock::now();
+  r->done = true;
+  r->info.status = status;
+  r->info.latency_ms = ms_between(r->t_submit, r->t_done);
+  r->info.queue_ms = ms_between(r->t_submit, r->t_admit);
+  if (status == RPLSH_OK) {
+    s->completed += 1;
+    s->queries += r->nq;
+    s->lat.push_back(r->info.latency_ms);
+  } else if (status == RPLSH_ERR_BUDGET) {
+    s->rejected += 1;
+  }
+  s->last_done = r->t_done;
+}
+
+// Runs one batch of requests on a slot (called without the lock).
+int run_batch(rplsh_server* s, Slot& sl, const std::vector<Request*>& batch, std::string& err) {
+  try {
+    rplsh_bf* b = s->indexes[batch[0]->index];
+    const int k = batch[0]->k;
+    const DbView v = bf_db_view(b);
+    const SearchParams p = bf_search_params(b, k);
+    int64_t total = 0;
+    for (auto* r : batch) total += r->nq;
+    float* q = sl.qbuf.as<float>((size_t)total * v.d);
+    int32_t* I = sl.ibuf.as<int32_t>((size_t)total * k);
+    float* D = sl.dbuf.as<float>((size_t)total * k);
+    int64_t off = 0;
+    for (auto* r : batch) {
+      RPLSH_HIP(hipMemcpyAsync(q + off * v.d, r->q, (size_t)r->nq * v.d * sizeof(float),
+                               hipMemcpyHostToDevice, sl.st));
+      off += r->nq;
+    }
+    if (v.n == 0) {
+      // Empty index: FLT_MAX / -1 (search.cu:93-100).
+      launch_fill_i32(I, total * k, -1, sl.st);
+      launch_fill_f32(D, total * k, FLT_MAX, sl.st);
+    } else {
+      rplsh_search_stats stats{};
+      search_batch(v, q, (int)total, p, I, D, sl.wss, 0, sl.st, stats);
+    }
+    off = 0;
+    for (auto* r : batch) {
+      RPLSH_HIP(hipMemcpyAsync(r->I, I + off * k, (size_t)r->nq * k * sizeof(int32_t),
+                               hipMemcpyDeviceToHost, sl.st));
+      RPLSH_HIP(hipMemcpyAsync(r->D, D + off * k, (size_t)r->nq * k * sizeof(float),
+                               hipMemcpyDeviceToHost, sl.st));
+      off += r->nq;
+    }
+    RPLSH_HIP(hipStreamSynchronize(sl.st));
+    return RPLSH_OK;
+  } catch (const Error& e) {
+    err = e.what();
+    return e.status;
+  } catch (const std::exception& e) {
+    err = e.what();
+    return RPLSH_ERR_STATE;
+  }
+}
+
+void worker(rplsh_server* s, Slot* sl) {
+  (void)hipSetDevice(s->cfg.device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    std::vector<Request*> batch;
+    size_t need = 0;
+    for (;;) {
+      if (s->queue.empty()) {
+        if (s->stop) return;
+        s->cv.wait(lk);
+        continue;
+      }
+      Request* head = s->queue.front();
+      rplsh_bf* b = s->indexes[head->index];
+      size_t hb = 0;
+      try {
+        hb = batch_bytes(b, head->nq, head->k);
+      } catch (...) {
+        hb = SIZE_MAX;
+      }
+      if (hb > s->cfg.hbm_budget_bytes) {
+        // Can never fit: complete it with a budget error.
+        s->queue.pop_front();
+        head->t_admit = Clock::now();
+        head->err = "batch workspace exceeds the HBM budget";
+        finish(s, head, RPLSH_ERR_BUDGET);
+        s->done_cv.notify_all();
+        continue;
+      }
+      if (s->reserved + hb > s->cfg.hbm_budget_bytes) {
+        s->cv.wait(lk);  // a finishing batch will release memory
+        continue;
+      }
+      // Admit the head and the same-(index, k) requests queued behind it.
+      s->queue.pop_front();
+      batch.push_back(head);
+      int64_t total = head->nq;
+      need = hb;
+      for (auto it = s->queue.begin(); it != s->queue.end();) {
+        Request* r = *it;
+        if (r->index == head->index && r->k == head->k &&
+            total + r->nq <= s->cfg.max_batch_queries) {
+          const size_t nb = batch_bytes(b, total + r->nq, head->k);
+          if (s->reserved + nb <= s->cfg.hbm_budget_bytes) {
+            batch.push_back(r);
+            total += r->nq;
+            need = nb;
+            it = s->queue.erase(it);
+            continue;
+          }
+        }
+        ++it;
+      }
+      break;
+    }
+    const auto now = Clock::now();
+    for (auto* r : batch) {
+      r->t_admit = now;
+      r->info.batch_requests = (int)batch.size();
+    }
+    int64_t total = 0;
+    for (auto* r : batch) total += r->nq;
+    for (auto* r : batch) r->info.batch_queries = total;
+    s->reserved += need;
+    s->inflight += 1;
+    s->batches += 1;
+    s->peak_reserved = std::max(s->peak_reserved, s->reserved);
+    s->peak_inflight = std::max(s->peak_inflight, s->inflight);
+    lk.unlock();
+    std::string err;
+    const int st = run_batch(s, *sl, batch, err);
+    lk.lock();
+    s->reserved -= need;
+    s->inflight -= 1;
+    for (auto* r : batch) {
+      r->err = err;
+      finish(s, r, st);
+    }
+    s->done_cv.notify_all();
+    s->cv.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void rplsh_server_config_default(rplsh_server_config* cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->device = 0;
+  cfg->slots = 4;
+  cfg->hbm_budget_bytes = 0;
+  cfg->max_batch_queries = 4096;
+}
+
+rplsh_status rplsh_server_create(const rplsh_server_config* cfg, rplsh_server** out) {
+  return guard([&] {
+    RPLSH_REQUIRE(out, "out is NULL");
+    rplsh_server_config c;
+    rplsh_server_config_default(&c);
+    if (cfg) c = *cfg;
+    if (c.slots <= 0) c.slots = 4;
+    if (c.max_batch_queries <= 0) c.max_batch_queries = 4096;
+    int count = 0;
+    rplsh_device_count(&count);
+    RPLSH_REQUIRE(c.device >= 0 && c.device < count, "device %d out of range [0, %d)", c.device, count);
+    DeviceScope scope(c.device);
+    if (c.hbm_budget_bytes == 0) {
+      size_t fr = 0, tot = 0;
+      RPLSH_HIP(hipMemGetInfo(&fr, &tot));
+      c.hbm_budget_bytes = fr / 4;
+    }
+    auto* s = new rplsh_server();
+    s->cfg = c;
+    try {
+      for (int i = 0; i < c.slots; ++i) {
+        auto sl = std::make_unique<Slot>();
+        RPLSH_HIP(hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking));
+        sl->wss.resize(4);
+        s->slots.push_back(std::move(sl));
+      }
+      for (auto& sl : s->slots) sl->th = std::thread(worker, s, sl.get());
+    } catch (...) {
+      for (auto& sl : s->slots)
+        if (sl->st) (void)hipStreamDestroy(sl->st);
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+rplsh_status rplsh_server_destroy(rplsh_server* s) {
+  return guard([&] {
+    if (!s) return;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop = true;
+    }
+    s->cv.notify_all();
+    for (auto& sl : s->slots)
+      if (sl->th.joinable()) sl->th.join();
+    DeviceScope scope(s->cfg.device);
+    for (auto& sl : s->slots) {
+      sl->wss.clear();
+      sl->qbuf = DevBuf();
+      sl->ibuf = DevBuf();
+      sl->dbuf = DevBuf();
+      (void)hipStreamDestroy(sl->st);
+    }
+    delete s;
+  });
+}
+
+rplsh_status rplsh_server_add_index(rplsh_server* s, rplsh_bf* index, int* index_id) {
+  return guard([&] {
+    RPLSH_REQUIRE(s && index && index_id, "NULL argument");
+    RPLSH_REQUIRE(bf_device_of(index) == s->cfg.device, "index lives on another device");
+    bf_prepare(index);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->indexes.push_back(index);
+    *index_id = (int)s->indexes.size() - 1;
+  });
+}
+
+rplsh_status rplsh_server_submit(rplsh_server* s, int index_id, int64_t nq, const float* queries,
+                                 int k, int32_t* indices, float* distances, uint64_t* request) {
+  return guard([&] {
+    RPLSH_REQUIRE(s && request, "NULL argument");
+    RPLSH_REQUIRE(nq > 0 && queries && indices && distances, "empty request or NULL buffer");
+    RPLSH_REQUIRE(k > 0, "k must be > 0");
+    std::lock_guard<std::mutex> lk(s->mu);
+    RPLSH_REQUIRE(index_id >= 0 && index_id < (int)s->indexes.size(), "unknown index %d", index_id);
+    RPLSH_REQUIRE(!s->stop, "server is stopping");
+    auto r = std::make_unique<Request>();
+    r->id = s->next_id++;
+    r->index = index_id;
+    r->nq = nq;
+    r->q = queries;
+    r->k = k;
+    r->I = indices;
+    r->D = distances;
+    r->t_submit = Clock::now();
+    if (!s->have_first) {
+      s->have_first = true;
+      s->first_submit = r->t_submit;
+    }
+    s->submitted += 1;
+    *request = r->id;
+    s->queue.push_back(r.get());
+    s->reqs.emplace(r->id, std::move(r));
+    s->cv.notify_one();
+  });
+}
+
+rplsh_status rplsh_server_poll(rplsh_server* s, uint64_t request, int* done) {
+  return guard([&] {
+    RPLSH_REQUIRE(s && done, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(request);
+    RPLSH_REQUIRE(it != s->reqs.end(), "unknown request %llu", (unsigned long long)request);
+    *done = it->second->done ? 1 : 0;
+  });
+}
+
+rplsh_status rplsh_server_wait(rplsh_server* s, uint64_t request, double timeout_s,
+                               rplsh_request_info* info) {
+  rplsh_status st = guard([&] {
+    RPLSH_REQUIRE(s, "server is NULL");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(request);
+    RPLSH_REQUIRE(it != s->reqs.end(), "unknown request %llu", (unsigned long long)request);
+    Request* r = it->second.get();
+    auto pred = [&] { return r->done; };
+    if (timeout_s < 0) {
+      s->done_cv.wait(lk, pred);
+    } else if (!s->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) {
+      fail(RPLSH_ERR_STATE, "request %llu not done after %.3f s", (unsigned long long)request, timeout_s);
+    }
+    if (info) *info = r->info;
+    const int rs = r->info.status;
+    const std::string err = r->err;
+    s->reqs.erase(it);
+    if (rs != RPLSH_OK)
+      fail((rplsh_status)rs, "request %llu failed: %s", (unsigned long long)request, err.c_str());
+  });
+  return st;
+}
+
+rplsh_status rplsh_server_drain(rplsh_server* s) {
+  return guard([&] {
+    RPLSH_REQUIRE(s, "server is NULL");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->done_cv.wait(lk, [&] { return s->queue.empty() && s->inflight == 0; });
+  });
+}
+
+rplsh_status rplsh_server_get_stats(rplsh_server* s, rplsh_server_stats* out) {
+  return guard([&] {
+    RPLSH_REQUIRE(s && out, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(out, 0, sizeof(*out));
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->rejected = s->rejected;
+    out->queries = s->queries;
+    out->batches = s->batches;
+    out->peak_reserved_bytes = s->peak_reserved;
+    out->peak_inflight = s->peak_inflight;
+    if (!s->lat.empty()) {
+      std::vector<double> v = s->lat;
+      std::sort(v.begin(), v.end());
+      auto pct = [&](double p) {
+        size_t i = (size_t)std::ceil(p * (double)v.size()) - 1;
+        return v[std::min(i, v.size() - 1)];
+      };
+      out->p50_ms = pct(0.50);
+      out->p99_ms = pct(0.99);
+      double sum = 0;
+      for (double x : v) sum += x;
+      out->mean_ms = sum / (double)v.size();
+      out->max_ms = v.back();
+    }
+    if (s->have_first && s->completed > 0) {
+      out->wall_s = ms_between(s->first_submit, s->last_done) / 1000.0;
+      out->qps = out->wall_s > 0 ? (double)s->queries / out->wall_s : 0.0;
+    }
+  });
+}
+
+rplsh_status rplsh_server_reset_stats(rplsh_server* s) {
+  return guard([&] {
+    RPLSH_REQUIRE(s, "server is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->lat.clear();
+    s->submitted = s->completed = s->rejected = s->queries = s->batches = 0;
+    s->peak_reserved = 0;
+    s->peak_inflight = 0;
+    s->have_first = false;
+  });
+}
+
+rplsh_status rplsh_bf_batch_bytes(rplsh_bf* index, int64_t nq, int k, size_t* bytes) {
+  return guard([&] {
+    RPLSH_REQUIRE(index && bytes, "NULL argument");
+    RPLSH_REQUIRE(nq >= 0 && k > 0, "bad arguments");
+    bf_prepare(index);
+    *bytes = batch_bytes(index, nq, k);
+  });
+}
+
+}  // extern "C"

@@ -201,6 +201,62 @@ typedef struct rplsh_search_stats {
 } rplsh_search_stats;
 rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out);
 
+/* ----------------------------------------------------------- serving -- */
+/* Continuous-batching request loop over resident indexes (no reference
+ * counterpart). `slots` worker threads each own a HIP stream and a workspace;
+ * an idle worker admits the oldest queued request -- plus the queued
+ * requests for the same index and k behind it, up to max_batch_queries --
+ * when the batch's workspace fits in what is left of the HBM budget, so
+ * requests arriving while others run are admitted as slots and memory free
+ * up, and their kernels overlap on the device. A request whose workspace can
+ * never fit is completed with RPLSH_ERR_BUDGET. Results equal those of
+ * rplsh_bf_search on the same queries. Registered indexes must not be
+ * modified (add/reset/set_precision) while the server runs. */
+typedef struct rplsh_server rplsh_server;
+
+typedef struct rplsh_server_config {
+  int device;
+  int slots;               /* concurrent batches (default 4) */
+  size_t hbm_budget_bytes; /* admission budget for batch workspaces (0 = 1/4 of free HBM) */
+  int max_batch_queries;   /* co-batching limit (default 4096) */
+} rplsh_server_config;
+
+typedef struct rplsh_request_info {
+  int status;           /* rplsh_status of the request itself */
+  double latency_ms;    /* submit -> results in place */
+  double queue_ms;      /* submit -> admitted */
+  int batch_requests;   /* requests in the batch it ran in */
+  int64_t batch_queries;
+} rplsh_request_info;
+
+typedef struct rplsh_server_stats {
+  int64_t submitted, completed, rejected, queries, batches;
+  double p50_ms, p99_ms, mean_ms, max_ms; /* latency of completed requests */
+  double wall_s;                          /* first submit -> last completion */
+  double qps;                             /* completed queries / wall_s */
+  size_t peak_reserved_bytes;             /* peak sum of admitted batch workspaces */
+  int peak_inflight;                      /* peak concurrent batches */
+} rplsh_server_stats;
+
+void rplsh_server_config_default(rplsh_server_config* cfg);
+rplsh_status rplsh_server_create(const rplsh_server_config* cfg, rplsh_server** out);
+/* Drains the queue, stops the workers. */
+rplsh_status rplsh_server_destroy(rplsh_server* s);
+/* Registers a resident index (not owned); its id is returned. */
+rplsh_status rplsh_server_add_index(rplsh_server* s, rplsh_bf* index, int* index_id);
+/* Enqueues a request; host pointers, kept alive by the caller until it completes. */
+rplsh_status rplsh_server_submit(rplsh_server* s, int index_id, int64_t nq, const float* queries,
+                                 int k, int32_t* indices, float* distances, uint64_t* request);
+rplsh_status rplsh_server_poll(rplsh_server* s, uint64_t request, int* done);
+/* Waits up to timeout_s (< 0: forever) for a request; RPLSH_ERR_STATE on timeout. */
+rplsh_status rplsh_server_wait(rplsh_server* s, uint64_t request, double timeout_s,
+                               rplsh_request_info* info);
+rplsh_status rplsh_server_drain(rplsh_server* s);
+rplsh_status rplsh_server_get_stats(rplsh_server* s, rplsh_server_stats* out);
+rplsh_status rplsh_server_reset_stats(rplsh_server* s);
+/* Device bytes one batch of nq queries with this k needs on an index. */
+rplsh_status rplsh_bf_batch_bytes(rplsh_bf* index, int64_t nq, int k, size_t* bytes);
+
 /* --------------------------------------------------- synthetic data -- */
 /* Counter-based, thread-count independent generators (splitmix64 hashing +
  * Box-Muller in double); identical output on every x86-64 host running the
