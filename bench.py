@@ -61,6 +61,13 @@ def main() -> int:
     ap.add_argument("--recall-queries", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--mode", default="headline", choices=["headline", "serving"],
+                    help="headline: the metric's workload; serving: the 64-index request loop")
+    ap.add_argument("--rate", type=float, default=400.0, help="serving: Poisson arrival rate (req/s)")
+    ap.add_argument("--requests", type=int, default=4096, help="serving: number of requests")
+    ap.add_argument("--indexes", type=int, default=64, help="serving: resident indexes")
+    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent batches")
+    ap.add_argument("--serving-precision", default="bf16", choices=["bf16", "fp8", "fp32"])
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -73,6 +80,8 @@ def main() -> int:
     if ca.device_count() < 1:
         log("no GPU visible")
         return 1
+    if args.mode == "serving":
+        return serving(args, ca)
     n, nq, d, k = args.n, args.nq, args.d, args.k
     log(f"[bench] generating {n}x{d} database and {nq} queries")
     t0 = time.time()
@@ -226,6 +235,92 @@ def main() -> int:
     if "fp32" in results and form != "fp32":
         line["speedup_vs_plain_fp32"] = round(head["qps"] / results["fp32"]["qps"], 3)
     line.update(extra)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def serving(args, ca) -> int:
+    """64 resident indexes, n log-uniform in [1e4, 2e6], d from {64,96,128,256},
+    U[-1,1); requests with Poisson arrivals at --rate, each to a random index
+    with nq log-uniform in [1, 2048] and k from {1, 10, 100}; continuous
+    batching with HBM-budget admission (rplsh_server)."""
+    import threading
+
+    rng = np.random.default_rng(12345)
+    res = ca.DeviceResources(0)
+    idxs, dims = [], []
+    t0 = time.time()
+    total_rows = 0
+    for i in range(args.indexes):
+        n = int(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        d = int(rng.choice([64, 96, 128, 256]))
+        x = ca.gen_uniform(n, d, 70_000 + i, -1.0, 1.0)
+        ix = ca.IndexBF(res, d, ca.IndexBFConfig(precision=args.serving_precision))
+        ix.add(x)
+        del x
+        idxs.append(ix)
+        dims.append(d)
+        total_rows += n
+    log(f"[serving] {args.indexes} indexes, {total_rows} rows, built in {time.time() - t0:.1f}s")
+    reqs = []
+    for r in range(args.requests):
+        i = int(rng.integers(args.indexes))
+        nq = int(np.clip(np.floor(np.exp(rng.uniform(0.0, np.log(2049.0)))), 1, 2048))
+        k = int(rng.choice([1, 10, 100]))
+        reqs.append((i, ca.gen_uniform(nq, dims[i], 900_000 + r, -1.0, 1.0), k))
+    gaps = rng.exponential(1.0 / args.rate, size=args.requests)
+    srv = ca.Server(slots=args.slots)
+    ids = [srv.add_index(ix) for ix in idxs]
+    # warm-up: one small request per index (workspaces, code paths)
+    for i in range(args.indexes):
+        srv.wait(srv.submit(ids[i], ca.gen_uniform(1, dims[i], 5, -1.0, 1.0), 10), timeout_s=120.0)
+    srv.reset_stats()
+    handles = [None] * len(reqs)
+
+    def submitter():
+        nxt = time.perf_counter()
+        for j, (i, q, k) in enumerate(reqs):
+            nxt += gaps[j]
+            dt = nxt - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            handles[j] = srv.submit(ids[i], q, k)
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    infos = []
+    for h in handles:
+        infos.append(srv.wait(h, timeout_s=600.0)[2])
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    nqs = sum(q.shape[0] for _, q, _ in reqs)
+    log(f"[serving] stats {st}")
+    line = {
+        "metric": "serving QPS, 64 resident indexes, Poisson arrivals",
+        "value": round(st["qps"], 1),
+        "unit": "QPS",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": args.serving_precision,
+        "data": "synthetic: seeded U[-1,1) indexes and queries",
+        "config": {"workload": f"{args.indexes} indexes (n log-uniform 1e4..2e6, d in 64/96/128/256), "
+                               f"{args.requests} requests, nq log-uniform 1..2048, k in 1/10/100",
+                   "rate_rps": args.rate, "slots": args.slots, "rows": total_rows},
+        "requests_per_s": round(len(reqs) / wall, 1),
+        "queries": nqs,
+        "p50_ms": round(st["p50_ms"], 3),
+        "p99_ms": round(st["p99_ms"], 3),
+        "mean_ms": round(st["mean_ms"], 3),
+        "max_ms": round(st["max_ms"], 3),
+        "batches": st["batches"],
+        "co_batched_requests": sum(1 for inf in infos if inf["batch_requests"] > 1),
+        "peak_inflight": st["peak_inflight"],
+        "peak_reserved_gb": round(st["peak_reserved_bytes"] / 2**30, 3),
+        "wall_s": round(wall, 3),
+    }
     print(json.dumps(line), flush=True)
     return 0
 

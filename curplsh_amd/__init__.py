@@ -260,6 +260,25 @@ class IndexBF:
         call("rplsh_bf_data_dev", self._h, C.byref(x), C.byref(n))
         return x.value or 0, n.value or 0
 
+    def autotune(self, sample, k: int, batch_nq: int, target_recall: float = 0.999) -> dict:
+        """rplsh_bf_autotune: tunes (k' factor, grid, survivor target) for k
+        and this batch size, installs the result and returns it."""
+        if isinstance(sample, np.ndarray):
+            sample = np.ascontiguousarray(sample, dtype=np.float32)
+        r = _lib.TuneResult()
+        call("rplsh_bf_autotune", self._h, _ptr(sample), int(sample.shape[0]), int(k), int(batch_nq),
+             float(target_recall), C.byref(r))
+        return {f: getattr(r, f) for f, _ in r._fields_}
+
+    def set_tuning(self, t: dict) -> None:
+        r = _lib.TuneResult()
+        for f, _ in r._fields_:
+            setattr(r, f, t[f])
+        call("rplsh_bf_set_tuning", self._h, C.byref(r))
+
+    def clear_tuning(self) -> None:
+        call("rplsh_bf_clear_tuning", self._h)
+
     def last_stats(self) -> dict:
         s = _lib.SearchStats()
         call("rplsh_bf_last_stats", self._h, C.byref(s))
@@ -286,12 +305,14 @@ class Server:
     overlap on the GPU; admission is bounded by an HBM budget."""
 
     def __init__(self, device: int = 0, slots: int = 4, hbm_budget_bytes: int = 0,
-                 max_batch_queries: int = 4096):
+                 max_batch_queries: int = 4096, use_graphs: bool = True,
+                 graph_max_queries: int = 1024):
         load()
         cfg = _lib.ServerConfig()
         _lib.load().rplsh_server_config_default(C.byref(cfg))
         cfg.device, cfg.slots = device, slots
         cfg.hbm_budget_bytes, cfg.max_batch_queries = hbm_budget_bytes, max_batch_queries
+        cfg.use_graphs, cfg.graph_max_queries = int(use_graphs), graph_max_queries
         self._h = C.c_void_p()
         call("rplsh_server_create", C.byref(cfg), C.byref(self._h))
         self._pending = {}

@@ -70,12 +70,28 @@ class SearchStats(C.Structure):
     ]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [
+        ("k", C.c_int),
+        ("batch_nq", C.c_int64),
+        ("kprime_factor", C.c_int),
+        ("target_wgs", C.c_int),
+        ("c_factor", C.c_int),
+        ("recall", C.c_float),
+        ("ms_default", C.c_float),
+        ("ms_tuned", C.c_float),
+    ]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [
         ("device", C.c_int),
         ("slots", C.c_int),
         ("hbm_budget_bytes", C.c_size_t),
         ("max_batch_queries", C.c_int),
+        ("use_graphs", C.c_int),
+        ("graph_max_queries", C.c_int),
+        ("graph_cache_bytes", C.c_size_t),
     ]
 
 
@@ -104,6 +120,9 @@ class ServerStats(C.Structure):
         ("qps", C.c_double),
         ("peak_reserved_bytes", C.c_size_t),
         ("peak_inflight", C.c_int),
+        ("graph_launches", C.c_int64),
+        ("graph_captures", C.c_int64),
+        ("graph_cache_bytes", C.c_size_t),
     ]
 
 
@@ -161,6 +180,9 @@ _SIGS = {
     "rplsh_bf_set_precision": [_vp, _i32, _i32],
     "rplsh_bf_data_dev": [_vp, C.POINTER(_vp), C.POINTER(_vp)],
     "rplsh_bf_last_stats": [_vp, C.POINTER(SearchStats)],
+    "rplsh_bf_autotune": [_vp, _vp, _i64, _i32, _i64, _f32, C.POINTER(TuneResult)],
+    "rplsh_bf_set_tuning": [_vp, C.POINTER(TuneResult)],
+    "rplsh_bf_clear_tuning": [_vp],
     "rplsh_server_config_default": [C.POINTER(ServerConfig)],
     "rplsh_server_create": [C.POINTER(ServerConfig), C.POINTER(_vp)],
     "rplsh_server_destroy": [_vp],

@@ -56,6 +56,7 @@ struct FilterArgs {
   const float* qscale;    // [nq] per-row scale (FP8 only)
   const float* tau;       // [nq] threshold on the approximate relative distance
   int nq;
+  const int* nq_dev;      // if set: the number of live queries (<= nq), read on device
   int num_qtiles;
   int num_slices;
   int cap;                // candidate slots per query
@@ -68,6 +69,7 @@ struct SelectArgs {
   const int* counts;     // [nq]
   int cap;
   int nq;
+  const int* nq_dev;     // if set: live queries (<= nq), read on device
   int K;                 // rank (threshold mode) or list length (top-K mode)
   int topk_mode;         // 0: tau_out = key of K-th smallest; 1: sorted K smallest
   const float* tau_in;   // [nq] threshold the candidates were filtered with
@@ -89,6 +91,7 @@ struct RerankArgs {
   int kp;
   int k;
   int nq;
+  const int* nq_dev;     // if set: live queries (<= nq), read on device
   int exact;             // add ||q||^2
   const float* bound;    // [nq] certification bound (plain path) or nullptr
   float eps_coeff;       // plain: certified margin = eps_coeff * (||q||^2 + 2 xn_max)

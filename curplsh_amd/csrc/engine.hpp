@@ -106,8 +106,19 @@ struct SearchParams {
   int fp64_boundary = 0;
   bool certify = false;
   int c_factor = 8;    // last-level target survivors = c_factor * kp
+  int target_wgs = 4096;  // filter workgroups aimed for (query tiles x database slices)
 };
 
+
+// The capturable part of a batch (no host synchronisation; allocates only
+// when ws grows). nq_dev, if set, is a device int with the live query count.
+void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
+                   const SearchParams& p, int32_t* out_i, float* out_d, Workspace& ws,
+                   hipStream_t st, PhaseEvents* pe);
+// Waits for st, reads the counters, re-runs failed queries eagerly.
+void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
+                  int32_t* out_i, float* out_d, std::vector<Workspace>& wss, int depth,
+                  hipStream_t st, rplsh_search_stats& stats, const PhaseEvents* pe);
 
 // One batch of nq device-resident FP32 queries [nq][d] -> device outputs
 // [nq][k], on stream st, with the workspaces wss (one per retry depth).
@@ -122,9 +133,32 @@ size_t search_workspace_bytes(const DbView& db, int nq, const SearchParams& p);
 // The view and the parameters IndexBF::search uses for k on an index; the
 // candidate copy must already match the configured precision.
 DbView bf_db_view(const rplsh_bf* b);
-SearchParams bf_search_params(const rplsh_bf* b, int k);
+// The tuned entry for (k, nq) is used when the index has one (rplsh_bf_autotune).
+SearchParams bf_search_params(const rplsh_bf* b, int k, int64_t nq = 0);
 int bf_device_of(const rplsh_bf* b);
 // Brings the candidate copy up to date with the configured precision.
 void bf_prepare(rplsh_bf* b);
 
 }  // namespace rplsh
+
+// IndexBF state (C-ABI handle), shared by search.hip, server.hip and tune.hip.
+struct rplsh_bf {
+  rplsh_resources* res = nullptr;
+  int dim = 0;
+  rplsh_bf_config cfg{};
+  int64_t n = 0;
+  int64_t capacity = 0;
+  int64_t ldx = 0;
+  float* x = nullptr;
+  float* xn = nullptr;
+  float* xn_max_dev = nullptr;
+  float xn_max = 0.0f;
+  void* xenc = nullptr;
+  float* xscale = nullptr;
+  int enc_precision = -1;  // precision xenc currently holds
+  int dpad = 0;
+  std::vector<rplsh::Workspace> ws;
+  rplsh::DevBuf qdev, idev, ddev;
+  rplsh_search_stats last{};
+  std::vector<rplsh_tune_result> tunes;  // autotuned parameters per (k, batch size)
+};

@@ -170,6 +170,8 @@ __global__ __launch_bounds__(kBlock, 2) void filter_kernel(FilterArgs a) {
   if (logical >= a.num_qtiles * a.num_slices) return;
   const int qtile = logical % a.num_qtiles;
   const int slice = logical / a.num_qtiles;
+  const int nq = a.nq_dev ? min(*a.nq_dev, a.nq) : a.nq;
+  if (qtile * BQ >= nq) return;  // whole workgroup leaves together
 
   const int64_t r_begin = (int64_t)slice * a.rows_per_slice;
   int64_t r_end = r_begin + a.rows_per_slice;
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(kBlock, 2) void filter_kernel(FilterArgs a) {
   for (int nt = 0; nt < TN; ++nt) {
     const int qr = qtile * BQ + (wave * TN + nt) * 32 + col;
     qrow[nt] = qr;
-    if (qr < a.nq) {
+    if (qr < nq) {
       const char* qp = reinterpret_cast<const char*>(a.q) + (int64_t)qr * ROWB;
 #pragma unroll
       for (int c = 0; c < NC; ++c) qf[nt][c] = load_frag<LB>(qp + CB * c + LB * h);

@@ -120,6 +120,8 @@ __global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) 
   if (logical >= a.num_qtiles * a.num_slices) return;
   const int qtile = logical % a.num_qtiles;
   const int slice = logical / a.num_qtiles;
+  const int nq = a.nq_dev ? min(*a.nq_dev, a.nq) : a.nq;
+  if (qtile * BQ >= nq) return;
   const int64_t r_begin = (int64_t)slice * a.rows_per_slice;
   int64_t r_end = r_begin + a.rows_per_slice;
   if (r_end > a.m) r_end = a.m;
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) 
   for (int nt = 0; nt < TN; ++nt) {
     const int qr = qtile * BQ + (wave * TN + nt) * 32 + col;
     qrow[nt] = qr;
-    if (qr < a.nq) {
+    if (qr < nq) {
       const char* qp = reinterpret_cast<const char*>(a.q) + (int64_t)qr * G::ROWB;
 #pragma unroll
       for (int c = 0; c < NC; ++c) qf[nt][c] = load_frag<LB>(qp + 2 * LB * c + LB * h);

@@ -46,8 +46,10 @@ void launch_encode_packed(Elem e, int dpad, const float* x, int64_t n, int d, in
 // select.hip
 void launch_select(const SelectArgs& a, hipStream_t st);
 void launch_rerank(const RerankArgs& a, hipStream_t st);
-// Compacts the ids of queries with a non-zero status into list, count in *n.
-void launch_collect_failed(const int* status, int nq, int* list, int* n, hipStream_t st);
+// Compacts the ids of live queries (< *nq_dev when given) with a non-zero
+// status into list, count in *n.
+void launch_collect_failed(const int* status, int nq, const int* nq_dev, int* list, int* n,
+                           hipStream_t st);
 // Sums counts[0..nq) into *out (int64).
 void launch_sum_counts(const int* counts, int nq, int cap, long long* out, hipStream_t st);
 // Row gather / scatter for re-running a subset of queries.

@@ -100,8 +100,8 @@ static int64_t level_cap(const std::vector<Level>& L, int kp, int c_factor) {
 }
 
 static void run_filter(const DbView& db, const Level& lv, const void* qenc, const float* qscale,
-                       const float* tau, int nq, int cap, int* counts, uint64_t* cand,
-                       hipStream_t st) {
+                       const float* tau, int nq, const int* nq_dev, int cap, int* counts,
+                       uint64_t* cand, int64_t target_wgs, hipStream_t st) {
   FilterArgs fa{};
   fa.x = db.xenc;
   fa.ldx = db.ldenc;
@@ -114,10 +114,10 @@ static void run_filter(const DbView& db, const Level& lv, const void* qenc, cons
   fa.qscale = qscale;
   fa.tau = tau;
   fa.nq = nq;
+  fa.nq_dev = nq_dev;
   const int bq = db.packed ? filter_packed_block_queries(db.elem, db.dpad)
                            : filter_block_queries(db.elem, db.dpad);
   fa.num_qtiles = (int)div_up(nq, bq);
-  const int64_t target_wgs = 4096;
   int64_t slices = std::max<int64_t>(1, div_up(target_wgs, fa.num_qtiles));
   slices = std::min<int64_t>(slices, div_up(lv.m, 256));
   slices = std::max<int64_t>(slices, 1);
@@ -134,18 +134,23 @@ static void run_filter(const DbView& db, const Level& lv, const void* qenc, cons
   }
 }
 
-// One batch of nq device-resident FP32 queries [nq][d] -> device outputs.
-void search_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
-                  int32_t* out_i, float* out_d, std::vector<Workspace>& wss, int depth,
-                  hipStream_t st, rplsh_search_stats& stats) {
-  if (nq <= 0) return;
-  Workspace& ws = wss.at(depth);
+// Capturable part of a batch: query norms and encoding, filter levels,
+// selects, re-rank and the failed-query list, all on st with no host
+// synchronisation (nothing is allocated once ws has seen this size). When
+// nq_dev is set it holds the live query count (<= nq); rows beyond it are
+// skipped by every kernel.
+void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
+                   const SearchParams& p, int32_t* out_i, float* out_d, Workspace& ws,
+                   hipStream_t st, PhaseEvents* pe) {
   const int d = db.d;
   const int kp = (int)std::min<int64_t>(p.kp, db.n);
   const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor);
   const int64_t cap64 = level_cap(L, kp, p.c_factor);
   RPLSH_REQUIRE(cap64 < (1ll << 31), "candidate capacity overflow");
   const int cap = (int)cap64;
+  auto rec = [&](int i) {
+    if (pe) pe->rec(i, st);
+  };
 
   // 1. query norms and encoding
   float* qn = ws.qnorm.as<float>(nq);
@@ -173,20 +178,19 @@ void search_batch(const DbView& db, const float* q, int nq, const SearchParams& 
   float* relk = ws.relk.as<float>(nq);
   launch_fill_f32(tau, nq, INFINITY, st);
   launch_fill_i32(status, nq, 0, st);
-  PhaseEvents pe;
-  if (depth == 0) pe.create();
-  pe.rec(0, st);
+  rec(0);
   for (size_t li = 0; li < L.size(); ++li) {
     const Level& lv = L[li];
     const bool last = li + 1 == L.size();
-    if (last) pe.rec(1, st);
-    run_filter(db, lv, qenc, qscale, tau, nq, cap, counts, cand, st);
-    if (last) pe.rec(2, st);
+    if (last) rec(1);
+    run_filter(db, lv, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs, st);
+    if (last) rec(2);
     SelectArgs sa{};
     sa.cand = cand;
     sa.counts = counts;
     sa.cap = cap;
     sa.nq = nq;
+    sa.nq_dev = nq_dev;
     sa.tau_in = tau;
     if (lv.rank_next > 0) {
       sa.K = lv.rank_next;
@@ -202,18 +206,12 @@ void search_batch(const DbView& db, const float* q, int nq, const SearchParams& 
       sa.bound_out = bound;
       sa.status = status;
       launch_select(sa, st);
-      pe.rec(3, st);
+      rec(3);
     }
   }
-  if (depth == 0) {
-    stats.levels = (int32_t)L.size();
-    stats.rows_filter_full = L.back().m;
-  }
-  {
-    long long* sv = ws.surv.as<long long>(1);
-    launch_fill_i32(reinterpret_cast<int*>(sv), 2, 0, st);
-    launch_sum_counts(counts, nq, cap, sv, st);
-  }
+  long long* sv = ws.surv.as<long long>(1);
+  launch_fill_i32(reinterpret_cast<int*>(sv), 2, 0, st);
+  launch_sum_counts(counts, nq, cap, sv, st);
 
   // 3. re-rank
   int* fp64c = ws.fp64.as<int>(1);
@@ -230,6 +228,7 @@ void search_batch(const DbView& db, const float* q, int nq, const SearchParams& 
   ra.kp = kp;
   ra.k = p.k;
   ra.nq = nq;
+  ra.nq_dev = nq_dev;
   ra.exact = p.exact;
   ra.bound = p.certify ? bound : nullptr;
   ra.eps_coeff = 4.0f * (float)(d + 4) * kUnitRoundoff;
@@ -241,27 +240,42 @@ void search_batch(const DbView& db, const float* q, int nq, const SearchParams& 
   ra.status = status;
   ra.relk_out = relk;
   ra.fp64_count = fp64c;
-  pe.rec(4, st);
+  rec(4);
   launch_rerank(ra, st);
-  pe.rec(5, st);
+  rec(5);
 
   // 4. failed queries
   int* flist = ws.flist.as<int>(nq);
   int* fcount = ws.fcount.as<int>(1);
   launch_fill_i32(fcount, 1, 0, st);
-  launch_collect_failed(status, nq, flist, fcount, st);
+  launch_collect_failed(status, nq, nq_dev, flist, fcount, st);
+}
+
+// Host side after enqueue_batch: waits for st, reads the counters and re-runs
+// the failed queries (eagerly, with wider margins) into out_i / out_d.
+void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
+                  int32_t* out_i, float* out_d, std::vector<Workspace>& wss, int depth,
+                  hipStream_t st, rplsh_search_stats& stats, const PhaseEvents* pe) {
+  Workspace& ws = wss.at(depth);
+  const int d = db.d;
+  const int kp = (int)std::min<int64_t>(p.kp, db.n);
   int nfail = 0, nfp64 = 0;
   long long nsurv = 0;
-  RPLSH_HIP(hipMemcpyAsync(&nfail, fcount, sizeof(int), hipMemcpyDeviceToHost, st));
-  RPLSH_HIP(hipMemcpyAsync(&nfp64, fp64c, sizeof(int), hipMemcpyDeviceToHost, st));
+  RPLSH_HIP(hipMemcpyAsync(&nfail, ws.fcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  RPLSH_HIP(hipMemcpyAsync(&nfp64, ws.fp64.p, sizeof(int), hipMemcpyDeviceToHost, st));
   RPLSH_HIP(hipMemcpyAsync(&nsurv, ws.surv.p, sizeof(long long), hipMemcpyDeviceToHost, st));
   RPLSH_HIP(hipStreamSynchronize(st));
   stats.boundary_fp64 += nfp64;
   if (depth == 0) {
-    stats.ms_filter_samples = pe.ms(0, 1);
-    stats.ms_filter_full = pe.ms(1, 2);
-    stats.ms_select = pe.ms(2, 3);
-    stats.ms_rerank = pe.ms(4, 5);
+    const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor);
+    stats.levels = (int32_t)L.size();
+    stats.rows_filter_full = L.back().m;
+    if (pe && pe->on) {
+      stats.ms_filter_samples = pe->ms(0, 1);
+      stats.ms_filter_full = pe->ms(1, 2);
+      stats.ms_select = pe->ms(2, 3);
+      stats.ms_rerank = pe->ms(4, 5);
+    }
     stats.survivors = nsurv;
     stats.candidates = (int64_t)nq * kp;
     stats.kprime = kp;
@@ -277,13 +291,24 @@ void search_batch(const DbView& db, const float* q, int nq, const SearchParams& 
   float* qs = sub.qsub.as<float>((size_t)nfail * d);
   int32_t* is = sub.isub.as<int32_t>((size_t)nfail * p.k);
   float* ds = sub.dsub.as<float>((size_t)nfail * p.k);
-  launch_gather_rows(q, d, flist, nfail, d, qs, st);
+  launch_gather_rows(q, d, static_cast<const int*>(ws.flist.p), nfail, d, qs, st);
   SearchParams p2 = p;
   p2.c_factor = p.c_factor * 4;
   p2.kp = (int)std::min<int64_t>(std::min<int64_t>(db.n, 4096), (int64_t)p.kp * 4);
   search_batch(db, qs, nfail, p2, is, ds, wss, depth + 1, st, stats);
-  launch_scatter_results(is, ds, flist, nfail, p.k, out_i, out_d, st);
+  launch_scatter_results(is, ds, static_cast<const int*>(ws.flist.p), nfail, p.k, out_i, out_d, st);
   RPLSH_HIP(hipStreamSynchronize(st));
+}
+
+// One batch of nq device-resident FP32 queries [nq][d] -> device outputs.
+void search_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
+                  int32_t* out_i, float* out_d, std::vector<Workspace>& wss, int depth,
+                  hipStream_t st, rplsh_search_stats& stats) {
+  if (nq <= 0) return;
+  PhaseEvents pe;
+  if (depth == 0) pe.create();
+  enqueue_batch(db, q, nq, nullptr, p, out_i, out_d, wss.at(depth), st, depth == 0 ? &pe : nullptr);
+  finish_batch(db, q, nq, p, out_i, out_d, wss, depth, st, stats, depth == 0 ? &pe : nullptr);
 }
 
 static int plain_kprime(int k) { return std::max(2 * k, k + 32); }
@@ -307,27 +332,8 @@ using namespace rplsh;
 
 // ------------------------------------------------------------------ IndexBF
 
-struct rplsh_bf {
-  rplsh_resources* res = nullptr;
-  int dim = 0;
-  rplsh_bf_config cfg{};
-  int64_t n = 0;
-  int64_t capacity = 0;
-  int64_t ldx = 0;
-  float* x = nullptr;
-  float* xn = nullptr;
-  float* xn_max_dev = nullptr;
-  float xn_max = 0.0f;
-  void* xenc = nullptr;
-  float* xscale = nullptr;
-  int enc_precision = -1;  // precision xenc currently holds
-  int dpad = 0;
-  std::vector<Workspace> ws;
-  DevBuf qdev, idev, ddev;
-  rplsh_search_stats last{};
-};
 
-static hipStream_t bf_stream(rplsh_bf* b) {
+hipStream_t bf_stream(rplsh_bf* b) {
   void* s = nullptr;
   if (rplsh_resources_default_stream(b->res, b->cfg.device, &s) != RPLSH_OK)
     fail(RPLSH_ERR_STATE, "default stream: %s", rplsh_last_error());
@@ -415,11 +421,12 @@ namespace rplsh {
 
 DbView bf_db_view(const rplsh_bf* b) { return bf_view(b); }
 
-SearchParams bf_search_params(const rplsh_bf* b, int k) {
+SearchParams bf_search_params(const rplsh_bf* b, int k, int64_t nq) {
   SearchParams p;
   p.k = k;
   p.exact = 1;  // IndexBF::search always asks for exact distances (index_bf.cpp:68-69)
-  if (elem_of(b->cfg.precision) == Elem::F32) {
+  const bool plain = elem_of(b->cfg.precision) == Elem::F32;
+  if (plain) {
     p.kp = plain_kprime(k);
     p.certify = true;
     p.fp64_boundary = 0;
@@ -427,6 +434,22 @@ SearchParams bf_search_params(const rplsh_bf* b, int k) {
     p.kp = k * b->cfg.kprime_factor;
     p.certify = false;
     p.fp64_boundary = b->cfg.fp64_boundary;
+  }
+  // Autotuned entry for this k with the nearest batch size.
+  const rplsh_tune_result* best = nullptr;
+  double bd = 0;
+  for (const auto& t : b->tunes) {
+    if (t.k != k) continue;
+    const double dist = nq > 0 ? std::fabs(std::log((double)nq / (double)std::max<int64_t>(1, t.batch_nq))) : 0.0;
+    if (!best || dist < bd) {
+      best = &t;
+      bd = dist;
+    }
+  }
+  if (best) {
+    if (!plain && best->kprime_factor > 0) p.kp = k * best->kprime_factor;
+    if (best->target_wgs > 0) p.target_wgs = best->target_wgs;
+    if (best->c_factor > 0) p.c_factor = best->c_factor;
   }
   return p;
 }
@@ -564,6 +587,7 @@ rplsh_status rplsh_bf_set_precision(rplsh_bf* b, int precision, int kprime_facto
     (void)elem_of(precision);
     RPLSH_REQUIRE(filter_dpad(elem_of(precision), b->dim) > 0, "dimension not supported");
     DeviceScope scope(b->cfg.device);
+    if (b->cfg.precision != precision) b->tunes.clear();
     b->cfg.precision = precision;
     if (kprime_factor > 0) b->cfg.kprime_factor = kprime_factor;
     hipStream_t st = bf_stream(b);
@@ -608,7 +632,7 @@ rplsh_status rplsh_bf_search(rplsh_bf* b, int64_t nq, const float* queries, int 
     } else {
       if (b->enc_precision != b->cfg.precision) bf_encode(b, 0, st);
       const DbView v = bf_view(b);
-      const SearchParams p = bf_search_params(b, k);
+      const SearchParams p = bf_search_params(b, k, nq);
       RPLSH_REQUIRE(std::min<int64_t>(p.kp, b->n) <= 4096, "k' = %d exceeds 4096", p.kp);
       search_batch(v, qd, (int)nq, p, id, dd, b->ws, 0, st, stats);
     }
@@ -718,6 +742,192 @@ rplsh_status rplsh_search_l2_dev(const float* x, int64_t n, int d, const float* 
     rplsh_search_stats stats{};
     search_batch(v, q, (int)nq, p, indices, distances, wss, 0, st, stats);
     RPLSH_HIP(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- autotuner
+
+namespace {
+
+// recall@k of got against truth (rows of k indices; -1 entries ignored).
+double host_recall(const std::vector<int32_t>& got, const std::vector<int32_t>& truth, int64_t nq,
+                   int k) {
+  double tot = 0;
+  int64_t rows = 0;
+  std::vector<int32_t> a, t;
+  for (int64_t i = 0; i < nq; ++i) {
+    a.assign(got.begin() + i * k, got.begin() + (i + 1) * k);
+    t.assign(truth.begin() + i * k, truth.begin() + (i + 1) * k);
+    std::sort(a.begin(), a.end());
+    std::sort(t.begin(), t.end());
+    int valid = 0, hit = 0;
+    for (int32_t v : t) valid += v >= 0;
+    if (valid == 0) continue;
+    size_t x = 0, y = 0;
+    while (x < a.size() && y < t.size()) {
+      if (a[x] < t[y]) ++x;
+      else if (t[y] < a[x]) ++y;
+      else {
+        hit += a[x] >= 0;
+        ++x;
+        ++y;
+      }
+    }
+    tot += (double)hit / valid;
+    ++rows;
+  }
+  return rows ? tot / rows : 1.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+rplsh_status rplsh_bf_autotune(rplsh_bf* b, const float* sample, int64_t nsample, int k,
+                               int64_t batch_nq, float target_recall, rplsh_tune_result* out) {
+  return guard([&] {
+    RPLSH_REQUIRE(b && sample && out, "NULL argument");
+    RPLSH_REQUIRE(nsample > 0 && k > 0 && batch_nq > 0, "bad sizes");
+    RPLSH_REQUIRE(b->n > 0, "autotune needs a non-empty index");
+    DeviceScope scope(b->cfg.device);
+    bf_prepare(b);
+    hipStream_t st = bf_stream(b);
+    const int d = b->dim;
+    const int ns = (int)std::min<int64_t>(nsample, batch_nq);
+    DevBuf qs, qb, gi, gd;
+    float* dq = qs.as<float>((size_t)ns * d);
+    RPLSH_HIP(hipMemcpyAsync(dq, sample, (size_t)ns * d * sizeof(float), hipMemcpyDefault, st));
+    std::vector<Workspace> wss(4);
+    rplsh_search_stats stats{};
+    const DbView v = bf_view(b);
+    const bool plain = v.elem == Elem::F32;
+    const int kmax_f = std::max(1, 4096 / k);
+
+    rplsh_tune_result r{};
+    r.k = k;
+    r.batch_nq = batch_nq;
+    r.recall = 1.0f;
+
+    // 1. k' factor by recall against the plain path (exact in canonical FP32).
+    if (!plain) {
+      DbView vf = v;
+      vf.elem = Elem::F32;
+      vf.dpad = filter_dpad(Elem::F32, d);
+      vf.xenc = b->x;
+      vf.ldenc = b->ldx;
+      vf.dvalid_bytes = (int)(b->ldx * 4);
+      vf.xscale = nullptr;
+      vf.packed = false;
+      SearchParams pf;
+      pf.k = k;
+      pf.kp = plain_kprime(k);
+      pf.certify = true;
+      int32_t* ti = gi.as<int32_t>((size_t)ns * k);
+      float* td = gd.as<float>((size_t)ns * k);
+      search_batch(vf, dq, ns, pf, ti, td, wss, 0, st, stats);
+      std::vector<int32_t> truth((size_t)ns * k), got((size_t)ns * k);
+      RPLSH_HIP(hipMemcpy(truth.data(), ti, truth.size() * 4, hipMemcpyDeviceToHost));
+      DevBuf ri, rd;
+      int32_t* oi = ri.as<int32_t>((size_t)ns * k);
+      float* od = rd.as<float>((size_t)ns * k);
+      const int factors[] = {1, 2, 3, 4, 6, 8, 12, 16};
+      int chosen = 0;
+      float chosen_recall = 0.0f;
+      for (int f : factors) {
+        if (f > kmax_f) break;
+        SearchParams p = bf_search_params(b, k, 0);
+        p.kp = f * k;
+        search_batch(v, dq, ns, p, oi, od, wss, 0, st, stats);
+        RPLSH_HIP(hipMemcpy(got.data(), oi, got.size() * 4, hipMemcpyDeviceToHost));
+        const double rec = host_recall(got, truth, ns, k);
+        chosen = f;
+        chosen_recall = (float)rec;
+        if (rec >= target_recall) break;
+      }
+      r.kprime_factor = chosen;
+      r.recall = chosen_recall;
+    }
+
+    // 2. Grid decomposition and survivor target by measured batch time.
+    float* dqb = qb.as<float>((size_t)batch_nq * d);
+    for (int64_t off = 0; off < batch_nq; off += ns) {
+      const int64_t cnt = std::min<int64_t>(ns, batch_nq - off);
+      RPLSH_HIP(hipMemcpyAsync(dqb + off * d, dq, (size_t)cnt * d * sizeof(float),
+                               hipMemcpyDeviceToDevice, st));
+    }
+    int32_t* oi = gi.as<int32_t>((size_t)batch_nq * k);
+    float* od = gd.as<float>((size_t)batch_nq * k);
+    hipEvent_t e0, e1;
+    RPLSH_HIP(hipEventCreate(&e0));
+    RPLSH_HIP(hipEventCreate(&e1));
+    auto time_it = [&](const SearchParams& p) {
+      search_batch(v, dqb, (int)batch_nq, p, oi, od, wss, 0, st, stats);  // warm-up
+      float best = 1e30f;
+      for (int rep = 0; rep < 2; ++rep) {
+        RPLSH_HIP(hipEventRecord(e0, st));
+        search_batch(v, dqb, (int)batch_nq, p, oi, od, wss, 0, st, stats);
+        RPLSH_HIP(hipEventRecord(e1, st));
+        RPLSH_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        RPLSH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms);
+      }
+      return best;
+    };
+    SearchParams base = bf_search_params(b, k, 0);
+    if (!plain) base.kp = k * r.kprime_factor;
+    SearchParams def = base;
+    def.target_wgs = 4096;
+    def.c_factor = 8;
+    if (!plain) def.kp = k * b->cfg.kprime_factor;
+    r.ms_default = time_it(def);
+    r.ms_tuned = 1e30f;
+    const int wgs[] = {1024, 2048, 4096, 8192};
+    const int cfs[] = {4, 8, 16};
+    for (int tw : wgs)
+      for (int cf : cfs) {
+        SearchParams p = base;
+        p.target_wgs = tw;
+        p.c_factor = cf;
+        const float ms = time_it(p);
+        if (ms < r.ms_tuned) {
+          r.ms_tuned = ms;
+          r.target_wgs = tw;
+          r.c_factor = cf;
+        }
+      }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->tunes.erase(std::remove_if(b->tunes.begin(), b->tunes.end(),
+                                  [&](const rplsh_tune_result& t) {
+                                    return t.k == k && t.batch_nq == batch_nq;
+                                  }),
+                   b->tunes.end());
+    b->tunes.push_back(r);
+    *out = r;
+  });
+}
+
+rplsh_status rplsh_bf_set_tuning(rplsh_bf* b, const rplsh_tune_result* t) {
+  return guard([&] {
+    RPLSH_REQUIRE(b && t, "NULL argument");
+    RPLSH_REQUIRE(t->k > 0 && t->batch_nq > 0, "bad tuning entry");
+    RPLSH_REQUIRE(t->kprime_factor >= 0 && (int64_t)t->kprime_factor * t->k <= 4096, "k' too large");
+    b->tunes.erase(std::remove_if(b->tunes.begin(), b->tunes.end(),
+                                  [&](const rplsh_tune_result& x) {
+                                    return x.k == t->k && x.batch_nq == t->batch_nq;
+                                  }),
+                   b->tunes.end());
+    b->tunes.push_back(*t);
+  });
+}
+
+rplsh_status rplsh_bf_clear_tuning(rplsh_bf* b) {
+  return guard([&] {
+    RPLSH_REQUIRE(b, "index is NULL");
+    b->tunes.clear();
   });
 }
 

@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(SelectArgs a) {
   __shared__ int hist[256];
   __shared__ int sh[4];
   const int q = blockIdx.x;
-  if (q >= a.nq) return;
+  if (q >= a.nq || (a.nq_dev && q >= *a.nq_dev)) return;
   const int count = a.counts[q];
   const int n = count < a.cap ? count : a.cap;
   int st = 0;
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kSelBlock) void rerank_kernel(RerankArgs a) {
   __shared__ uint64_t bw[kMaxBand];
   __shared__ int sh[4];
   const int q = blockIdx.x;
-  if (q >= a.nq) return;
+  if (q >= a.nq || (a.nq_dev && q >= *a.nq_dev)) return;
   const int d = a.d;
   for (int j = threadIdx.x; j < d; j += blockDim.x) sq[j] = a.q[(int64_t)q * a.ldq + j];
   __syncthreads();
@@ -312,10 +312,12 @@ __global__ __launch_bounds__(kSelBlock) void rerank_kernel(RerankArgs a) {
   }
 }
 
-__global__ void collect_failed_kernel(const int* __restrict__ status, int nq, int* __restrict__ list,
+__global__ void collect_failed_kernel(const int* __restrict__ status, int nq,
+                                      const int* __restrict__ nq_dev, int* __restrict__ list,
                                       int* __restrict__ n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nq && status[i] != 0) {
+  const int live = nq_dev ? *nq_dev : nq;
+  if (i < nq && i < live && status[i] != 0) {
     const int p = atomicAdd(n, 1);
     list[p] = i;
   }
@@ -378,9 +380,10 @@ void launch_rerank(const RerankArgs& a, hipStream_t st) {
   RPLSH_HIP(hipGetLastError());
 }
 
-void launch_collect_failed(const int* status, int nq, int* list, int* n, hipStream_t st) {
+void launch_collect_failed(const int* status, int nq, const int* nq_dev, int* list, int* n,
+                           hipStream_t st) {
   if (nq <= 0) return;
-  collect_failed_kernel<<<(unsigned)div_up(nq, 256), 256, 0, st>>>(status, nq, list, n);
+  collect_failed_kernel<<<(unsigned)div_up(nq, 256), 256, 0, st>>>(status, nq, nq_dev, list, n);
   RPLSH_HIP(hipGetLastError());
 }
 

@@ -9,6 +9,13 @@
 // concurrently on the device. Because every query is filtered, selected and
 // re-ranked independently of the others in its batch, a request's result is
 // the same alone and in any mix.
+//
+// Batches of up to graph_max_queries queries replay a hipGraph captured once
+// per (index, k, power-of-two bucket) on the slot: the pipeline up to the
+// failed-query list is launch-free after capture, the live query count is a
+// device scalar written before each replay, and the rare failed queries are
+// resolved eagerly afterwards. Captured graphs hold their own workspaces;
+// least recently used ones are evicted beyond graph_cache_bytes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -53,10 +60,27 @@ struct Request {
   std::string err;
 };
 
+// A captured pipeline for (index, k, bucket): fixed device buffers for up to
+// `bucket` queries, the live count in nqd, its own workspaces.
+struct GraphEntry {
+  int index = -1;
+  int k = 0;
+  int bucket = 0;
+  hipGraphExec_t exec = nullptr;
+  std::vector<Workspace> wss;
+  DevBuf q, I, D, nqd;
+  size_t bytes = 0;
+  uint64_t last_use = 0;
+  ~GraphEntry() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
 struct Slot {
   hipStream_t st = nullptr;
   std::vector<Workspace> wss;
   DevBuf qbuf, ibuf, dbuf;
+  std::vector<std::unique_ptr<GraphEntry>> graphs;
   std::thread th;
 };
 
@@ -82,6 +106,9 @@ struct rplsh_server {
   int peak_inflight = 0;
   bool have_first = false;
   Clock::time_point first_submit, last_done;
+  int64_t graph_launches = 0, graph_captures = 0;
+  size_t graph_bytes = 0;  // sum over slots of cached graph workspaces
+  uint64_t use_clock = 0;
 };
 
 namespace {
@@ -108,30 +135,125 @@ void finish(rplsh_server* s, Request* r, int status) {
   s->last_done = r->t_done;
 }
 
+size_t workspace_bytes(const std::vector<Workspace>& wss) {
+  size_t b = 0;
+  for (const auto& w : wss) {
+    for (const DevBuf* d : {&w.qnorm, &w.qenc, &w.qscale, &w.qpad, &w.tau, &w.tau2, &w.counts,
+                            &w.cand, &w.topk, &w.bound, &w.status, &w.relk, &w.flist, &w.fcount,
+                            &w.fp64, &w.surv, &w.qsub, &w.isub, &w.dsub})
+      b += d->bytes;
+  }
+  return b;
+}
+
+int bucket_of(int64_t total) {
+  int b = 16;
+  while (b < total) b <<= 1;
+  return b;
+}
+
+// Finds or captures the graph of (index, k, bucket) on this slot. Called
+// without the server lock; graph_bytes is adjusted under it by the caller.
+GraphEntry* slot_graph(rplsh_server* s, Slot& sl, int index, int k, int bucket, const DbView& v,
+                       const SearchParams& p, size_t& added, size_t& freed, bool& captured) {
+  added = freed = 0;
+  captured = false;
+  for (auto& g : sl.graphs)
+    if (g->index == index && g->k == k && g->bucket == bucket) return g.get();
+  auto g = std::make_unique<GraphEntry>();
+  g->index = index;
+  g->k = k;
+  g->bucket = bucket;
+  g->wss.resize(4);
+  float* q = g->q.as<float>((size_t)bucket * v.d);
+  int32_t* I = g->I.as<int32_t>((size_t)bucket * k);
+  float* D = g->D.as<float>((size_t)bucket * k);
+  int* nqd = g->nqd.as<int>(1);
+  // Size the workspace with a run over zero live queries, then capture.
+  launch_fill_i32(nqd, 1, 0, sl.st);
+  enqueue_batch(v, q, bucket, nqd, p, I, D, g->wss[0], sl.st, nullptr);
+  RPLSH_HIP(hipStreamSynchronize(sl.st));
+  hipGraph_t graph = nullptr;
+  RPLSH_HIP(hipStreamBeginCapture(sl.st, hipStreamCaptureModeThreadLocal));
+  try {
+    enqueue_batch(v, q, bucket, nqd, p, I, D, g->wss[0], sl.st, nullptr);
+  } catch (...) {
+    (void)hipStreamEndCapture(sl.st, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  RPLSH_HIP(hipStreamEndCapture(sl.st, &graph));
+  const hipError_t ie = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  RPLSH_HIP(ie);
+  g->bytes = workspace_bytes(g->wss) + g->q.bytes + g->I.bytes + g->D.bytes + g->nqd.bytes;
+  added = g->bytes;
+  captured = true;
+  // Evict this slot's least recently used graphs beyond its share of the cache.
+  const size_t share = s->cfg.graph_cache_bytes / std::max(1, s->cfg.slots);
+  size_t mine = added;
+  for (auto& e : sl.graphs) mine += e->bytes;
+  while (mine > share && !sl.graphs.empty()) {
+    auto it = std::min_element(sl.graphs.begin(), sl.graphs.end(),
+                               [](const auto& a, const auto& b) { return a->last_use < b->last_use; });
+    mine -= (*it)->bytes;
+    freed += (*it)->bytes;
+    sl.graphs.erase(it);
+  }
+  sl.graphs.push_back(std::move(g));
+  return sl.graphs.back().get();
+}
+
 // Runs one batch of requests on a slot (called without the lock).
 int run_batch(rplsh_server* s, Slot& sl, const std::vector<Request*>& batch, std::string& err) {
   try {
     rplsh_bf* b = s->indexes[batch[0]->index];
     const int k = batch[0]->k;
     const DbView v = bf_db_view(b);
-    const SearchParams p = bf_search_params(b, k);
     int64_t total = 0;
     for (auto* r : batch) total += r->nq;
-    float* q = sl.qbuf.as<float>((size_t)total * v.d);
-    int32_t* I = sl.ibuf.as<int32_t>((size_t)total * k);
-    float* D = sl.dbuf.as<float>((size_t)total * k);
+    const SearchParams p = bf_search_params(b, k, total);
+    const bool graph = s->cfg.use_graphs && v.n > 0 && total <= s->cfg.graph_max_queries;
+    float* q;
+    int32_t* I;
+    float* D;
+    GraphEntry* g = nullptr;
+    if (graph) {
+      size_t added = 0, freed = 0;
+      bool captured = false;
+      g = slot_graph(s, sl, batch[0]->index, k, bucket_of(total), v, p, added, freed, captured);
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->graph_bytes += added;
+        s->graph_bytes -= freed;
+        s->graph_captures += captured ? 1 : 0;
+        s->graph_launches += 1;
+        g->last_use = ++s->use_clock;
+      }
+      q = static_cast<float*>(g->q.p);
+      I = static_cast<int32_t*>(g->I.p);
+      D = static_cast<float*>(g->D.p);
+    } else {
+      q = sl.qbuf.as<float>((size_t)total * v.d);
+      I = sl.ibuf.as<int32_t>((size_t)total * k);
+      D = sl.dbuf.as<float>((size_t)total * k);
+    }
     int64_t off = 0;
     for (auto* r : batch) {
       RPLSH_HIP(hipMemcpyAsync(q + off * v.d, r->q, (size_t)r->nq * v.d * sizeof(float),
                                hipMemcpyHostToDevice, sl.st));
       off += r->nq;
     }
+    rplsh_search_stats stats{};
     if (v.n == 0) {
       // Empty index: FLT_MAX / -1 (search.cu:93-100).
       launch_fill_i32(I, total * k, -1, sl.st);
       launch_fill_f32(D, total * k, FLT_MAX, sl.st);
+    } else if (g) {
+      launch_fill_i32(static_cast<int*>(g->nqd.p), 1, (int)total, sl.st);
+      RPLSH_HIP(hipGraphLaunch(g->exec, sl.st));
+      finish_batch(v, q, (int)total, p, I, D, g->wss, 0, sl.st, stats, nullptr);
     } else {
-      rplsh_search_stats stats{};
       search_batch(v, q, (int)total, p, I, D, sl.wss, 0, sl.st, stats);
     }
     off = 0;
@@ -247,6 +369,9 @@ void rplsh_server_config_default(rplsh_server_config* cfg) {
   cfg->slots = 4;
   cfg->hbm_budget_bytes = 0;
   cfg->max_batch_queries = 4096;
+  cfg->use_graphs = 1;
+  cfg->graph_max_queries = 1024;
+  cfg->graph_cache_bytes = (size_t)8 << 30;
 }
 
 rplsh_status rplsh_server_create(const rplsh_server_config* cfg, rplsh_server** out) {
@@ -257,6 +382,8 @@ rplsh_status rplsh_server_create(const rplsh_server_config* cfg, rplsh_server** 
     if (cfg) c = *cfg;
     if (c.slots <= 0) c.slots = 4;
     if (c.max_batch_queries <= 0) c.max_batch_queries = 4096;
+    if (c.graph_max_queries <= 0) c.graph_max_queries = 1024;
+    if (c.graph_cache_bytes == 0) c.graph_cache_bytes = (size_t)8 << 30;
     int count = 0;
     rplsh_device_count(&count);
     RPLSH_REQUIRE(c.device >= 0 && c.device < count, "device %d out of range [0, %d)", c.device, count);
@@ -298,6 +425,7 @@ rplsh_status rplsh_server_destroy(rplsh_server* s) {
       if (sl->th.joinable()) sl->th.join();
     DeviceScope scope(s->cfg.device);
     for (auto& sl : s->slots) {
+      sl->graphs.clear();
       sl->wss.clear();
       sl->qbuf = DevBuf();
       sl->ibuf = DevBuf();
@@ -403,6 +531,9 @@ rplsh_status rplsh_server_get_stats(rplsh_server* s, rplsh_server_stats* out) {
     out->batches = s->batches;
     out->peak_reserved_bytes = s->peak_reserved;
     out->peak_inflight = s->peak_inflight;
+    out->graph_launches = s->graph_launches;
+    out->graph_captures = s->graph_captures;
+    out->graph_cache_bytes = s->graph_bytes;
     if (!s->lat.empty()) {
       std::vector<double> v = s->lat;
       std::sort(v.begin(), v.end());
@@ -433,6 +564,7 @@ rplsh_status rplsh_server_reset_stats(rplsh_server* s) {
     s->peak_reserved = 0;
     s->peak_inflight = 0;
     s->have_first = false;
+    s->graph_launches = s->graph_captures = 0;
   });
 }
 

@@ -201,6 +201,38 @@ typedef struct rplsh_search_stats {
 } rplsh_search_stats;
 rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out);
 
+/* --------------------------------------------------------- autotuner -- */
+/* Per-index tuning of the search for a given k and batch size (no reference
+ * counterpart; the reference's chooseTileSize, search.cu:55-76, is a fixed
+ * memory-budget heuristic):
+ *  - kprime_factor (BF16/FP8): the smallest k'/k in {1,2,3,4,6,8,12,16}
+ *    whose recall@k against the plain path's exact result on the sample
+ *    reaches target_recall;
+ *  - target_wgs: workgroups the filter grid aims for (query tiles x database
+ *    slices), from {1024, 2048, 4096, 8192}, by measured batch time;
+ *  - c_factor: last-level survivor target (the candidate queue per query is
+ *    ~c_factor * k' long, capacity 4x that), from {4, 8, 16}, by time.
+ * The result is installed on the index and used by later searches with the
+ * same k (the entry with the nearest batch size). */
+typedef struct rplsh_tune_result {
+  int k;
+  int64_t batch_nq;
+  int kprime_factor;  /* 0 for the plain path */
+  int target_wgs;
+  int c_factor;
+  float recall;       /* recall@k of the chosen k' on the sample (1 for plain) */
+  float ms_default;   /* batch time with the built-in defaults */
+  float ms_tuned;     /* batch time with the chosen parameters */
+} rplsh_tune_result;
+
+/* sample: nsample queries (host or device); batch_nq: batch size to time
+ * (the sample is repeated to fill it). */
+rplsh_status rplsh_bf_autotune(rplsh_bf* index, const float* sample, int64_t nsample, int k,
+                               int64_t batch_nq, float target_recall, rplsh_tune_result* out);
+/* Installs a result (for example one loaded from a cache file). */
+rplsh_status rplsh_bf_set_tuning(rplsh_bf* index, const rplsh_tune_result* t);
+rplsh_status rplsh_bf_clear_tuning(rplsh_bf* index);
+
 /* ----------------------------------------------------------- serving -- */
 /* Continuous-batching request loop over resident indexes (no reference
  * counterpart). `slots` worker threads each own a HIP stream and a workspace;
@@ -209,7 +241,10 @@ rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out)
  * when the batch's workspace fits in what is left of the HBM budget, so
  * requests arriving while others run are admitted as slots and memory free
  * up, and their kernels overlap on the device. A request whose workspace can
- * never fit is completed with RPLSH_ERR_BUDGET. Results equal those of
+ * never fit is completed with RPLSH_ERR_BUDGET. Batches up to
+ * graph_max_queries run as a replay of a hipGraph captured once per (index,
+ * k, power-of-two batch bucket); the live query count is read on device, so
+ * one graph serves every batch size of its bucket. Results equal those of
  * rplsh_bf_search on the same queries. Registered indexes must not be
  * modified (add/reset/set_precision) while the server runs. */
 typedef struct rplsh_server rplsh_server;
@@ -219,6 +254,11 @@ typedef struct rplsh_server_config {
   int slots;               /* concurrent batches (default 4) */
   size_t hbm_budget_bytes; /* admission budget for batch workspaces (0 = 1/4 of free HBM) */
   int max_batch_queries;   /* co-batching limit (default 4096) */
+  int use_graphs;          /* replay captured hipGraphs for batches up to
+                              graph_max_queries (default 1) */
+  int graph_max_queries;   /* largest batch bucket captured (default 1024) */
+  size_t graph_cache_bytes;/* device bytes of cached graph workspaces per server
+                              (default 8 GiB); least recently used are evicted */
 } rplsh_server_config;
 
 typedef struct rplsh_request_info {
@@ -236,6 +276,9 @@ typedef struct rplsh_server_stats {
   double qps;                             /* completed queries / wall_s */
   size_t peak_reserved_bytes;             /* peak sum of admitted batch workspaces */
   int peak_inflight;                      /* peak concurrent batches */
+  int64_t graph_launches;                 /* batches run by replaying a captured graph */
+  int64_t graph_captures;                 /* graphs captured */
+  size_t graph_cache_bytes;               /* device bytes held by cached graphs */
 } rplsh_server_stats;
 
 void rplsh_server_config_default(rplsh_server_config* cfg);

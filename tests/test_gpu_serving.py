@@ -97,3 +97,35 @@ def test_request_larger_than_budget_is_rejected(ca, indexes):
     Ia, Da = indexes[0].search(q[:1], 1)
     assert np.array_equal(I, Ia)
     srv.close()
+
+
+def test_graph_replays_match_eager(ca, indexes):
+    """The same (index, k, bucket) graph replayed many times, with eager
+    searches and other graphs in between, always equals the eager result."""
+    srv = ca.Server(slots=2)
+    ids = [srv.add_index(ix) for ix in indexes]
+    q = ca.gen_uniform(37, 96, 77, -1, 1)
+    Ia, Da = indexes[1].search(q, 10)
+    other = ca.gen_uniform(5, 64, 78, -1, 1)
+    for rep in range(6):
+        I, D, info = srv.wait(srv.submit(ids[1], q[: 20 + rep], 10), timeout_s=100.0)
+        assert np.array_equal(I, Ia[: 20 + rep])
+        assert np.array_equal(D.view(np.uint32), Da[: 20 + rep].view(np.uint32))
+        srv.wait(srv.submit(ids[0], other, 1), timeout_s=100.0)
+        indexes[2].search(ca.gen_uniform(3, 128, 79 + rep, -1, 1), 5)  # eager work between
+    st = srv.stats()
+    print("graph stats:", st)
+    assert st["graph_launches"] >= 12
+    assert st["graph_captures"] <= 4  # one per (index, k, bucket) and slot
+    srv.close()
+
+
+def test_eager_server_matches(ca, indexes):
+    srv = ca.Server(slots=3, use_graphs=False)
+    ids = [srv.add_index(ix) for ix in indexes]
+    for i, q, k in make_requests(ca, 20, 5):
+        I, D, _ = srv.wait(srv.submit(ids[i], q, k), timeout_s=100.0)
+        Ia, Da = indexes[i].search(q, k)
+        assert np.array_equal(I, Ia) and np.array_equal(D.view(np.uint32), Da.view(np.uint32))
+    assert srv.stats()["graph_launches"] == 0
+    srv.close()
