@@ -61,8 +61,13 @@ def main() -> int:
     ap.add_argument("--recall-queries", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", default="headline", choices=["headline", "serving"],
-                    help="headline: the metric's workload; serving: the 64-index request loop")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (grid, survivor target, filter variant; k' stays 4k)")
+    ap.add_argument("--mode", default="headline", choices=["headline", "serving", "copy"],
+                    help="headline: the metric's workload; serving: the 64-index request loop; "
+                         "copy: device-to-device copies of --copy-gib GiB (HBM bandwidth and the "
+                         "reference for scaling PMC byte counters)")
+    ap.add_argument("--copy-gib", type=float, default=1.0)
     ap.add_argument("--rate", type=float, default=400.0, help="serving: Poisson arrival rate (req/s)")
     ap.add_argument("--requests", type=int, default=4096, help="serving: number of requests")
     ap.add_argument("--indexes", type=int, default=64, help="serving: resident indexes")
@@ -82,6 +87,8 @@ def main() -> int:
         return 1
     if args.mode == "serving":
         return serving(args, ca)
+    if args.mode == "copy":
+        return copy_probe(args, ca)
     n, nq, d, k = args.n, args.nq, args.d, args.k
     log(f"[bench] generating {n}x{d} database and {nq} queries")
     t0 = time.time()
@@ -104,8 +111,15 @@ def main() -> int:
     dD = ca.DeviceArray((nq, k), np.float32)
 
     results = {}
+    tunings = {}
     for form in forms:
         idx.set_precision(form, 4)
+        if not args.no_tune:
+            # Untimed: tunes grid size, survivor target and filter variant for
+            # this batch; target_recall 0 keeps k' = 4k (the metric's setting).
+            t0 = time.time()
+            tunings[form] = idx.autotune(q[:256], k, nq, 0.0)
+            log(f"[bench] {form} autotune in {time.time() - t0:.1f}s: {tunings[form]}")
         steps = args.steps if form == args.precision else max(1, args.plain_steps)
         for _ in range(args.warmup):
             idx.search(dq, k, dI, dD)
@@ -232,10 +246,37 @@ def main() -> int:
                                  "filter_samples": round(r["ms_filter_samples"], 3),
                                  "select": round(r["ms_select"], 3), "rerank": round(r["ms_rerank"], 3)}
         line[f"{f}_retried_queries"] = r["stats"]["retried_queries"]
+        if f in tunings:
+            line[f"{f}_tuning"] = {key: tunings[f][key] for key in
+                                   ("target_wgs", "c_factor", "rho", "variant", "ms_default", "ms_tuned")}
     if "fp32" in results and form != "fp32":
         line["speedup_vs_plain_fp32"] = round(head["qps"] / results["fp32"]["qps"], 3)
     line.update(extra)
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def copy_probe(args, ca) -> int:
+    """hipMemcpy device-to-device of a known size: achieved HBM bandwidth
+    (read + write bytes / time) and the reference dispatch whose FETCH_SIZE /
+    WRITE_SIZE counters scale the filter's."""
+    nbytes = int(args.copy_gib * (1 << 30))
+    src = ca.DeviceArray((nbytes // 4,), np.float32)
+    dst = ca.DeviceArray((nbytes // 4,), np.float32)
+    ev0, ev1 = ca.Event(), ca.Event()
+    ca.call("rplsh_memcpy_async", dst.ptr, src.ptr, nbytes, None)
+    ca.call("rplsh_device_sync")
+    reps = max(1, args.steps)
+    ev0.record(0)
+    for _ in range(reps):
+        ca.call("rplsh_memcpy_async", dst.ptr, src.ptr, nbytes, None)
+    ev1.record(0)
+    ev1.synchronize()
+    ms = ev0.elapsed_ms(ev1) / reps
+    gbs = 2.0 * nbytes / (ms / 1000.0) / 1e9
+    print(json.dumps({"metric": "device copy bandwidth", "value": round(gbs, 1), "unit": "GB/s",
+                      "bytes_per_copy": nbytes, "ms_per_copy": round(ms, 4), "peak": HBM_PEAK_GBS,
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
     return 0
 
 

@@ -77,6 +77,8 @@ class TuneResult(C.Structure):
         ("kprime_factor", C.c_int),
         ("target_wgs", C.c_int),
         ("c_factor", C.c_int),
+        ("variant", C.c_int),
+        ("rho", C.c_int),
         ("recall", C.c_float),
         ("ms_default", C.c_float),
         ("ms_tuned", C.c_float),

@@ -105,8 +105,10 @@ struct SearchParams {
   int exact = 1;
   int fp64_boundary = 0;
   bool certify = false;
-  int c_factor = 8;    // last-level target survivors = c_factor * kp
+  int c_factor = 3;    // last-level target survivors = c_factor * kp
   int target_wgs = 4096;  // filter workgroups aimed for (query tiles x database slices)
+  int variant = 0;        // packed filter variant (FilterVariant)
+  int rho = 0;            // rank of the last threshold in its sample (0: automatic)
 };
 
 

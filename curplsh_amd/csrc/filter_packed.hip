@@ -255,6 +255,181 @@ __global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) 
   }
 }
 
+// LDS-staged variant: the workgroup's 4 waves take different queries and
+// share each 64-row stage of packed tiles, copied linearly (one coalesced 16 B
+// piece per thread and step, gathered row by row on strided sample levels)
+// into LDS, double buffered, one barrier per stage. Each wave works one
+// 32-row tile at a time with TN query tiles, so an A operand read from LDS
+// (consecutive lanes, consecutive 16 B: conflict-free) feeds TN MFMAs.
+template <Elem E, int DPAD, int TN>
+__global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
+  using G = Geometry<E, DPAD>;
+  constexpr int LB = G::LB;
+  constexpr int NC = G::NC;
+  constexpr int BQ = 4 * TN * 32;
+  constexpr int STAGE = 2 * G::TILE;  // 64 rows
+  constexpr int UNITS = STAGE / 16;
+  constexpr int UPT = (UNITS + kBlock - 1) / kBlock;
+  constexpr int TILE_UNITS = G::TILE / 16;
+  constexpr int CHUNK_UNITS = G::CHUNK / 16;
+  constexpr int LANE_UNITS = LB / 16;
+
+  __shared__ __attribute__((aligned(16))) char sx[2][STAGE];
+  __shared__ __attribute__((aligned(16))) float sn[2][64];
+  __shared__ int se[2][64];
+
+  const int per_xcd = (int)(gridDim.x >> 3);
+  const int logical = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (logical >= a.num_qtiles * a.num_slices) return;
+  const int qtile = logical % a.num_qtiles;
+  const int slice = logical / a.num_qtiles;
+  const int nq = a.nq_dev ? min(*a.nq_dev, a.nq) : a.nq;
+  if (qtile * BQ >= nq) return;
+  const int64_t r_begin = (int64_t)slice * a.rows_per_slice;
+  int64_t r_end = r_begin + a.rows_per_slice;
+  if (r_end > a.m) r_end = a.m;
+  if (r_begin >= r_end) return;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int h = lane >> 5, col = lane & 31;
+
+  Frag<LB> qf[TN][NC];
+  float thr[TN];
+  int qsc[TN];
+  int qrow[TN];
+#pragma unroll
+  for (int nt = 0; nt < TN; ++nt) {
+    const int qr = qtile * BQ + (wave * TN + nt) * 32 + col;
+    qrow[nt] = qr;
+    if (qr < nq) {
+      const char* qp = reinterpret_cast<const char*>(a.q) + (int64_t)qr * G::ROWB;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) qf[nt][c] = load_frag<LB>(qp + 2 * LB * c + LB * h);
+      thr[nt] = fmaxf(-0.5f * a.tau[qr], -FLT_MAX);
+      qsc[nt] = E == Elem::FP8 ? e8m0_of(a.qscale[qr]) : 127;
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) qf[nt][c] = zero_frag<LB>();
+      thr[nt] = INFINITY;
+      qsc[nt] = 127;
+    }
+  }
+
+  const char* xb = reinterpret_cast<const char*>(a.x);
+  const int64_t stride = a.stride;
+  i32x4 pre[UPT];
+  float pn = 0.0f;
+  int pe = 127;
+
+  auto gload = [&](int64_t j0) {
+#pragma unroll
+    for (int t = 0; t < UPT; ++t) {
+      const int u = tid + kBlock * t;
+      i32x4 v = {0, 0, 0, 0};
+      if (UNITS % kBlock == 0 || u < UNITS) {
+        const int mt = u / TILE_UNITS;
+        const int w = u - mt * TILE_UNITS;
+        const int c = w / CHUNK_UNITS;
+        const int l = (w - c * CHUNK_UNITS) / LANE_UNITS;
+        const int sub = w % LANE_UNITS;
+        const int64_t j = j0 + mt * 32 + (l & 31);
+        if (j < r_end) {
+          const int64_t p = j * stride;
+          v = *reinterpret_cast<const i32x4*>(xb + (p >> 5) * G::TILE + c * G::CHUNK +
+                                              ((p & 31) + 32 * (l >> 5)) * LB + sub * 16);
+        }
+      }
+      pre[t] = v;
+    }
+    if (tid < 64) {
+      const int64_t j = j0 + tid;
+      if (j < r_end) {
+        pn = -0.5f * a.xnorm[j * stride];
+        pe = E == Elem::FP8 ? e8m0_of(a.xscale[j * stride]) : 127;
+      } else {
+        pn = -INFINITY;
+        pe = 127;
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < UPT; ++t) {
+      const int u = tid + kBlock * t;
+      if (UNITS % kBlock == 0 || u < UNITS) *reinterpret_cast<i32x4*>(&sx[buf][u * 16]) = pre[t];
+    }
+    if (tid < 64) {
+      sn[buf][tid] = pn;
+      se[buf][tid] = pe;
+    }
+  };
+
+  int64_t j0 = r_begin;
+  gload(j0);
+  sstore(0);
+  __syncthreads();
+  int buf = 0;
+  for (; j0 < r_end; j0 += 64) {
+    const bool has_next = j0 + 64 < r_end;
+    if (has_next) gload(j0 + 64);
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      f32x16 ci;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(&sn[buf][mt * 32 + 8 * g + 4 * h]);
+        ci[4 * g + 0] = v[0];
+        ci[4 * g + 1] = v[1];
+        ci[4 * g + 2] = v[2];
+        ci[4 * g + 3] = v[3];
+      }
+      const int xsc = E == Elem::FP8 ? se[buf][mt * 32 + col] : 127;
+      const char* ap = &sx[buf][mt * G::TILE + lane * LB];
+      f32x16 acc[TN];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const Frag<LB> af = load_frag<LB>(ap + c * G::CHUNK);
+#pragma unroll
+        for (int nt = 0; nt < TN; ++nt)
+          acc[nt] = mfma_chunk<E>(af, qf[nt][c], c == 0 ? ci : acc[nt], xsc, qsc[nt]);
+      }
+      float mx[TN];
+      bool hit = false;
+#pragma unroll
+      for (int nt = 0; nt < TN; ++nt) {
+        float v = acc[nt][0];
+#pragma unroll
+        for (int i = 1; i < 16; ++i) v = fmaxf(v, acc[nt][i]);
+        mx[nt] = v;
+        hit |= v >= thr[nt];
+      }
+      if (__any(hit)) {
+#pragma unroll
+        for (int nt = 0; nt < TN; ++nt) {
+          if (mx[nt] >= thr[nt]) {
+            const int qr = qrow[nt];
+            int* cnt = a.counts + qr;
+            uint64_t* dst = a.cand + (int64_t)qr * a.cap;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+              const float v = acc[nt][i];
+              if (v >= thr[nt]) {
+                const int64_t row = j0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+                const int pos = atomicAdd(cnt, 1);
+                if (pos < a.cap) dst[pos] = pack_cand(-2.0f * v, (int32_t)(row * stride));
+              }
+            }
+          }
+        }
+      }
+    }
+    if (has_next) sstore(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+}
+
 // Packed encoders: one thread per 16-byte output piece.
 __device__ inline uint16_t bf16_rne(float v) {
   uint32_t u = __float_as_uint(v);
@@ -340,22 +515,52 @@ __global__ __launch_bounds__(256) void e4m3_scales_kernel(const float* __restric
   if (lane == 0) scales[row] = s;
 }
 
-// Query tiles per wave: as many as keep the query operands within ~160
-// registers (the accumulators take 16 per tile, the rest holds the current and
-// the prefetched database operands), at most 7.
+// Query tiles per wave of the direct variant: as many as keep the query
+// operands within ~160 registers (one wave per SIMD), at most 7.
 template <Elem E, int DPAD>
-constexpr int packed_tn() {
+constexpr int direct_tn() {
   using G = Geometry<E, DPAD>;
   constexpr int per_tile = G::NC * G::LB / 4;
   constexpr int tn = 160 / per_tile;
   return tn > 7 ? 7 : (tn < 1 ? 1 : tn);
 }
 
+// Query tiles per wave of the LDS variants (two waves per SIMD): "wide" keeps
+// the query operands within ~80 registers (at most 4 tiles), "narrow" half.
+template <Elem E, int DPAD, bool WIDE>
+constexpr int lds_tn() {
+  using G = Geometry<E, DPAD>;
+  constexpr int per_tile = G::NC * G::LB / 4;
+  constexpr int cap = E == Elem::FP8 ? 3 : 4;
+  constexpr int wide = 80 / per_tile > cap ? cap : (80 / per_tile < 1 ? 1 : 80 / per_tile);
+  constexpr int narrow = wide / 2 < 1 ? 1 : wide / 2;
+  return WIDE ? wide : narrow;
+}
+
 template <Elem E, int DPAD>
-void launch_packed_one(const FilterArgs& a, hipStream_t st) {
-  constexpr int TN = packed_tn<E, DPAD>();
+int tn_of(int variant) {
+  switch (variant) {
+    case kFilterDirect: return direct_tn<E, DPAD>();
+    case kFilterLdsNarrow: return lds_tn<E, DPAD, false>();
+    default: return lds_tn<E, DPAD, true>();
+  }
+}
+
+template <Elem E, int DPAD>
+void launch_packed_one(int variant, const FilterArgs& a, hipStream_t st) {
   const int total = a.num_qtiles * a.num_slices;
-  filter_packed_kernel<E, DPAD, TN><<<(unsigned)round_up(total, 8), kBlock, 0, st>>>(a);
+  const unsigned grid = (unsigned)round_up(total, 8);
+  switch (variant) {
+    case kFilterDirect:
+      filter_packed_kernel<E, DPAD, direct_tn<E, DPAD>()><<<grid, kBlock, 0, st>>>(a);
+      break;
+    case kFilterLdsNarrow:
+      filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, false>()><<<grid, kBlock, 0, st>>>(a);
+      break;
+    default:
+      filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, true>()><<<grid, kBlock, 0, st>>>(a);
+      break;
+  }
   RPLSH_HIP(hipGetLastError());
 }
 
@@ -377,82 +582,86 @@ void encode_one(const float* x, int64_t n, int d, int64_t ldx, int64_t row_from,
   RPLSH_HIP(hipGetLastError());
 }
 
+// Calls f.template operator()<E, DPAD>() for the instantiated (E, dpad).
+template <typename F>
+auto dispatch(Elem e, int dpad, F&& f) {
+  if (e == Elem::BF16) {
+    switch (dpad) {
+      case 32: return f.template operator()<Elem::BF16, 32>();
+      case 64: return f.template operator()<Elem::BF16, 64>();
+      case 96: return f.template operator()<Elem::BF16, 96>();
+      case 128: return f.template operator()<Elem::BF16, 128>();
+      case 160: return f.template operator()<Elem::BF16, 160>();
+      case 192: return f.template operator()<Elem::BF16, 192>();
+      case 256: return f.template operator()<Elem::BF16, 256>();
+      default: break;
+    }
+  } else if (e == Elem::FP8) {
+    switch (dpad) {
+      case 64: return f.template operator()<Elem::FP8, 64>();
+      case 128: return f.template operator()<Elem::FP8, 128>();
+      case 256: return f.template operator()<Elem::FP8, 256>();
+      default: break;
+    }
+  }
+  fail(RPLSH_ERR_UNSUPPORTED, "packed kernels: element kind / dimension %d not instantiated", dpad);
+}
+
 }  // namespace
 
 int64_t packed_tile_bytes(Elem e, int dpad) {
   return e == Elem::BF16 ? (int64_t)32 * dpad * 2 : (int64_t)32 * dpad;
 }
 
-int filter_packed_block_queries(Elem e, int dpad) {
-  int tn = 1;
-  if (e == Elem::BF16) {
-    switch (dpad) {
-      case 32: tn = packed_tn<Elem::BF16, 32>(); break;
-      case 64: tn = packed_tn<Elem::BF16, 64>(); break;
-      case 96: tn = packed_tn<Elem::BF16, 96>(); break;
-      case 128: tn = packed_tn<Elem::BF16, 128>(); break;
-      case 160: tn = packed_tn<Elem::BF16, 160>(); break;
-      case 192: tn = packed_tn<Elem::BF16, 192>(); break;
-      case 256: tn = packed_tn<Elem::BF16, 256>(); break;
-      default: break;
-    }
-  } else if (e == Elem::FP8) {
-    switch (dpad) {
-      case 64: tn = packed_tn<Elem::FP8, 64>(); break;
-      case 128: tn = packed_tn<Elem::FP8, 128>(); break;
-      case 256: tn = packed_tn<Elem::FP8, 256>(); break;
-      default: break;
-    }
+namespace {
+
+struct TnOf {
+  int variant;
+  template <Elem E, int DPAD>
+  int operator()() const {
+    return tn_of<E, DPAD>(variant);
   }
-  return 4 * 32 * tn;
+};
+
+struct LaunchFilter {
+  int variant;
+  const FilterArgs& a;
+  hipStream_t st;
+  template <Elem E, int DPAD>
+  int operator()() const {
+    launch_packed_one<E, DPAD>(variant, a, st);
+    return 0;
+  }
+};
+
+struct EncodePacked {
+  const float* x;
+  int64_t n;
+  int d;
+  int64_t ldx, row_from;
+  char* out;
+  float* scales;
+  hipStream_t st;
+  template <Elem E, int DPAD>
+  int operator()() const {
+    encode_one<E, DPAD>(x, n, d, ldx, row_from, out, scales, st);
+    return 0;
+  }
+};
+
+}  // namespace
+
+int filter_packed_block_queries(Elem e, int dpad, int variant) {
+  return 4 * 32 * dispatch(e, dpad, TnOf{variant});
 }
 
-void launch_filter_packed(Elem e, int dpad, const FilterArgs& a, hipStream_t st) {
-  if (e == Elem::BF16) {
-    switch (dpad) {
-      case 32: return launch_packed_one<Elem::BF16, 32>(a, st);
-      case 64: return launch_packed_one<Elem::BF16, 64>(a, st);
-      case 96: return launch_packed_one<Elem::BF16, 96>(a, st);
-      case 128: return launch_packed_one<Elem::BF16, 128>(a, st);
-      case 160: return launch_packed_one<Elem::BF16, 160>(a, st);
-      case 192: return launch_packed_one<Elem::BF16, 192>(a, st);
-      case 256: return launch_packed_one<Elem::BF16, 256>(a, st);
-      default: break;
-    }
-  } else if (e == Elem::FP8) {
-    switch (dpad) {
-      case 64: return launch_packed_one<Elem::FP8, 64>(a, st);
-      case 128: return launch_packed_one<Elem::FP8, 128>(a, st);
-      case 256: return launch_packed_one<Elem::FP8, 256>(a, st);
-      default: break;
-    }
-  }
-  fail(RPLSH_ERR_UNSUPPORTED, "packed filter: element kind / dimension %d not instantiated", dpad);
+void launch_filter_packed(Elem e, int dpad, int variant, const FilterArgs& a, hipStream_t st) {
+  dispatch(e, dpad, LaunchFilter{variant, a, st});
 }
 
 void launch_encode_packed(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx,
                           int64_t row_from, void* out, float* scales, hipStream_t st) {
-  char* o = static_cast<char*>(out);
-  if (e == Elem::BF16) {
-    switch (dpad) {
-      case 32: return encode_one<Elem::BF16, 32>(x, n, d, ldx, row_from, o, scales, st);
-      case 64: return encode_one<Elem::BF16, 64>(x, n, d, ldx, row_from, o, scales, st);
-      case 96: return encode_one<Elem::BF16, 96>(x, n, d, ldx, row_from, o, scales, st);
-      case 128: return encode_one<Elem::BF16, 128>(x, n, d, ldx, row_from, o, scales, st);
-      case 160: return encode_one<Elem::BF16, 160>(x, n, d, ldx, row_from, o, scales, st);
-      case 192: return encode_one<Elem::BF16, 192>(x, n, d, ldx, row_from, o, scales, st);
-      case 256: return encode_one<Elem::BF16, 256>(x, n, d, ldx, row_from, o, scales, st);
-      default: break;
-    }
-  } else if (e == Elem::FP8) {
-    switch (dpad) {
-      case 64: return encode_one<Elem::FP8, 64>(x, n, d, ldx, row_from, o, scales, st);
-      case 128: return encode_one<Elem::FP8, 128>(x, n, d, ldx, row_from, o, scales, st);
-      case 256: return encode_one<Elem::FP8, 256>(x, n, d, ldx, row_from, o, scales, st);
-      default: break;
-    }
-  }
-  fail(RPLSH_ERR_UNSUPPORTED, "packed encoder: dimension %d not instantiated", dpad);
+  dispatch(e, dpad, EncodePacked{x, n, d, ldx, row_from, static_cast<char*>(out), scales, st});
 }
 
 }  // namespace rplsh

@@ -36,8 +36,11 @@ void launch_filter(Elem e, int dpad, const FilterArgs& a, hipStream_t st);
 
 // filter_packed.hip: the BF16 / FP8 candidate copies in 32-row MFMA tiles.
 int64_t packed_tile_bytes(Elem e, int dpad);
-int filter_packed_block_queries(Elem e, int dpad);
-void launch_filter_packed(Elem e, int dpad, const FilterArgs& a, hipStream_t st);
+// Variants of the packed filter (chosen by the autotuner): LDS-staged with
+// wide or narrow query tiles per wave, or direct-to-register loads.
+enum FilterVariant : int { kFilterLdsWide = 0, kFilterLdsNarrow = 1, kFilterDirect = 2 };
+int filter_packed_block_queries(Elem e, int dpad, int variant);
+void launch_filter_packed(Elem e, int dpad, int variant, const FilterArgs& a, hipStream_t st);
 // Encodes rows [row_from, n) (re-encoding the partial tile before row_from);
 // FP8 also writes the per-row power-of-two scales of rows [row_from, n).
 void launch_encode_packed(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx,

@@ -51,9 +51,14 @@ struct Level {
 };
 
 // Filter levels for n rows and k' candidates (see header).
-static std::vector<Level> plan_levels(int64_t n, int kp, int c_factor) {
+static std::vector<Level> plan_levels(int64_t n, int kp, int c_factor, int rho_final) {
   const int64_t kM0 = 1024;   // rows of the first (unfiltered) sample
-  const int kRhoFinal = 8;    // rank of the last threshold
+  // Rank of the last threshold: larger ranks give a tighter survivor count for
+  // the same risk but need a denser last sample (stride c*k'/rho); automatic
+  // (rho_final <= 0) keeps that stride >= 16 with rho in [8, 32].
+  const int kRhoFinal =
+      rho_final > 0 ? rho_final
+                    : (int)std::min<int64_t>(32, std::max<int64_t>(8, (int64_t)c_factor * kp / 16));
   const int kRhoMid = 16;     // rank of intermediate thresholds
   const double kGrowth = 16;  // density step between sample levels
   std::vector<Level> L;
@@ -101,7 +106,7 @@ static int64_t level_cap(const std::vector<Level>& L, int kp, int c_factor) {
 
 static void run_filter(const DbView& db, const Level& lv, const void* qenc, const float* qscale,
                        const float* tau, int nq, const int* nq_dev, int cap, int* counts,
-                       uint64_t* cand, int64_t target_wgs, hipStream_t st) {
+                       uint64_t* cand, int64_t target_wgs, int variant, hipStream_t st) {
   FilterArgs fa{};
   fa.x = db.xenc;
   fa.ldx = db.ldenc;
@@ -115,7 +120,7 @@ static void run_filter(const DbView& db, const Level& lv, const void* qenc, cons
   fa.tau = tau;
   fa.nq = nq;
   fa.nq_dev = nq_dev;
-  const int bq = db.packed ? filter_packed_block_queries(db.elem, db.dpad)
+  const int bq = db.packed ? filter_packed_block_queries(db.elem, db.dpad, variant)
                            : filter_block_queries(db.elem, db.dpad);
   fa.num_qtiles = (int)div_up(nq, bq);
   int64_t slices = std::max<int64_t>(1, div_up(target_wgs, fa.num_qtiles));
@@ -128,7 +133,7 @@ static void run_filter(const DbView& db, const Level& lv, const void* qenc, cons
   fa.cand = cand;
   launch_fill_i32(counts, nq, 0, st);
   if (db.packed) {
-    launch_filter_packed(db.elem, db.dpad, fa, st);
+    launch_filter_packed(db.elem, db.dpad, variant, fa, st);
   } else {
     launch_filter(db.elem, db.dpad, fa, st);
   }
@@ -144,7 +149,7 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
                    hipStream_t st, PhaseEvents* pe) {
   const int d = db.d;
   const int kp = (int)std::min<int64_t>(p.kp, db.n);
-  const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor);
+  const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
   const int64_t cap64 = level_cap(L, kp, p.c_factor);
   RPLSH_REQUIRE(cap64 < (1ll << 31), "candidate capacity overflow");
   const int cap = (int)cap64;
@@ -183,7 +188,7 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
     const Level& lv = L[li];
     const bool last = li + 1 == L.size();
     if (last) rec(1);
-    run_filter(db, lv, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs, st);
+    run_filter(db, lv, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs, p.variant, st);
     if (last) rec(2);
     SelectArgs sa{};
     sa.cand = cand;
@@ -267,7 +272,7 @@ void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& 
   RPLSH_HIP(hipStreamSynchronize(st));
   stats.boundary_fp64 += nfp64;
   if (depth == 0) {
-    const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor);
+    const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
     stats.levels = (int32_t)L.size();
     stats.rows_filter_full = L.back().m;
     if (pe && pe->on) {
@@ -315,7 +320,7 @@ static int plain_kprime(int k) { return std::max(2 * k, k + 32); }
 
 size_t search_workspace_bytes(const DbView& db, int nq, const SearchParams& p) {
   const int kp = (int)std::min<int64_t>(p.kp, db.n);
-  const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor);
+  const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
   const size_t cap = (size_t)level_cap(L, kp, p.c_factor);
   const size_t eb = db.elem == Elem::F32 ? 4 : (db.elem == Elem::BF16 ? 2 : 1);
   size_t b = 0;
@@ -450,6 +455,8 @@ SearchParams bf_search_params(const rplsh_bf* b, int k, int64_t nq) {
     if (!plain && best->kprime_factor > 0) p.kp = k * best->kprime_factor;
     if (best->target_wgs > 0) p.target_wgs = best->target_wgs;
     if (best->c_factor > 0) p.c_factor = best->c_factor;
+    if (best->variant >= 0 && best->variant <= 2) p.variant = best->variant;
+    if (best->rho > 0) p.rho = best->rho;
   }
   return p;
 }
@@ -811,7 +818,10 @@ rplsh_status rplsh_bf_autotune(rplsh_bf* b, const float* sample, int64_t nsample
     r.recall = 1.0f;
 
     // 1. k' factor by recall against the plain path (exact in canonical FP32).
-    if (!plain) {
+    if (!plain && target_recall <= 0.0f) {
+      r.kprime_factor = b->cfg.kprime_factor;
+      r.recall = -1.0f;  // not measured
+    } else if (!plain) {
       DbView vf = v;
       vf.elem = Elem::F32;
       vf.dpad = filter_dpad(Elem::F32, d);
@@ -880,24 +890,35 @@ rplsh_status rplsh_bf_autotune(rplsh_bf* b, const float* sample, int64_t nsample
     if (!plain) base.kp = k * r.kprime_factor;
     SearchParams def = base;
     def.target_wgs = 4096;
-    def.c_factor = 8;
+    def.c_factor = 3;
+    def.rho = 0;
+    def.variant = kFilterLdsWide;
     if (!plain) def.kp = k * b->cfg.kprime_factor;
     r.ms_default = time_it(def);
     r.ms_tuned = 1e30f;
-    const int wgs[] = {1024, 2048, 4096, 8192};
-    const int cfs[] = {4, 8, 16};
-    for (int tw : wgs)
-      for (int cf : cfs) {
-        SearchParams p = base;
-        p.target_wgs = tw;
-        p.c_factor = cf;
-        const float ms = time_it(p);
-        if (ms < r.ms_tuned) {
-          r.ms_tuned = ms;
-          r.target_wgs = tw;
-          r.c_factor = cf;
+    // (survivor multiple c, rank rho of the last threshold): the last sample
+    // level has stride c*k'/rho, so a larger rho buys a tighter c at the same
+    // risk P(Gamma(rho) < rho/c) of a query keeping fewer than k' rows.
+    const int wgs[] = {2048, 4096, 8192};
+    const int cr[][2] = {{3, 0}, {4, 16}, {2, 32}, {3, 32}, {2, 64}, {8, 8}};
+    const int nvar = v.packed ? 2 : 1;
+    for (int var = 0; var < nvar; ++var)
+      for (int tw : wgs)
+        for (const auto& c : cr) {
+          SearchParams p = base;
+          p.target_wgs = tw;
+          p.c_factor = c[0];
+          p.rho = c[1];
+          p.variant = var;
+          const float ms = time_it(p);
+          if (ms < r.ms_tuned) {
+            r.ms_tuned = ms;
+            r.target_wgs = tw;
+            r.c_factor = c[0];
+            r.rho = c[1];
+            r.variant = var;
+          }
         }
-      }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->tunes.erase(std::remove_if(b->tunes.begin(), b->tunes.end(),
@@ -915,6 +936,8 @@ rplsh_status rplsh_bf_set_tuning(rplsh_bf* b, const rplsh_tune_result* t) {
     RPLSH_REQUIRE(b && t, "NULL argument");
     RPLSH_REQUIRE(t->k > 0 && t->batch_nq > 0, "bad tuning entry");
     RPLSH_REQUIRE(t->kprime_factor >= 0 && (int64_t)t->kprime_factor * t->k <= 4096, "k' too large");
+    RPLSH_REQUIRE(t->variant >= 0 && t->variant <= 2, "unknown filter variant %d", t->variant);
+    RPLSH_REQUIRE(t->rho >= 0 && t->rho <= 256, "bad rho %d", t->rho);
     b->tunes.erase(std::remove_if(b->tunes.begin(), b->tunes.end(),
                                   [&](const rplsh_tune_result& x) {
                                     return x.k == t->k && x.batch_nq == t->batch_nq;

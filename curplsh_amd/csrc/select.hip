@@ -121,7 +121,10 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(SelectArgs a) {
   if (n < K) st |= kStatusShort;
 
   if (!a.topk_mode) {
-    float tau = INFINITY;
+    // Too few survivors for rank K: keep the previous threshold, which every
+    // survivor already met (an infinite one would let every row of the next
+    // level through).
+    float tau = a.tau_in[q];
     if (n >= K && K > 0) tau = cand_key(radix_kth(src, n, K, hist, sh));
     if (threadIdx.x == 0) {
       a.tau_out[q] = tau;

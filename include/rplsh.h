@@ -210,8 +210,13 @@ rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out)
  *    reaches target_recall;
  *  - target_wgs: workgroups the filter grid aims for (query tiles x database
  *    slices), from {1024, 2048, 4096, 8192}, by measured batch time;
- *  - c_factor: last-level survivor target (the candidate queue per query is
- *    ~c_factor * k' long, capacity 4x that), from {4, 8, 16}, by time.
+ *  - (c_factor, rho): the last level keeps ~c_factor * k' rows per query (its
+ *    candidate queue; capacity 4x that); its threshold is the rho-th smallest
+ *    key of a sample with stride c_factor * k' / rho. Pairs from
+ *    {(3,auto),(4,16),(2,32),(3,32),(2,64),(8,8)}, by time;
+ *  - variant (BF16/FP8): the filter's tile shape -- LDS-staged with wide or
+ *    narrow query tiles per wave, or direct register loads -- by time.
+ * target_recall <= 0 keeps the configured k' factor and tunes the rest.
  * The result is installed on the index and used by later searches with the
  * same k (the entry with the nearest batch size). */
 typedef struct rplsh_tune_result {
@@ -220,6 +225,8 @@ typedef struct rplsh_tune_result {
   int kprime_factor;  /* 0 for the plain path */
   int target_wgs;
   int c_factor;
+  int variant;        /* 0 LDS wide, 1 LDS narrow, 2 direct (BF16/FP8 only) */
+  int rho;            /* rank of the last threshold within its sample (0: automatic) */
   float recall;       /* recall@k of the chosen k' on the sample (1 for plain) */
   float ms_default;   /* batch time with the built-in defaults */
   float ms_tuned;     /* batch time with the chosen parameters */
