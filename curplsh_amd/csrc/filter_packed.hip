@@ -37,6 +37,7 @@ typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kBlock = 256;
+constexpr int kQueueCap = 1024;  // survivors queued in LDS per workgroup
 
 template <Elem E>
 struct PackTraits;
@@ -261,6 +262,8 @@ __global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) 
 // into LDS, double buffered, one barrier per stage. Each wave works one
 // 32-row tile at a time with TN query tiles, so an A operand read from LDS
 // (consecutive lanes, consecutive 16 B: conflict-free) feeds TN MFMAs.
+// Survivors go to an LDS queue (LDS atomics, no global round trip in the
+// loop) that is flushed to the per-query lists at stage barriers.
 template <Elem E, int DPAD, int TN>
 __global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
   using G = Geometry<E, DPAD>;
@@ -277,6 +280,11 @@ __global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
   __shared__ __attribute__((aligned(16))) char sx[2][STAGE];
   __shared__ __attribute__((aligned(16))) float sn[2][64];
   __shared__ int se[2][64];
+  // Survivor queue of the workgroup: appended with LDS atomics in the hot
+  // loop, flushed to the global per-query lists at stage barriers.
+  __shared__ uint64_t qword[kQueueCap];
+  __shared__ int qquery[kQueueCap];
+  __shared__ int qcount;
 
   const int per_xcd = (int)(gridDim.x >> 3);
   const int logical = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
@@ -323,6 +331,23 @@ __global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
   int pe = 127;
 
   auto gload = [&](int64_t j0) {
+    if (stride == 1 && j0 + 64 <= r_end) {
+      // Full stage of the full level: j0 is a multiple of 64, so the stage is
+      // two whole consecutive tiles, one contiguous run of STAGE bytes.
+      const char* src = xb + (j0 >> 5) * G::TILE;
+#pragma unroll
+      for (int t = 0; t < UPT; ++t) {
+        const int u = tid + kBlock * t;
+        i32x4 v = {0, 0, 0, 0};
+        if (UNITS % kBlock == 0 || u < UNITS) v = *reinterpret_cast<const i32x4*>(src + u * 16);
+        pre[t] = v;
+      }
+      if (tid < 64) {
+        pn = -0.5f * a.xnorm[j0 + tid];
+        pe = E == Elem::FP8 ? e8m0_of(a.xscale[j0 + tid]) : 127;
+      }
+      return;
+    }
 #pragma unroll
     for (int t = 0; t < UPT; ++t) {
       const int u = tid + kBlock * t;
@@ -365,6 +390,31 @@ __global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
     }
   };
 
+  // Appends one survivor: to the LDS queue, or straight to global memory
+  // when the queue is full.
+  auto emit = [&](int qr, float v, int64_t row) {
+    const uint64_t w = pack_cand(-2.0f * v, (int32_t)(row * stride));
+    const int slot = atomicAdd(&qcount, 1);
+    if (slot < kQueueCap) {
+      qword[slot] = w;
+      qquery[slot] = qr;
+    } else {
+      const int pos = atomicAdd(a.counts + qr, 1);
+      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = w;
+    }
+  };
+  // Whole workgroup, between barriers: moves the queue to global memory with
+  // all threads' atomics in flight together.
+  auto flush = [&](int n) {
+    if (n > kQueueCap) n = kQueueCap;
+    for (int i = tid; i < n; i += kBlock) {
+      const int qr = qquery[i];
+      const int pos = atomicAdd(a.counts + qr, 1);
+      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = qword[i];
+    }
+  };
+
+  if (tid == 0) qcount = 0;
   int64_t j0 = r_begin;
   gload(j0);
   sstore(0);
@@ -408,17 +458,10 @@ __global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
 #pragma unroll
         for (int nt = 0; nt < TN; ++nt) {
           if (mx[nt] >= thr[nt]) {
-            const int qr = qrow[nt];
-            int* cnt = a.counts + qr;
-            uint64_t* dst = a.cand + (int64_t)qr * a.cap;
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
               const float v = acc[nt][i];
-              if (v >= thr[nt]) {
-                const int64_t row = j0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
-                const int pos = atomicAdd(cnt, 1);
-                if (pos < a.cap) dst[pos] = pack_cand(-2.0f * v, (int32_t)(row * stride));
-              }
+              if (v >= thr[nt]) emit(qrow[nt], v, j0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3));
             }
           }
         }
@@ -427,6 +470,13 @@ __global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
     if (has_next) sstore(buf ^ 1);
     __syncthreads();
     buf ^= 1;
+    const int queued = qcount;  // same value in every thread
+    if (queued > kQueueCap / 2 || (!has_next && queued > 0)) {
+      flush(queued);
+      __syncthreads();
+      if (tid == 0) qcount = 0;
+      __syncthreads();
+    }
   }
 }
 
