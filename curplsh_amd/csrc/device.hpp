@@ -60,6 +60,8 @@ struct FilterArgs {
   int num_qtiles;
   int num_slices;
   int cap;                // candidate slots per query
+  int dense;              // packed kernels: every row survives (tau = +inf); row j's word goes to
+                          // slot j, counts are preset to m by the caller
   int* counts;            // [nq] survivors (may exceed cap: overflow)
   uint64_t* cand;         // [nq][cap]
 };

@@ -96,6 +96,12 @@ struct DbView {
   int64_t ldenc = 0;
   int dvalid_bytes = 0;
   const float* xscale = nullptr;  // FP8 per-row scales
+  // Packed sample copy for the sample levels: row r is row (r * sstep) mod n,
+  // r < srows (0: none); same layout, with its own norm and scale arrays.
+  const void* senc = nullptr;
+  const float* sscale = nullptr;
+  const float* snorm = nullptr;
+  int64_t srows = 0;
   bool packed = false;            // xenc is in 32-row MFMA tiles (filter_packed.hip)
 };
 
@@ -157,6 +163,12 @@ struct rplsh_bf {
   float xn_max = 0.0f;
   void* xenc = nullptr;
   float* xscale = nullptr;
+  // Packed sample copy (sample levels): rows (r * sstep) mod n for r < srows.
+  void* senc = nullptr;
+  float* sscale = nullptr;
+  float* snorm = nullptr;
+  int64_t srows = 0, scap = 0, sstep = 0;
+  int sprec = -1;  // precision the sample buffers were allocated for
   int enc_precision = -1;  // precision xenc currently holds
   int dpad = 0;
   std::vector<rplsh::Workspace> ws;

@@ -19,6 +19,8 @@
 //     next tile is prefetched at the top of the iteration.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <type_traits>
 #include <cfloat>
 #include <cstdint>
 
@@ -37,7 +39,6 @@ typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kBlock = 256;
-constexpr int kQueueCap = 1024;  // survivors queued in LDS per workgroup
 
 template <Elem E>
 struct PackTraits;
@@ -256,35 +257,58 @@ __global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) 
   }
 }
 
-// LDS-staged variant: the workgroup's 4 waves take different queries and
-// share each 64-row stage of packed tiles, copied linearly (one coalesced 16 B
-// piece per thread and step, gathered row by row on strided sample levels)
-// into LDS, double buffered, one barrier per stage. Each wave works one
-// 32-row tile at a time with TN query tiles, so an A operand read from LDS
-// (consecutive lanes, consecutive 16 B: conflict-free) feeds TN MFMAs.
-// Survivors go to an LDS queue (LDS atomics, no global round trip in the
-// loop) that is flushed to the per-query lists at stage barriers.
-template <Elem E, int DPAD, int TN>
-__global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
+// LDS-staged variants (wide / narrow query tiles): the workgroup's 4 waves
+// take different queries and share each stage of SM packed 32-row tiles,
+// copied linearly into LDS (one coalesced 16 B piece per thread and step;
+// gathered row by row on strided levels), double buffered: the global loads
+// of stage s+1 are issued before stage s is computed and stored to LDS after
+// it, one barrier per stage. Each wave works one 32-row tile at a time with TN
+// query tiles, so an A operand read from LDS (consecutive lanes, consecutive
+// 16 B: conflict-free) feeds TN MFMAs.
+// Survivors are appended to one of two LDS queues (by stage parity) with LDS
+// atomics. After the barrier that ends stage s, the queue of stage s is
+// flushed to the per-query lists if more than half full: nobody appends to it
+// during stage s+1, so every thread reads the same tail and takes the same
+// branch, and it is not reused before the barrier that ends stage s+1.
+constexpr int kQueue = 512;  // entries per survivor queue (power of two)
+
+template <Elem E, int DPAD, bool WIDE>
+struct StageShape {
   using G = Geometry<E, DPAD>;
+  // Stage tiles: the double-buffered stage within ~48 KB (wide, two
+  // workgroups per CU) or ~36 KB (narrow, three).
+  static constexpr int kFit = (WIDE ? 48 * 1024 : 36 * 1024) / (2 * G::TILE);
+  static constexpr int SM = kFit > 4 ? 4 : (kFit < 1 ? 1 : kFit);
+  static constexpr int SR = 32 * SM;  // rows per stage
+  static constexpr int STAGE = SM * G::TILE;
+  static constexpr int LDS = 2 * STAGE + 2 * SR * 8 + 2 * kQueue * 12;
+  static constexpr int WAVES_PER_SIMD = (!WIDE && 3 * LDS <= 160 * 1024) ? 3 : 2;
+};
+
+template <Elem E, int DPAD, int TN, bool WIDE>
+__global__ __launch_bounds__(kBlock, (StageShape<E, DPAD, WIDE>::WAVES_PER_SIMD)) void filter_lds_kernel(
+    FilterArgs a) {
+  using G = Geometry<E, DPAD>;
+  using SS = StageShape<E, DPAD, WIDE>;
   constexpr int LB = G::LB;
   constexpr int NC = G::NC;
   constexpr int BQ = 4 * TN * 32;
-  constexpr int STAGE = 2 * G::TILE;  // 64 rows
+  constexpr int SM = SS::SM;
+  constexpr int SR = SS::SR;
+  constexpr int STAGE = SS::STAGE;
   constexpr int UNITS = STAGE / 16;
   constexpr int UPT = (UNITS + kBlock - 1) / kBlock;
   constexpr int TILE_UNITS = G::TILE / 16;
   constexpr int CHUNK_UNITS = G::CHUNK / 16;
   constexpr int LANE_UNITS = LB / 16;
+  static_assert(SR <= kBlock, "one row norm per thread");
 
   __shared__ __attribute__((aligned(16))) char sx[2][STAGE];
-  __shared__ __attribute__((aligned(16))) float sn[2][64];
-  __shared__ int se[2][64];
-  // Survivor queue of the workgroup: appended with LDS atomics in the hot
-  // loop, flushed to the global per-query lists at stage barriers.
-  __shared__ uint64_t qword[kQueueCap];
-  __shared__ int qquery[kQueueCap];
-  __shared__ int qcount;
+  __shared__ __attribute__((aligned(16))) float sn[2][SR];  // -||x||^2/2, -inf past the end
+  __shared__ int se[2][SR];                                 // E8M0 row scales (FP8)
+  __shared__ uint64_t qword[2][kQueue];
+  __shared__ int qquery[2][kQueue];
+  __shared__ unsigned qtail[2];
 
   const int per_xcd = (int)(gridDim.x >> 3);
   const int logical = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
@@ -299,7 +323,8 @@ __global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
   if (r_begin >= r_end) return;
 
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int h = lane >> 5, col = lane & 31;
 
   Frag<LB> qf[TN][NC];
@@ -331,9 +356,9 @@ __global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
   int pe = 127;
 
   auto gload = [&](int64_t j0) {
-    if (stride == 1 && j0 + 64 <= r_end) {
-      // Full stage of the full level: j0 is a multiple of 64, so the stage is
-      // two whole consecutive tiles, one contiguous run of STAGE bytes.
+    if (stride == 1 && j0 + SR <= r_end) {
+      // Whole stage at stride 1: j0 is a multiple of 32, so the stage is SM
+      // consecutive whole tiles, one contiguous run of STAGE bytes.
       const char* src = xb + (j0 >> 5) * G::TILE;
 #pragma unroll
       for (int t = 0; t < UPT; ++t) {
@@ -342,7 +367,7 @@ __global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
         if (UNITS % kBlock == 0 || u < UNITS) v = *reinterpret_cast<const i32x4*>(src + u * 16);
         pre[t] = v;
       }
-      if (tid < 64) {
+      if (tid < SR) {
         pn = -0.5f * a.xnorm[j0 + tid];
         pe = E == Elem::FP8 ? e8m0_of(a.xscale[j0 + tid]) : 127;
       }
@@ -367,7 +392,7 @@ __global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
       }
       pre[t] = v;
     }
-    if (tid < 64) {
+    if (tid < SR) {
       const int64_t j = j0 + tid;
       if (j < r_end) {
         pn = -0.5f * a.xnorm[j * stride];
@@ -384,100 +409,126 @@ __global__ __launch_bounds__(kBlock, 2) void filter_lds_kernel(FilterArgs a) {
       const int u = tid + kBlock * t;
       if (UNITS % kBlock == 0 || u < UNITS) *reinterpret_cast<i32x4*>(&sx[buf][u * 16]) = pre[t];
     }
-    if (tid < 64) {
+    if (tid < SR) {
       sn[buf][tid] = pn;
       se[buf][tid] = pe;
     }
   };
 
-  // Appends one survivor: to the LDS queue, or straight to global memory
-  // when the queue is full.
-  auto emit = [&](int qr, float v, int64_t row) {
+  unsigned head[2] = {0u, 0u};  // flushed up to here (same in every thread)
+  auto emit = [&](int par, int qr, float v, int64_t row) {
     const uint64_t w = pack_cand(-2.0f * v, (int32_t)(row * stride));
-    const int slot = atomicAdd(&qcount, 1);
-    if (slot < kQueueCap) {
-      qword[slot] = w;
-      qquery[slot] = qr;
+    const unsigned t = atomicAdd(&qtail[par], 1u);
+    if (t - head[par] < (unsigned)kQueue) {
+      qword[par][t & (kQueue - 1)] = w;
+      qquery[par][t & (kQueue - 1)] = qr;
     } else {
       const int pos = atomicAdd(a.counts + qr, 1);
       if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = w;
     }
   };
-  // Whole workgroup, between barriers: moves the queue to global memory with
-  // all threads' atomics in flight together.
-  auto flush = [&](int n) {
-    if (n > kQueueCap) n = kQueueCap;
-    for (int i = tid; i < n; i += kBlock) {
-      const int qr = qquery[i];
+  // Whole workgroup: moves the entries of queue par that were stored in it
+  // (at most kQueue since the last flush) to the per-query lists.
+  auto flush = [&](int par, unsigned tail) {
+    const unsigned n = min(tail - head[par], (unsigned)kQueue);
+    for (unsigned i = tid; i < n; i += kBlock) {
+      const unsigned t = (head[par] + i) & (kQueue - 1);
+      const int qr = qquery[par][t];
       const int pos = atomicAdd(a.counts + qr, 1);
-      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = qword[i];
+      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = qword[par][t];
     }
+    head[par] = tail;
   };
 
-  if (tid == 0) qcount = 0;
-  int64_t j0 = r_begin;
-  gload(j0);
-  sstore(0);
-  __syncthreads();
-  int buf = 0;
-  for (; j0 < r_end; j0 += 64) {
-    const bool has_next = j0 + 64 < r_end;
-    if (has_next) gload(j0 + 64);
+  if (tid < 2) qtail[tid] = 0u;
+  // The stage loop, instantiated twice: the hot loop (DENSE false) stays
+  // small enough for the instruction cache; the dense first pass writes every
+  // row's word to its own slot.
+  auto run = [&](auto dense_tag) {
+    constexpr bool DENSE = decltype(dense_tag)::value;
+    int64_t j0 = r_begin;
+    gload(j0);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int s = 0; j0 < r_end; j0 += SR, ++s) {
+      const bool has_next = j0 + SR < r_end;
+      if (has_next) gload(j0 + SR);
+      const int par = s & 1;
+      // One 32-row tile per iteration; not unrolled (code size).
+#pragma unroll 1
+      for (int mt = 0; mt < SM; ++mt) {
+        f32x16 ci;
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-      f32x16 ci;
+        for (int g = 0; g < 4; ++g) {
+          const f32x4 v = *reinterpret_cast<const f32x4*>(&sn[buf][mt * 32 + 8 * g + 4 * h]);
+          ci[4 * g + 0] = v[0];
+          ci[4 * g + 1] = v[1];
+          ci[4 * g + 2] = v[2];
+          ci[4 * g + 3] = v[3];
+        }
+        const int xsc = E == Elem::FP8 ? se[buf][mt * 32 + col] : 127;
+        const char* ap = &sx[buf][mt * G::TILE + lane * LB];
+        f32x16 acc[TN];
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const f32x4 v = *reinterpret_cast<const f32x4*>(&sn[buf][mt * 32 + 8 * g + 4 * h]);
-        ci[4 * g + 0] = v[0];
-        ci[4 * g + 1] = v[1];
-        ci[4 * g + 2] = v[2];
-        ci[4 * g + 3] = v[3];
-      }
-      const int xsc = E == Elem::FP8 ? se[buf][mt * 32 + col] : 127;
-      const char* ap = &sx[buf][mt * G::TILE + lane * LB];
-      f32x16 acc[TN];
+        for (int c = 0; c < NC; ++c) {
+          const Frag<LB> af = load_frag<LB>(ap + c * G::CHUNK);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const Frag<LB> af = load_frag<LB>(ap + c * G::CHUNK);
+          for (int nt = 0; nt < TN; ++nt)
+            acc[nt] = mfma_chunk<E>(af, qf[nt][c], c == 0 ? ci : acc[nt], xsc, qsc[nt]);
+        }
+        if (DENSE) {
 #pragma unroll
-        for (int nt = 0; nt < TN; ++nt)
-          acc[nt] = mfma_chunk<E>(af, qf[nt][c], c == 0 ? ci : acc[nt], xsc, qsc[nt]);
-      }
-      float mx[TN];
-      bool hit = false;
-#pragma unroll
-      for (int nt = 0; nt < TN; ++nt) {
-        float v = acc[nt][0];
-#pragma unroll
-        for (int i = 1; i < 16; ++i) v = fmaxf(v, acc[nt][i]);
-        mx[nt] = v;
-        hit |= v >= thr[nt];
-      }
-      if (__any(hit)) {
-#pragma unroll
-        for (int nt = 0; nt < TN; ++nt) {
-          if (mx[nt] >= thr[nt]) {
+          for (int nt = 0; nt < TN; ++nt) {
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-              const float v = acc[nt][i];
-              if (v >= thr[nt]) emit(qrow[nt], v, j0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3));
+              const int64_t row = j0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+              if (row < r_end && qrow[nt] < nq)
+                a.cand[(int64_t)qrow[nt] * a.cap + row] = pack_cand(-2.0f * acc[nt][i], (int32_t)(row * stride));
+            }
+          }
+          continue;
+        }
+        float mx[TN];
+        bool hit = false;
+#pragma unroll
+        for (int nt = 0; nt < TN; ++nt) {
+          float v = acc[nt][0];
+#pragma unroll
+          for (int i = 1; i < 16; ++i) v = fmaxf(v, acc[nt][i]);
+          mx[nt] = v;
+          hit |= v >= thr[nt];
+        }
+        if (__any(hit)) {
+#pragma unroll
+          for (int nt = 0; nt < TN; ++nt) {
+            if (mx[nt] >= thr[nt]) {
+#pragma unroll
+              for (int i = 0; i < 16; ++i) {
+                const float v = acc[nt][i];
+                if (v >= thr[nt]) emit(par, qrow[nt], v, j0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3));
+              }
             }
           }
         }
       }
-    }
-    if (has_next) sstore(buf ^ 1);
-    __syncthreads();
-    buf ^= 1;
-    const int queued = qcount;  // same value in every thread
-    if (queued > kQueueCap / 2 || (!has_next && queued > 0)) {
-      flush(queued);
+      if (has_next) sstore(buf ^ 1);
       __syncthreads();
-      if (tid == 0) qcount = 0;
-      __syncthreads();
+      buf ^= 1;
+      if (!DENSE) {
+        const unsigned tail = qtail[par];
+        if (tail - head[par] > (unsigned)(kQueue / 2)) flush(par, tail);
+      }
     }
+  };
+  if (a.dense) {
+    run(std::true_type{});
+    return;
   }
+  run(std::false_type{});
+  // Every append is complete (the last barrier follows the last stage).
+  flush(0, qtail[0]);
+  flush(1, qtail[1]);
 }
 
 // Packed encoders: one thread per 16-byte output piece.
@@ -510,7 +561,8 @@ __device__ inline uint8_t e4m3_rne_sat(float v) {
 template <Elem E, int DPAD>
 __global__ void encode_packed_kernel(const float* __restrict__ x, int64_t n, int d, int64_t ldx,
                                      int64_t tile_from, int64_t tile_to, char* __restrict__ out,
-                                     const float* __restrict__ scales) {
+                                     const float* __restrict__ scales, int64_t map_step,
+                                     int64_t map_n) {
   using G = Geometry<E, DPAD>;
   constexpr int LB = G::LB;
   constexpr int PIECES = G::TILE / 16;  // 16-byte pieces per tile
@@ -527,16 +579,18 @@ __global__ void encode_packed_kernel(const float* __restrict__ x, int64_t n, int
                    sub * (16 / PackTraits<E>::kBytes);
     i32x4 piece = {0, 0, 0, 0};
     if (row < n) {
+      // Source row: row itself, or (row * map_step) mod map_n for a sample copy.
+      const int64_t src = map_step > 0 ? (int64_t)(((uint64_t)row * (uint64_t)map_step) % (uint64_t)map_n) : row;
       if (E == Elem::BF16) {
         uint16_t v[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = k0 + i < d ? bf16_rne(x[row * ldx + k0 + i]) : 0;
+        for (int i = 0; i < 8; ++i) v[i] = k0 + i < d ? bf16_rne(x[src * ldx + k0 + i]) : 0;
         __builtin_memcpy(&piece, v, 16);
       } else {
         const float s = scales[row];
         uint8_t v[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) v[i] = k0 + i < d ? e4m3_rne_sat(x[row * ldx + k0 + i] / s) : 0;
+        for (int i = 0; i < 16; ++i) v[i] = k0 + i < d ? e4m3_rne_sat(x[src * ldx + k0 + i] / s) : 0;
         __builtin_memcpy(&piece, v, 16);
       }
     }
@@ -575,15 +629,16 @@ constexpr int direct_tn() {
   return tn > 7 ? 7 : (tn < 1 ? 1 : tn);
 }
 
-// Query tiles per wave of the LDS variants (two waves per SIMD): "wide" keeps
-// the query operands within ~80 registers (at most 4 tiles), "narrow" half.
+// Query tiles per wave of the LDS variants: "wide" (two waves per SIMD)
+// keeps the query operands within ~80 registers (at most 4 tiles), "narrow"
+// (three waves per SIMD) one tile fewer.
 template <Elem E, int DPAD, bool WIDE>
 constexpr int lds_tn() {
   using G = Geometry<E, DPAD>;
   constexpr int per_tile = G::NC * G::LB / 4;
   constexpr int cap = E == Elem::FP8 ? 3 : 4;
   constexpr int wide = 80 / per_tile > cap ? cap : (80 / per_tile < 1 ? 1 : 80 / per_tile);
-  constexpr int narrow = wide / 2 < 1 ? 1 : wide / 2;
+  constexpr int narrow = wide - 1 < 1 ? 1 : wide - 1;
   return WIDE ? wide : narrow;
 }
 
@@ -605,10 +660,10 @@ void launch_packed_one(int variant, const FilterArgs& a, hipStream_t st) {
       filter_packed_kernel<E, DPAD, direct_tn<E, DPAD>()><<<grid, kBlock, 0, st>>>(a);
       break;
     case kFilterLdsNarrow:
-      filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, false>()><<<grid, kBlock, 0, st>>>(a);
+      filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, false>(), false><<<grid, kBlock, 0, st>>>(a);
       break;
     default:
-      filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, true>()><<<grid, kBlock, 0, st>>>(a);
+      filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, true>(), true><<<grid, kBlock, 0, st>>>(a);
       break;
   }
   RPLSH_HIP(hipGetLastError());
@@ -628,7 +683,20 @@ void encode_one(const float* x, int64_t n, int d, int64_t ldx, int64_t row_from,
   const int64_t work = (t1 - t0) * (G::TILE / 16);
   int64_t g = div_up(work, 256);
   if (g > 65536) g = 65536;
-  encode_packed_kernel<E, DPAD><<<(unsigned)g, 256, 0, st>>>(x, n, d, ldx, t0, t1, out, scales);
+  encode_packed_kernel<E, DPAD><<<(unsigned)g, 256, 0, st>>>(x, n, d, ldx, t0, t1, out, scales, 0, 0);
+  RPLSH_HIP(hipGetLastError());
+}
+
+// Sample copy: row r holds source row (r * step) mod n, r < sm.
+template <Elem E, int DPAD>
+void encode_sample_one(const float* x, int64_t n, int d, int64_t ldx, int64_t sm, int64_t step,
+                       char* out, const float* scales, hipStream_t st) {
+  using G = Geometry<E, DPAD>;
+  const int64_t t1 = div_up(sm, 32);
+  const int64_t work = t1 * (G::TILE / 16);
+  int64_t g = div_up(work, 256);
+  if (g > 65536) g = 65536;
+  encode_packed_kernel<E, DPAD><<<(unsigned)g, 256, 0, st>>>(x, sm, d, ldx, 0, t1, out, scales, step, n);
   RPLSH_HIP(hipGetLastError());
 }
 
@@ -657,7 +725,28 @@ auto dispatch(Elem e, int dpad, F&& f) {
   fail(RPLSH_ERR_UNSUPPORTED, "packed kernels: element kind / dimension %d not instantiated", dpad);
 }
 
+__global__ void sample_rows_kernel(const float* __restrict__ xn, const float* __restrict__ xscale,
+                                   int64_t n, int64_t sm, int64_t step, float* __restrict__ snorm,
+                                   float* __restrict__ sscale) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < sm) {
+    const int64_t src = (int64_t)(((uint64_t)r * (uint64_t)step) % (uint64_t)n);
+    snorm[r] = xn[src];
+    if (sscale) sscale[r] = xscale[src];
+  }
+}
+
+
 }  // namespace
+
+void launch_sample_rows(const float* xn, const float* xscale, int64_t n, int64_t sm, int64_t step,
+                        float* snorm, float* sscale, hipStream_t st) {
+  RPLSH_REQUIRE(sm > 0 && n > 0 && step > 0, "sample rows: bad arguments");
+  sample_rows_kernel<<<(unsigned)div_up(sm, (int64_t)256), 256, 0, st>>>(xn, xscale, n, sm, step,
+                                                                        snorm, sscale);
+  RPLSH_HIP(hipGetLastError());
+}
+
 
 int64_t packed_tile_bytes(Elem e, int dpad) {
   return e == Elem::BF16 ? (int64_t)32 * dpad * 2 : (int64_t)32 * dpad;
@@ -680,6 +769,21 @@ struct LaunchFilter {
   template <Elem E, int DPAD>
   int operator()() const {
     launch_packed_one<E, DPAD>(variant, a, st);
+    return 0;
+  }
+};
+
+struct EncodeSample {
+  const float* x;
+  int64_t n;
+  int d;
+  int64_t ldx, sm, step;
+  char* out;
+  const float* scales;
+  hipStream_t st;
+  template <Elem E, int DPAD>
+  int operator()() const {
+    encode_sample_one<E, DPAD>(x, n, d, ldx, sm, step, out, scales, st);
     return 0;
   }
 };
@@ -712,6 +816,11 @@ void launch_filter_packed(Elem e, int dpad, int variant, const FilterArgs& a, hi
 void launch_encode_packed(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx,
                           int64_t row_from, void* out, float* scales, hipStream_t st) {
   dispatch(e, dpad, EncodePacked{x, n, d, ldx, row_from, static_cast<char*>(out), scales, st});
+}
+
+void launch_encode_sample(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx, int64_t sm,
+                          int64_t step, void* out, const float* scales, hipStream_t st) {
+  dispatch(e, dpad, EncodeSample{x, n, d, ldx, sm, step, static_cast<char*>(out), scales, st});
 }
 
 }  // namespace rplsh

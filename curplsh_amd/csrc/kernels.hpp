@@ -43,6 +43,12 @@ int filter_packed_block_queries(Elem e, int dpad, int variant);
 void launch_filter_packed(Elem e, int dpad, int variant, const FilterArgs& a, hipStream_t st);
 // Encodes rows [row_from, n) (re-encoding the partial tile before row_from);
 // FP8 also writes the per-row power-of-two scales of rows [row_from, n).
+// Sample copy of n rows: row r < sm holds source row (r * step) mod n.
+// snorm[r] = xn[src], sscale[r] = xscale[src] (if sscale).
+void launch_sample_rows(const float* xn, const float* xscale, int64_t n, int64_t sm, int64_t step,
+                        float* snorm, float* sscale, hipStream_t st);
+void launch_encode_sample(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx, int64_t sm,
+                          int64_t step, void* out, const float* scales, hipStream_t st);
 void launch_encode_packed(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx,
                           int64_t row_from, void* out, float* scales, hipStream_t st);
 

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <numeric>
 #include <vector>
 
 #include "common.hpp"
@@ -104,9 +105,15 @@ static int64_t level_cap(const std::vector<Level>& L, int kp, int c_factor) {
   return cap;
 }
 
-static void run_filter(const DbView& db, const Level& lv, const void* qenc, const float* qscale,
-                       const float* tau, int nq, const int* nq_dev, int cap, int* counts,
-                       uint64_t* cand, int64_t target_wgs, int variant, hipStream_t st) {
+// One filter pass. A sample level (stride > 1) of a packed index whose sample
+// copy holds at least lv.m rows reads the first lv.m rows of that copy
+// instead of every stride-th row (its candidates only set a threshold, their
+// row numbers are never used). `dense`: tau is +inf for every query (first
+// pass), so every row survives and is written to its own slot.
+static void run_filter(const DbView& db, const Level& lv, bool dense, const void* qenc,
+                       const float* qscale, const float* tau, int nq, const int* nq_dev, int cap,
+                       int* counts, uint64_t* cand, int64_t target_wgs, int variant,
+                       hipStream_t st) {
   FilterArgs fa{};
   fa.x = db.xenc;
   fa.ldx = db.ldenc;
@@ -115,6 +122,13 @@ static void run_filter(const DbView& db, const Level& lv, const void* qenc, cons
   fa.xscale = db.xscale;
   fa.m = lv.m;
   fa.stride = lv.stride;
+  if (db.packed && lv.stride > 1 && db.srows >= lv.m) {
+    fa.x = db.senc;
+    fa.xscale = db.sscale;
+    fa.xnorm = db.snorm;
+    fa.stride = 1;
+  }
+  fa.dense = dense && db.packed && variant != kFilterDirect && lv.m <= cap ? 1 : 0;
   fa.q = qenc;
   fa.qscale = qscale;
   fa.tau = tau;
@@ -131,7 +145,7 @@ static void run_filter(const DbView& db, const Level& lv, const void* qenc, cons
   fa.cap = cap;
   fa.counts = counts;
   fa.cand = cand;
-  launch_fill_i32(counts, nq, 0, st);
+  launch_fill_i32(counts, nq, fa.dense ? (int)lv.m : 0, st);
   if (db.packed) {
     launch_filter_packed(db.elem, db.dpad, variant, fa, st);
   } else {
@@ -188,7 +202,8 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
     const Level& lv = L[li];
     const bool last = li + 1 == L.size();
     if (last) rec(1);
-    run_filter(db, lv, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs, p.variant, st);
+    run_filter(db, lv, li == 0, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs,
+               p.variant, st);
     if (last) rec(2);
     SelectArgs sa{};
     sa.cand = cand;
@@ -357,11 +372,50 @@ static Elem elem_of(int precision) {
 static int elem_bytes(Elem e) { return e == Elem::F32 ? 4 : (e == Elem::BF16 ? 2 : 1); }
 
 // (Re-)encode rows [from, n) of the candidate copy for the configured precision.
+static void bf_free_sample(rplsh_bf* b) {
+  if (b->senc) (void)hipFree(b->senc);
+  if (b->sscale) (void)hipFree(b->sscale);
+  if (b->snorm) (void)hipFree(b->snorm);
+  b->senc = nullptr;
+  b->sscale = nullptr;
+  b->snorm = nullptr;
+  b->srows = b->scap = b->sstep = 0;
+  b->sprec = -1;
+}
+
+// Sample copy for the sample levels: srows = max(n / 8, 32768) rows (at most
+// n), row r = (r * P) mod n with P ~ 0.618 n coprime to n, so every prefix is
+// a spread-out set of distinct rows. Rebuilt whenever the rows change.
+static void bf_encode_sample(rplsh_bf* b, Elem e, int dpad, hipStream_t st) {
+  constexpr int64_t kMinRows = 32768;
+  if (b->n < 2 * kMinRows) {
+    bf_free_sample(b);
+    return;
+  }
+  const int64_t sm = std::min<int64_t>(b->n, std::max<int64_t>(b->n / 8, kMinRows));
+  if (sm > b->scap || b->sprec != b->cfg.precision) {
+    bf_free_sample(b);
+    const int64_t tiles = div_up(sm + sm / 4, 32);  // room to grow with the index
+    RPLSH_HIP(hipMalloc(&b->senc, (size_t)(tiles * packed_tile_bytes(e, dpad))));
+    if (e == Elem::FP8) RPLSH_HIP(hipMalloc((void**)&b->sscale, (size_t)tiles * 32 * sizeof(float)));
+    RPLSH_HIP(hipMalloc((void**)&b->snorm, (size_t)tiles * 32 * sizeof(float)));
+    b->scap = tiles * 32;
+    b->sprec = b->cfg.precision;
+  }
+  int64_t p = (int64_t)std::llround(0.6180339887498949 * (double)b->n);
+  while (std::gcd(p, b->n) != 1) ++p;
+  b->srows = sm;
+  b->sstep = p;
+  launch_sample_rows(b->xn, b->xscale, b->n, sm, p, b->snorm, b->sscale, st);
+  launch_encode_sample(e, dpad, b->x, b->n, b->dim, b->ldx, sm, p, b->senc, b->sscale, st);
+}
+
 static void bf_encode(rplsh_bf* b, int64_t from, hipStream_t st) {
   const Elem e = elem_of(b->cfg.precision);
   if (e == Elem::F32) {
     b->enc_precision = b->cfg.precision;
     b->dpad = filter_dpad(e, b->dim);
+    bf_free_sample(b);
     return;
   }
   const int dpad = filter_dpad(e, b->dim);
@@ -382,6 +436,7 @@ static void bf_encode(rplsh_bf* b, int64_t from, hipStream_t st) {
   }
   if (b->n - from <= 0) return;
   launch_encode_packed(e, dpad, b->x, b->n, b->dim, b->ldx, from, b->xenc, b->xscale, st);
+  bf_encode_sample(b, e, dpad, st);
 }
 
 static DbView bf_view(const rplsh_bf* b) {
@@ -403,6 +458,10 @@ static DbView bf_view(const rplsh_bf* b) {
     v.ldenc = b->dpad;
     v.dvalid_bytes = b->dpad * elem_bytes(v.elem);
     v.xscale = b->xscale;
+    v.senc = b->senc;
+    v.sscale = b->sscale;
+    v.snorm = b->snorm;
+    v.srows = b->senc ? b->srows : 0;
     v.packed = true;
   }
   return v;
@@ -413,6 +472,7 @@ static void bf_free_data(rplsh_bf* b) {
   if (b->xn) (void)hipFree(b->xn);
   if (b->xenc) (void)hipFree(b->xenc);
   if (b->xscale) (void)hipFree(b->xscale);
+  bf_free_sample(b);
   b->x = nullptr;
   b->xn = nullptr;
   b->xenc = nullptr;
