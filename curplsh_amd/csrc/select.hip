@@ -162,6 +162,172 @@ __global__ __launch_bounds__(kSelBlock) void select_kernel(SelectArgs a) {
   }
 }
 
+// Wave-per-query select (threshold mode, and top-K mode for K <= kWaveSelK):
+// four 8-bit radix passes over the 32-bit key find the key T of the K-th
+// smallest word; only when several words share T, four more passes over
+// their indices find the K-th word exactly. A histogram lives in LDS, its
+// scan is a wave prefix sum: no workgroup barriers. Candidates are read from
+// global memory (L2) once per pass.
+constexpr int kWaveSelK = 512;
+constexpr int kWaveSelQ = 4;  // queries (waves) per 256-thread block
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Inclusive prefix sum over the 64 lanes.
+__device__ inline int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+// One 8-bit radix step: among values v (selected by `pick`) finds the bin
+// holding the kr-th smallest; returns the bin, subtracts the counts of lower
+// bins from kr and sets cnt to the bin's count.
+template <typename Get>
+__device__ inline int wave_radix_step(int* hist, int lane, int n, int& kr, int& cnt, Get get) {
+  reinterpret_cast<int4*>(hist)[lane] = int4{0, 0, 0, 0};
+  wave_sync();
+  for (int i = lane; i < n; i += 64) {
+    int b;
+    if (get(i, b)) atomicAdd(&hist[b], 1);
+  }
+  wave_sync();
+  const int4 h4 = reinterpret_cast<const int4*>(hist)[lane];
+  const int local = h4.x + h4.y + h4.z + h4.w;
+  const int incl = wave_incl_scan(local, lane);
+  const int excl = incl - local;
+  const bool mine = excl < kr && kr <= incl;
+  int bin = lane * 4, before = excl, c = h4.x;
+  if (mine) {
+    if (kr > before + h4.x) { before += h4.x; ++bin; c = h4.y;
+      if (kr > before + h4.y) { before += h4.y; ++bin; c = h4.z;
+        if (kr > before + h4.z) { before += h4.z; ++bin; c = h4.w; } } }
+  }
+  const unsigned long long m = __ballot(mine);
+  const int src = m ? __builtin_ctzll(m) : 0;
+  bin = __shfl(bin, src, 64);
+  before = __shfl(before, src, 64);
+  cnt = __shfl(c, src, 64);
+  kr -= before;
+  wave_sync();
+  return bin;
+}
+
+__global__ __launch_bounds__(64 * kWaveSelQ) void select_wave_kernel(SelectArgs a) {
+  __shared__ __attribute__((aligned(16))) int hist_all[kWaveSelQ][256];
+  __shared__ uint64_t out_all[kWaveSelQ][kWaveSelK];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int q = blockIdx.x * kWaveSelQ + w;
+  if (q >= a.nq || (a.nq_dev && q >= *a.nq_dev)) return;
+  int* hist = hist_all[w];
+  uint64_t* out = out_all[w];
+  const int count = a.counts[q];
+  const int n = count < a.cap ? count : a.cap;
+  int st = 0;
+  if (count > a.cap) st |= kStatusOverflow;
+  const int K = a.K;
+  if (n < K) st |= kStatusShort;
+  const uint64_t* src = a.cand + (int64_t)q * a.cap;
+
+  // Key T of the K-th smallest word; kr: its rank among the words with key T.
+  uint32_t T = 0xffffffffu;
+  int kr = K, cnt = 0;
+  const bool ranked = n >= K && K > 0;
+  if (ranked) {
+    uint32_t prefix = 0;
+#pragma unroll 1
+    for (int pass = 3; pass >= 0; --pass) {
+      const int shift = 8 * pass;
+      const uint32_t hmask = pass == 3 ? 0u : (0xffffffffu << (shift + 8));
+      const int bin = wave_radix_step(hist, lane, n, kr, cnt, [&](int i, int& b) {
+        const uint32_t k = (uint32_t)(src[i] >> 32);
+        b = (int)((k >> shift) & 255u);
+        return (k & hmask) == prefix;
+      });
+      prefix |= (uint32_t)bin << shift;
+    }
+    T = prefix;
+  }
+
+  if (!a.topk_mode) {
+    if (lane == 0) {
+      a.tau_out[q] = ranked ? key_from_bits(T) : a.tau_in[q];
+      if (st && a.status) atomicOr(a.status + q, st);
+    }
+    return;
+  }
+
+  // Top-K mode: W = the K-th smallest word; every word <= W is selected.
+  uint64_t W = kEmptyCand;
+  if (ranked && n > K) {
+    uint32_t idx = 0xffffffffu;
+    if (kr < cnt) {  // ties at T: the kr-th smallest index among them
+      uint32_t prefix = 0;
+#pragma unroll 1
+      for (int pass = 3; pass >= 0; --pass) {
+        const int shift = 8 * pass;
+        const uint32_t hmask = pass == 3 ? 0u : (0xffffffffu << (shift + 8));
+        int c2 = 0;
+        const int bin = wave_radix_step(hist, lane, n, kr, c2, [&](int i, int& b) {
+          const uint64_t v = src[i];
+          const uint32_t lo = (uint32_t)v;
+          b = (int)((lo >> shift) & 255u);
+          return (uint32_t)(v >> 32) == T && (lo & hmask) == prefix;
+        });
+        prefix |= (uint32_t)bin << shift;
+      }
+      idx = prefix;
+    }
+    W = ((uint64_t)T << 32) | idx;
+  }
+  const int take = n < K ? n : K;
+  int base = 0;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i = i0 + lane;
+    const uint64_t v = i < n ? src[i] : kEmptyCand;
+    const bool sel = i < n && v <= W;
+    const unsigned long long m = __ballot(sel);
+    if (sel) {
+      const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < kWaveSelK) out[pos] = v;
+    }
+    base += __popcll(m);
+  }
+  int P = 1;
+  while (P < take) P <<= 1;
+  for (int i = take + lane; i < P; i += 64) out[i] = kEmptyCand;
+  wave_sync();
+  // Bitonic sort of out[0..P), one wave.
+  for (int k2 = 2; k2 <= P; k2 <<= 1) {
+    for (int j = k2 >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < (P >> 1); i += 64) {
+        const int lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+        const int hi = lo + j;
+        const bool up = (lo & k2) == 0;
+        const uint64_t x = out[lo], y = out[hi];
+        if ((x > y) == up) {
+          out[lo] = y;
+          out[hi] = x;
+        }
+      }
+      wave_sync();
+    }
+  }
+  uint64_t* dst = a.topk_out + (int64_t)q * K;
+  for (int i = lane; i < K; i += 64) dst[i] = i < take ? out[i] : kEmptyCand;
+  if (lane == 0) {
+    a.bound_out[q] = count > K && K > 0 ? key_from_bits(T) : a.tau_in[q];
+    if (st && a.status) atomicOr(a.status + q, st);
+  }
+}
+
 // Canonical FP32 relative distance of query sq (in LDS) to row idx.
 __device__ inline float canonical_rel(const float* __restrict__ x, int64_t ldx,
                                       const float* __restrict__ xnorm, int d, const float* sq,
@@ -371,7 +537,11 @@ inline unsigned grid_for(int64_t work) {
 void launch_select(const SelectArgs& a, hipStream_t st) {
   if (a.nq <= 0) return;
   if (a.topk_mode && a.K > kMaxSortK) fail(RPLSH_ERR_UNSUPPORTED, "top-K select limited to %d", kMaxSortK);
-  select_kernel<<<a.nq, kSelBlock, 0, st>>>(a);
+  if (!a.topk_mode || a.K <= kWaveSelK) {
+    select_wave_kernel<<<(unsigned)div_up(a.nq, kWaveSelQ), 64 * kWaveSelQ, 0, st>>>(a);
+  } else {
+    select_kernel<<<a.nq, kSelBlock, 0, st>>>(a);
+  }
   RPLSH_HIP(hipGetLastError());
 }
 
