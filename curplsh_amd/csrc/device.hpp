@@ -49,6 +49,8 @@ struct FilterArgs {
   int dvalid_bytes;       // bytes of a row holding data; the rest of the row is zero
   const float* xnorm;     // [n] squared norms (FP32, canonical)
   const float* xscale;    // [n] per-row scale (FP8 only)
+  const float* xnh;       // packed ring kernels: -||x||^2 / 2 per row; xnh[xnh_inf] = -inf
+  int64_t xnh_inf;
   int64_t m;              // logical rows of this pass
   int64_t stride;         // physical row = logical row * stride
   int64_t rows_per_slice; // logical rows per workgroup slice (multiple of 64)

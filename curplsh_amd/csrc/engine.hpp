@@ -102,6 +102,12 @@ struct DbView {
   const float* sscale = nullptr;
   const float* snorm = nullptr;
   int64_t srows = 0;
+  // -||x||^2 / 2 per row, with one -inf entry (index *_inf) for rows past
+  // the end: the C operand the ring kernels copy into LDS as is.
+  const float* xnh = nullptr;
+  int64_t xnh_inf = 0;
+  const float* snh = nullptr;
+  int64_t snh_inf = 0;
   bool packed = false;            // xenc is in 32-row MFMA tiles (filter_packed.hip)
 };
 
@@ -167,6 +173,8 @@ struct rplsh_bf {
   void* senc = nullptr;
   float* sscale = nullptr;
   float* snorm = nullptr;
+  float* snh = nullptr;   // -snorm / 2, then one -inf
+  float* xnh = nullptr;   // packed copies: -||x||^2 / 2, then one -inf
   int64_t srows = 0, scap = 0, sstep = 0;
   int sprec = -1;  // precision the sample buffers were allocated for
   int enc_precision = -1;  // precision xenc currently holds

@@ -531,6 +531,263 @@ __global__ __launch_bounds__(kBlock, (StageShape<E, DPAD, WIDE>::WAVES_PER_SIMD)
   flush(1, qtail[1]);
 }
 
+// Ring variants (wide / narrow query tiles): the same tiles, waves, queries
+// and arithmetic as the LDS variants, different data movement. Stages are
+// copied global -> LDS by the LDS-DMA path (global_load_lds_dwordx4: no
+// staging registers, no VALU) into a ring of kRing slots, so the copies of
+// stages s+1 and s+2 are in flight while stage s is computed. One barrier per
+// stage; before it each wave waits (vmcnt) only for its own copies of stage s.
+// The C operand -||x||^2/2 is copied in beside the tile (-inf for rows past
+// the end), FP8 row scales as floats.
+constexpr int kRing = 3;
+
+template <Elem E, int DPAD, bool WIDE>
+struct RingShape {
+  using G = Geometry<E, DPAD>;
+  static constexpr int kFit = (WIDE ? 56 * 1024 : 36 * 1024) / (kRing * G::TILE);
+  static constexpr int SM = kFit > 4 ? 4 : (kFit < 1 ? 1 : kFit);  // tiles per stage
+  static constexpr int SR = 32 * SM;
+  static constexpr int STAGE = SM * G::TILE;
+  static constexpr int DATA_OPS = STAGE / 1024;  // one op: 64 lanes x 16 B
+  static constexpr int ROW_OPS = (SR + 63) / 64;  // ops per per-row array
+  static constexpr int OPS = DATA_OPS + ROW_OPS * (E == Elem::FP8 ? 2 : 1);
+  static constexpr int LDS = kRing * STAGE + kRing * SR * 8 + 2 * kQueue * 12;
+  static constexpr int WAVES_PER_SIMD = (!WIDE && 3 * LDS <= 160 * 1024) ? 3 : 2;
+  static_assert(STAGE % 1024 == 0, "stage is whole 1 KB copies");
+};
+
+// s_waitcnt vmcnt(N) lgkmcnt(0) (gfx9 encoding; expcnt not waited on).
+template <int N>
+__device__ inline void wait_vm_lgkm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4));
+}
+
+// Workgroup barrier without the release fence of __syncthreads (which would
+// wait for every copy in flight); the caller waits for what must be complete.
+__device__ inline void stage_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+template <Elem E, int DPAD, int TN, bool WIDE>
+__global__ __launch_bounds__(kBlock, (RingShape<E, DPAD, WIDE>::WAVES_PER_SIMD)) void filter_ring_kernel(
+    FilterArgs a) {
+  using G = Geometry<E, DPAD>;
+  using RS = RingShape<E, DPAD, WIDE>;
+  constexpr int LB = G::LB;
+  constexpr int NC = G::NC;
+  constexpr int BQ = 4 * TN * 32;
+  constexpr int SM = RS::SM;
+  constexpr int SR = RS::SR;
+  constexpr int TILE_UNITS = G::TILE / 16;
+  constexpr int CHUNK_UNITS = G::CHUNK / 16;
+  constexpr int LANE_UNITS = LB / 16;
+  typedef __attribute__((address_space(3))) void lds_void;
+
+  __shared__ __attribute__((aligned(16))) char sx[kRing][RS::STAGE];
+  __shared__ __attribute__((aligned(16))) float sn[kRing][SR];
+  __shared__ __attribute__((aligned(16))) float se[kRing][E == Elem::FP8 ? SR : 1];
+  __shared__ uint64_t qword[2][kQueue];
+  __shared__ int qquery[2][kQueue];
+  __shared__ unsigned qtail[2];
+
+  const int per_xcd = (int)(gridDim.x >> 3);
+  const int logical = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (logical >= a.num_qtiles * a.num_slices) return;
+  const int qtile = logical % a.num_qtiles;
+  const int slice = logical / a.num_qtiles;
+  const int nq = a.nq_dev ? min(*a.nq_dev, a.nq) : a.nq;
+  if (qtile * BQ >= nq) return;
+  const int64_t r_begin = (int64_t)slice * a.rows_per_slice;
+  int64_t r_end = r_begin + a.rows_per_slice;
+  if (r_end > a.m) r_end = a.m;
+  if (r_begin >= r_end) return;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int h = lane >> 5, col = lane & 31;
+
+  Frag<LB> qf[TN][NC];
+  float thr[TN];
+  int qsc[TN];
+  int qrow[TN];
+#pragma unroll
+  for (int nt = 0; nt < TN; ++nt) {
+    const int qr = qtile * BQ + (wave * TN + nt) * 32 + col;
+    qrow[nt] = qr;
+    if (qr < nq) {
+      const char* qp = reinterpret_cast<const char*>(a.q) + (int64_t)qr * G::ROWB;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) qf[nt][c] = load_frag<LB>(qp + 2 * LB * c + LB * h);
+      thr[nt] = fmaxf(-0.5f * a.tau[qr], -FLT_MAX);
+      qsc[nt] = E == Elem::FP8 ? e8m0_of(a.qscale[qr]) : 127;
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) qf[nt][c] = zero_frag<LB>();
+      thr[nt] = INFINITY;
+      qsc[nt] = 127;
+    }
+  }
+
+  const char* xb = reinterpret_cast<const char*>(a.x);
+  const int64_t stride = a.stride;
+  const int n_st = (int)((r_end - r_begin + SR - 1) / SR);
+
+  // Copies stage [j0, j0 + SR) into ring slot `slot`. Wave w issues ops w,
+  // w + 4, ...: every wave issues at least OPS / 4 per stage.
+  auto issue = [&](int slot, int64_t j0) {
+    const bool whole = stride == 1 && j0 + SR <= r_end;
+    for (int o = wave; o < RS::OPS; o += 4) {
+      if (o < RS::DATA_OPS) {
+        const int u = o * 64 + lane;
+        const char* src;
+        if (whole) {
+          src = xb + (j0 >> 5) * G::TILE + (int64_t)u * 16;
+        } else {
+          const int mt = u / TILE_UNITS;
+          const int w = u - mt * TILE_UNITS;
+          const int c = w / CHUNK_UNITS;
+          const int l = (w - c * CHUNK_UNITS) / LANE_UNITS;
+          const int sub = w % LANE_UNITS;
+          int64_t j = j0 + mt * 32 + (l & 31);
+          if (j >= r_end) j = r_end - 1;  // any valid row: its C operand is -inf
+          const int64_t p = j * stride;
+          src = xb + (p >> 5) * G::TILE + c * G::CHUNK + ((p & 31) + 32 * (l >> 5)) * LB + sub * 16;
+        }
+        __builtin_amdgcn_global_load_lds(src, (lds_void*)&sx[slot][o * 1024], 16, 0, 0);
+      } else {
+        const int r = o - RS::DATA_OPS;
+        const int part = r % RS::ROW_OPS;
+        const int row = part * 64 + lane;
+        if (row < SR) {
+          const int64_t j = j0 + row;
+          if (r < RS::ROW_OPS) {
+            const float* src = j < r_end ? a.xnh + j * stride : a.xnh + a.xnh_inf;
+            __builtin_amdgcn_global_load_lds(src, (lds_void*)&sn[slot][part * 64], 4, 0, 0);
+          } else {
+            const float* src = a.xscale + (j < r_end ? j : r_end - 1) * stride;
+            __builtin_amdgcn_global_load_lds(src, (lds_void*)&se[slot][part * 64], 4, 0, 0);
+          }
+        }
+      }
+    }
+  };
+
+  unsigned head[2] = {0u, 0u};
+  auto emit = [&](int par, int qr, float v, int64_t row) {
+    const uint64_t w = pack_cand(-2.0f * v, (int32_t)(row * stride));
+    const unsigned t = atomicAdd(&qtail[par], 1u);
+    if (t - head[par] < (unsigned)kQueue) {
+      qword[par][t & (kQueue - 1)] = w;
+      qquery[par][t & (kQueue - 1)] = qr;
+    } else {
+      const int pos = atomicAdd(a.counts + qr, 1);
+      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = w;
+    }
+  };
+  auto flush = [&](int par, unsigned tail) {
+    const unsigned n = min(tail - head[par], (unsigned)kQueue);
+    for (unsigned i = tid; i < n; i += kBlock) {
+      const unsigned t = (head[par] + i) & (kQueue - 1);
+      const int qr = qquery[par][t];
+      const int pos = atomicAdd(a.counts + qr, 1);
+      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = qword[par][t];
+    }
+    head[par] = tail;
+  };
+
+  if (tid < 2) qtail[tid] = 0u;
+  auto run = [&](auto dense_tag) {
+    constexpr bool DENSE = decltype(dense_tag)::value;
+#pragma unroll
+    for (int i = 0; i < kRing - 1; ++i)
+      if (i < n_st) issue(i, r_begin + (int64_t)i * SR);
+    int slot = 0;
+    for (int s = 0; s < n_st; ++s) {
+      const int64_t j0 = r_begin + (int64_t)s * SR;
+      // This wave's copies of stage s are done when at most those of the
+      // later stages already issued are outstanding.
+      if (s + 1 < n_st) {
+        wait_vm_lgkm<RS::OPS / 4>();
+      } else {
+        wait_vm_lgkm<0>();
+      }
+      stage_barrier();
+      if (s + kRing - 1 < n_st) issue(slot == 0 ? kRing - 1 : slot - 1, j0 + (int64_t)(kRing - 1) * SR);
+      if (!DENSE && s > 0) {
+        const int par = (s - 1) & 1;
+        const unsigned tail = qtail[par];
+        if (tail - head[par] > (unsigned)(kQueue / 2)) flush(par, tail);
+      }
+      const int par = s & 1;
+#pragma unroll 1
+      for (int mt = 0; mt < SM; ++mt) {
+        f32x16 ci;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const f32x4 v = *reinterpret_cast<const f32x4*>(&sn[slot][mt * 32 + 8 * g + 4 * h]);
+          ci[4 * g + 0] = v[0];
+          ci[4 * g + 1] = v[1];
+          ci[4 * g + 2] = v[2];
+          ci[4 * g + 3] = v[3];
+        }
+        const int xsc = E == Elem::FP8 ? e8m0_of(se[slot][E == Elem::FP8 ? mt * 32 + col : 0]) : 127;
+        const char* ap = &sx[slot][mt * G::TILE + lane * LB];
+        f32x16 acc[TN];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const Frag<LB> af = load_frag<LB>(ap + c * G::CHUNK);
+#pragma unroll
+          for (int nt = 0; nt < TN; ++nt)
+            acc[nt] = mfma_chunk<E>(af, qf[nt][c], c == 0 ? ci : acc[nt], xsc, qsc[nt]);
+        }
+        if (DENSE) {
+#pragma unroll
+          for (int nt = 0; nt < TN; ++nt) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+              const int64_t row = j0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+              if (row < r_end && qrow[nt] < nq)
+                a.cand[(int64_t)qrow[nt] * a.cap + row] = pack_cand(-2.0f * acc[nt][i], (int32_t)(row * stride));
+            }
+          }
+          continue;
+        }
+        float mx[TN];
+        bool hit = false;
+#pragma unroll
+        for (int nt = 0; nt < TN; ++nt) {
+          float v = acc[nt][0];
+#pragma unroll
+          for (int i = 1; i < 16; ++i) v = fmaxf(v, acc[nt][i]);
+          mx[nt] = v;
+          hit |= v >= thr[nt];
+        }
+        if (__any(hit)) {
+#pragma unroll
+          for (int nt = 0; nt < TN; ++nt) {
+            if (mx[nt] >= thr[nt]) {
+#pragma unroll
+              for (int i = 0; i < 16; ++i) {
+                const float v = acc[nt][i];
+                if (v >= thr[nt]) emit(par, qrow[nt], v, j0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3));
+              }
+            }
+          }
+        }
+      }
+      slot = slot == kRing - 1 ? 0 : slot + 1;
+    }
+  };
+  if (a.dense) {
+    run(std::true_type{});
+    return;
+  }
+  run(std::false_type{});
+  __syncthreads();
+  flush(0, qtail[0]);
+  flush(1, qtail[1]);
+}
+
 // Packed encoders: one thread per 16-byte output piece.
 __device__ inline uint16_t bf16_rne(float v) {
   uint32_t u = __float_as_uint(v);
@@ -646,7 +903,8 @@ template <Elem E, int DPAD>
 int tn_of(int variant) {
   switch (variant) {
     case kFilterDirect: return direct_tn<E, DPAD>();
-    case kFilterLdsNarrow: return lds_tn<E, DPAD, false>();
+    case kFilterLdsNarrow:
+    case kFilterRingNarrow: return lds_tn<E, DPAD, false>();
     default: return lds_tn<E, DPAD, true>();
   }
 }
@@ -661,6 +919,14 @@ void launch_packed_one(int variant, const FilterArgs& a, hipStream_t st) {
       break;
     case kFilterLdsNarrow:
       filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, false>(), false><<<grid, kBlock, 0, st>>>(a);
+      break;
+    case kFilterRingWide:
+      RPLSH_REQUIRE(a.xnh, "ring filter: C-operand array missing");
+      filter_ring_kernel<E, DPAD, lds_tn<E, DPAD, true>(), true><<<grid, kBlock, 0, st>>>(a);
+      break;
+    case kFilterRingNarrow:
+      RPLSH_REQUIRE(a.xnh, "ring filter: C-operand array missing");
+      filter_ring_kernel<E, DPAD, lds_tn<E, DPAD, false>(), false><<<grid, kBlock, 0, st>>>(a);
       break;
     default:
       filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, true>(), true><<<grid, kBlock, 0, st>>>(a);
@@ -727,23 +993,41 @@ auto dispatch(Elem e, int dpad, F&& f) {
 
 __global__ void sample_rows_kernel(const float* __restrict__ xn, const float* __restrict__ xscale,
                                    int64_t n, int64_t sm, int64_t step, float* __restrict__ snorm,
+                                   float* __restrict__ snh, int64_t nh_inf,
                                    float* __restrict__ sscale) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r < sm) {
     const int64_t src = (int64_t)(((uint64_t)r * (uint64_t)step) % (uint64_t)n);
     snorm[r] = xn[src];
+    snh[r] = -0.5f * xn[src];
     if (sscale) sscale[r] = xscale[src];
   }
+  if (r == 0) snh[nh_inf] = -INFINITY;
+}
+
+__global__ void neg_half_norms_kernel(const float* __restrict__ xn, int64_t from, int64_t n,
+                                      float* __restrict__ nh, int64_t inf_at) {
+  const int64_t i = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) nh[i] = -0.5f * xn[i];
+  if (i == from) nh[inf_at] = -INFINITY;
 }
 
 
 }  // namespace
 
 void launch_sample_rows(const float* xn, const float* xscale, int64_t n, int64_t sm, int64_t step,
-                        float* snorm, float* sscale, hipStream_t st) {
-  RPLSH_REQUIRE(sm > 0 && n > 0 && step > 0, "sample rows: bad arguments");
+                        float* snorm, float* snh, int64_t nh_inf, float* sscale, hipStream_t st) {
+  RPLSH_REQUIRE(sm > 0 && n > 0 && step > 0 && sm <= nh_inf, "sample rows: bad arguments");
   sample_rows_kernel<<<(unsigned)div_up(sm, (int64_t)256), 256, 0, st>>>(xn, xscale, n, sm, step,
-                                                                        snorm, sscale);
+                                                                        snorm, snh, nh_inf, sscale);
+  RPLSH_HIP(hipGetLastError());
+}
+
+void launch_neg_half_norms(const float* xn, int64_t from, int64_t n, float* nh, int64_t inf_at,
+                           hipStream_t st) {
+  RPLSH_REQUIRE(from >= 0 && n <= inf_at && nh, "half norms: bad range");
+  const int64_t cnt = std::max<int64_t>(n - from, 1);
+  neg_half_norms_kernel<<<(unsigned)div_up(cnt, (int64_t)256), 256, 0, st>>>(xn, from, n, nh, inf_at);
   RPLSH_HIP(hipGetLastError());
 }
 

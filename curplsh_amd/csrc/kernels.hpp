@@ -38,15 +38,25 @@ void launch_filter(Elem e, int dpad, const FilterArgs& a, hipStream_t st);
 int64_t packed_tile_bytes(Elem e, int dpad);
 // Variants of the packed filter (chosen by the autotuner): LDS-staged with
 // wide or narrow query tiles per wave, or direct-to-register loads.
-enum FilterVariant : int { kFilterLdsWide = 0, kFilterLdsNarrow = 1, kFilterDirect = 2 };
+enum FilterVariant : int {
+  kFilterLdsWide = 0,
+  kFilterLdsNarrow = 1,
+  kFilterDirect = 2,
+  kFilterRingWide = 3,
+  kFilterRingNarrow = 4,
+};
 int filter_packed_block_queries(Elem e, int dpad, int variant);
 void launch_filter_packed(Elem e, int dpad, int variant, const FilterArgs& a, hipStream_t st);
 // Encodes rows [row_from, n) (re-encoding the partial tile before row_from);
 // FP8 also writes the per-row power-of-two scales of rows [row_from, n).
 // Sample copy of n rows: row r < sm holds source row (r * step) mod n.
-// snorm[r] = xn[src], sscale[r] = xscale[src] (if sscale).
+// snorm[r] = xn[src], snh[r] = -xn[src] / 2, sscale[r] = xscale[src] (if
+// sscale), snh[nh_inf] = -inf.
 void launch_sample_rows(const float* xn, const float* xscale, int64_t n, int64_t sm, int64_t step,
-                        float* snorm, float* sscale, hipStream_t st);
+                        float* snorm, float* snh, int64_t nh_inf, float* sscale, hipStream_t st);
+// nh[i] = -xn[i] / 2 for i in [from, n); nh[inf_at] = -inf.
+void launch_neg_half_norms(const float* xn, int64_t from, int64_t n, float* nh, int64_t inf_at,
+                           hipStream_t st);
 void launch_encode_sample(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx, int64_t sm,
                           int64_t step, void* out, const float* scales, hipStream_t st);
 void launch_encode_packed(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx,

@@ -120,12 +120,16 @@ static void run_filter(const DbView& db, const Level& lv, bool dense, const void
   fa.dvalid_bytes = db.dvalid_bytes;
   fa.xnorm = db.xnorm;
   fa.xscale = db.xscale;
+  fa.xnh = db.xnh;
+  fa.xnh_inf = db.xnh_inf;
   fa.m = lv.m;
   fa.stride = lv.stride;
   if (db.packed && lv.stride > 1 && db.srows >= lv.m) {
     fa.x = db.senc;
     fa.xscale = db.sscale;
     fa.xnorm = db.snorm;
+    fa.xnh = db.snh;
+    fa.xnh_inf = db.snh_inf;
     fa.stride = 1;
   }
   fa.dense = dense && db.packed && variant != kFilterDirect && lv.m <= cap ? 1 : 0;
@@ -376,9 +380,11 @@ static void bf_free_sample(rplsh_bf* b) {
   if (b->senc) (void)hipFree(b->senc);
   if (b->sscale) (void)hipFree(b->sscale);
   if (b->snorm) (void)hipFree(b->snorm);
+  if (b->snh) (void)hipFree(b->snh);
   b->senc = nullptr;
   b->sscale = nullptr;
   b->snorm = nullptr;
+  b->snh = nullptr;
   b->srows = b->scap = b->sstep = 0;
   b->sprec = -1;
 }
@@ -399,6 +405,7 @@ static void bf_encode_sample(rplsh_bf* b, Elem e, int dpad, hipStream_t st) {
     RPLSH_HIP(hipMalloc(&b->senc, (size_t)(tiles * packed_tile_bytes(e, dpad))));
     if (e == Elem::FP8) RPLSH_HIP(hipMalloc((void**)&b->sscale, (size_t)tiles * 32 * sizeof(float)));
     RPLSH_HIP(hipMalloc((void**)&b->snorm, (size_t)tiles * 32 * sizeof(float)));
+    RPLSH_HIP(hipMalloc((void**)&b->snh, (size_t)(tiles * 32 + 1) * sizeof(float)));
     b->scap = tiles * 32;
     b->sprec = b->cfg.precision;
   }
@@ -406,7 +413,7 @@ static void bf_encode_sample(rplsh_bf* b, Elem e, int dpad, hipStream_t st) {
   while (std::gcd(p, b->n) != 1) ++p;
   b->srows = sm;
   b->sstep = p;
-  launch_sample_rows(b->xn, b->xscale, b->n, sm, p, b->snorm, b->sscale, st);
+  launch_sample_rows(b->xn, b->xscale, b->n, sm, p, b->snorm, b->snh, b->scap, b->sscale, st);
   launch_encode_sample(e, dpad, b->x, b->n, b->dim, b->ldx, sm, p, b->senc, b->sscale, st);
 }
 
@@ -423,12 +430,15 @@ static void bf_encode(rplsh_bf* b, int64_t from, hipStream_t st) {
   if (b->enc_precision != b->cfg.precision || !b->xenc) {
     if (b->xenc) RPLSH_HIP(hipFree(b->xenc));
     if (b->xscale) RPLSH_HIP(hipFree(b->xscale));
+    if (b->xnh) RPLSH_HIP(hipFree(b->xnh));
     b->xenc = nullptr;
     b->xscale = nullptr;
+    b->xnh = nullptr;
     if (b->capacity > 0) {
       const int64_t tiles = div_up(b->capacity, 32);
       RPLSH_HIP(hipMalloc(&b->xenc, (size_t)(tiles * packed_tile_bytes(e, dpad))));
       if (e == Elem::FP8) RPLSH_HIP(hipMalloc((void**)&b->xscale, (size_t)tiles * 32 * sizeof(float)));
+      RPLSH_HIP(hipMalloc((void**)&b->xnh, (size_t)(tiles * 32 + 1) * sizeof(float)));
     }
     b->enc_precision = b->cfg.precision;
     b->dpad = dpad;
@@ -436,6 +446,7 @@ static void bf_encode(rplsh_bf* b, int64_t from, hipStream_t st) {
   }
   if (b->n - from <= 0) return;
   launch_encode_packed(e, dpad, b->x, b->n, b->dim, b->ldx, from, b->xenc, b->xscale, st);
+  launch_neg_half_norms(b->xn, from, b->n, b->xnh, div_up(b->capacity, 32) * 32, st);
   bf_encode_sample(b, e, dpad, st);
 }
 
@@ -461,6 +472,10 @@ static DbView bf_view(const rplsh_bf* b) {
     v.senc = b->senc;
     v.sscale = b->sscale;
     v.snorm = b->snorm;
+    v.snh = b->snh;
+    v.snh_inf = b->scap;
+    v.xnh = b->xnh;
+    v.xnh_inf = div_up(b->capacity, 32) * 32;
     v.srows = b->senc ? b->srows : 0;
     v.packed = true;
   }
@@ -473,6 +488,8 @@ static void bf_free_data(rplsh_bf* b) {
   if (b->xenc) (void)hipFree(b->xenc);
   if (b->xscale) (void)hipFree(b->xscale);
   bf_free_sample(b);
+  if (b->xnh) (void)hipFree(b->xnh);
+  b->xnh = nullptr;
   b->x = nullptr;
   b->xn = nullptr;
   b->xenc = nullptr;
@@ -621,8 +638,10 @@ rplsh_status rplsh_bf_add(rplsh_bf* b, int64_t num, const float* data) {
       if (b->xn) RPLSH_HIP(hipFree(b->xn));
       if (b->xenc) RPLSH_HIP(hipFree(b->xenc));
       if (b->xscale) RPLSH_HIP(hipFree(b->xscale));
+      if (b->xnh) RPLSH_HIP(hipFree(b->xnh));
       b->xenc = nullptr;
       b->xscale = nullptr;
+      b->xnh = nullptr;
       b->enc_precision = -1;
       b->x = nx;
       b->xn = nn;
@@ -959,24 +978,29 @@ rplsh_status rplsh_bf_autotune(rplsh_bf* b, const float* sample, int64_t nsample
     // (survivor multiple c, rank rho of the last threshold): the last sample
     // level has stride c*k'/rho, so a larger rho buys a tighter c at the same
     // risk P(Gamma(rho) < rho/c) of a query keeping fewer than k' rows.
-    const int wgs[] = {2048, 4096, 8192};
+    const int wgs[] = {4096, 8192, 16384};
     const int cr[][2] = {{3, 0}, {4, 16}, {2, 32}, {3, 32}, {2, 64}, {8, 8}};
-    const int nvar = v.packed ? 2 : 1;
-    for (int var = 0; var < nvar; ++var)
+    // Packed copies: LDS wide / narrow, ring wide (the direct and ring-narrow
+    // variants are left to explicit tuning entries).
+    const int vars_packed[] = {kFilterLdsWide, kFilterLdsNarrow, kFilterRingWide};
+    const int vars_plain[] = {kFilterLdsWide};
+    const int* vars = v.packed ? vars_packed : vars_plain;
+    const int nvar = v.packed ? 3 : 1;
+    for (int vi = 0; vi < nvar; ++vi)
       for (int tw : wgs)
         for (const auto& c : cr) {
           SearchParams p = base;
           p.target_wgs = tw;
           p.c_factor = c[0];
           p.rho = c[1];
-          p.variant = var;
+          p.variant = vars[vi];
           const float ms = time_it(p);
           if (ms < r.ms_tuned) {
             r.ms_tuned = ms;
             r.target_wgs = tw;
             r.c_factor = c[0];
             r.rho = c[1];
-            r.variant = var;
+            r.variant = vars[vi];
           }
         }
     (void)hipEventDestroy(e0);
@@ -996,7 +1020,7 @@ rplsh_status rplsh_bf_set_tuning(rplsh_bf* b, const rplsh_tune_result* t) {
     RPLSH_REQUIRE(b && t, "NULL argument");
     RPLSH_REQUIRE(t->k > 0 && t->batch_nq > 0, "bad tuning entry");
     RPLSH_REQUIRE(t->kprime_factor >= 0 && (int64_t)t->kprime_factor * t->k <= 4096, "k' too large");
-    RPLSH_REQUIRE(t->variant >= 0 && t->variant <= 2, "unknown filter variant %d", t->variant);
+    RPLSH_REQUIRE(t->variant >= kFilterLdsWide && t->variant <= kFilterRingNarrow, "unknown filter variant %d", t->variant);
     RPLSH_REQUIRE(t->rho >= 0 && t->rho <= 256, "bad rho %d", t->rho);
     b->tunes.erase(std::remove_if(b->tunes.begin(), b->tunes.end(),
                                   [&](const rplsh_tune_result& x) {

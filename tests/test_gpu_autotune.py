@@ -19,8 +19,10 @@ def test_autotune_precision_form(ca, orc, res, tmp_path):
     print("tuned:", t)
     assert 1 <= t["kprime_factor"] <= 16
     assert t["recall"] >= 0.999 or t["kprime_factor"] == 16
-    # the tuner's grid: target workgroups {2048, 4096, 8192}, survivor multiple {2, 3, 4, 8}
-    assert t["target_wgs"] in (2048, 4096, 8192) and t["c_factor"] in (2, 3, 4, 8)
+    # the tuner's grid: target workgroups {4096, 8192, 16384}, survivor multiple
+    # {2, 3, 4, 8}, filter variant LDS wide / narrow / ring wide
+    assert t["target_wgs"] in (4096, 8192, 16384) and t["c_factor"] in (2, 3, 4, 8)
+    assert t["variant"] in (0, 1, 3)
     assert t["ms_tuned"] <= t["ms_default"] * 1.5
     I, _ = idx.search(q, 10)
     assert idx.last_stats()["kprime"] == 10 * t["kprime_factor"]

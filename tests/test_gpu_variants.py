@@ -1,0 +1,39 @@
+"""Every filter variant (LDS wide / narrow, direct, ring wide / narrow) and
+workgroup target computes the same MFMA chain and the same threshold tests,
+so a search returns bit-identical indices and distances under each of them,
+on the sample levels (sample copy, dense first pass) and the full level."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [0, 1, 2, 3, 4]
+
+
+def tuned(k, nq, variant, tw, c=2, rho=32):
+    return dict(k=k, batch_nq=nq, kprime_factor=4, target_wgs=tw, c_factor=c, variant=variant, rho=rho,
+                recall=0.0, ms_default=0.0, ms_tuned=0.0)
+
+
+@pytest.mark.parametrize("precision,n,d", [("bf16", 300_000, 96), ("fp8", 200_000, 128), ("bf16", 70_000, 64)])
+def test_variants_bit_identical(ca, orc, res, precision, n, d):
+    x = ca.gen_gaussian(n, d, 41, True)
+    q = ca.gen_gaussian(700, d, 42, True)
+    k = 10
+    idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=precision, kprime_factor=4))
+    idx.add(x)
+    ref = None
+    for v in VARIANTS:
+        for tw in (4096, 16384):
+            idx.set_tuning(tuned(k, q.shape[0], v, tw))
+            I, D = idx.search(q, k)
+            st = idx.last_stats()
+            assert st["kprime"] == 4 * k
+            if ref is None:
+                ref = (I, D)
+                I64, _ = orc.search_fp64(x, q, k)
+                print(precision, "recall", orc.recall(I, I64), "levels", st["levels"])
+                continue
+            assert np.array_equal(I, ref[0]), f"variant {v} tw {tw}"
+            assert np.array_equal(D.view(np.uint32), ref[1].view(np.uint32)), f"variant {v} tw {tw}"
+    idx.clear_tuning()
