@@ -63,10 +63,14 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (grid, survivor target, filter variant; k' stays 4k)")
-    ap.add_argument("--mode", default="headline", choices=["headline", "serving", "copy"],
+    ap.add_argument("--mode", default="headline", choices=["headline", "serving", "copy", "pmc-pass"],
                     help="headline: the metric's workload; serving: the 64-index request loop; "
                          "copy: device-to-device copies of --copy-gib GiB (HBM bandwidth and the "
-                         "reference for scaling PMC byte counters)")
+                         "reference for scaling PMC byte counters); pmc-pass: one search of the "
+                         "headline form (what the traffic passes profile)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 --pmc passes that measure the filter's HBM traffic")
+    ap.add_argument("--tuning", default="", help="pmc-pass: JSON tuning entry to install")
     ap.add_argument("--copy-gib", type=float, default=1.0)
     ap.add_argument("--rate", type=float, default=400.0, help="serving: Poisson arrival rate (req/s)")
     ap.add_argument("--requests", type=int, default=4096, help="serving: number of requests")
@@ -89,6 +93,8 @@ def main() -> int:
         return serving(args, ca)
     if args.mode == "copy":
         return copy_probe(args, ca)
+    if args.mode == "pmc-pass":
+        return pmc_pass(args, ca)
     n, nq, d, k = args.n, args.nq, args.d, args.k
     log(f"[bench] generating {n}x{d} database and {nq} queries")
     t0 = time.time()
@@ -188,15 +194,16 @@ def main() -> int:
         if oracle_mod is None:
             import oracle as oracle_mod
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cal = min(nq, threads)
-        t0 = time.perf_counter()
-        oracle_mod.search_canonical(x, q[:cal], k, threads=threads)
-        tc = time.perf_counter() - t0
-        S = int(min(nq, max(cal, cal * args.cpu_seconds / max(tc, 1e-6))))
-        S = max(threads, (S // threads) * threads)
-        t0 = time.perf_counter()
-        oracle_mod.search_canonical(x, q[:S], k, threads=threads)
-        tc = time.perf_counter() - t0
+        # Grow the sample until one run takes at least half the target time.
+        S = min(nq, threads)
+        for _ in range(4):
+            t0 = time.perf_counter()
+            oracle_mod.search_canonical(x, q[:S], k, threads=threads)
+            tc = time.perf_counter() - t0
+            if tc >= 0.5 * args.cpu_seconds or S >= nq:
+                break
+            S = int(min(nq, max(2 * S, S * args.cpu_seconds / max(tc, 1e-6))))
+            S = max(threads, (S // threads) * threads)
         cpu = {"value": S / tc, "unit": "QPS", "cores": threads, "kind": "port",
                "sample": f"first {S} of the {nq} queries against the full {n}x{d} database, k={k}, "
                          f"canonical FP32 restatement of searchL2Distance (oracle/oracle.c), "
@@ -206,6 +213,11 @@ def main() -> int:
     form = args.precision
     peak = MI355X_PEAK[form]
     ach = head["filter_tflops"]
+    variant = tunings[form]["variant"] if form in tunings else 0
+    traffic = {}
+    if not args.no_traffic:
+        traffic = pmc_traffic(args, tunings.get(form))
+        log(f"[bench] traffic: {traffic}")
     line = {
         "metric": "QPS at 10M x 96, k=100: BF16/FP8+rerank vs FP32 plain, recall@100 >= 0.999",
         "value": round(head["qps"], 1),
@@ -230,8 +242,8 @@ def main() -> int:
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": round(ach / peak, 4),
-            "traffic": None,
-            "kernel": f"filter_kernel<{form}> full-database launch",
+            "traffic": traffic.get("bytes"),
+            "kernel": f"{KERNEL_OF_VARIANT.get(variant, 'filter')}<{form}> full-database launch",
             "algorithmic_flops_per_launch": 2.0 * nq * n * d,
             "ms_per_launch": round(head["ms_filter_full"], 4),
         },
@@ -251,9 +263,103 @@ def main() -> int:
                                    ("target_wgs", "c_factor", "rho", "variant", "ms_default", "ms_tuned")}
     if "fp32" in results and form != "fp32":
         line["speedup_vs_plain_fp32"] = round(head["qps"] / results["fp32"]["qps"], 3)
+    if traffic:
+        line["traffic_detail"] = traffic
     line.update(extra)
     print(json.dumps(line), flush=True)
     return 0
+
+
+KERNEL_OF_VARIANT = {0: "filter_lds_kernel(wide)", 1: "filter_lds_kernel(narrow)",
+                     2: "filter_packed_kernel", 3: "filter_ring_kernel(wide)",
+                     4: "filter_ring_kernel(narrow)"}
+
+
+def pmc_pass(args, ca) -> int:
+    """One search of the headline workload with the given tuning: the program
+    the traffic passes run under rocprofv3 --pmc."""
+    n, nq, d, k = args.n, args.nq, args.d, args.k
+    x = ca.gen_gaussian(n, d, 4, True)
+    q = ca.gen_gaussian(nq, d, 5, True)
+    res = ca.DeviceResources(0)
+    idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=args.precision, kprime_factor=4))
+    idx.add(x)
+    if args.tuning:
+        idx.set_tuning(json.loads(args.tuning))
+    dq = ca.DeviceArray.from_host(q)
+    dI = ca.DeviceArray((nq, k), np.int32)
+    dD = ca.DeviceArray((nq, k), np.float32)
+    idx.search(dq, k, dI, dD)
+    ca.call("rplsh_device_sync")
+    print(json.dumps({"pmc_pass": "done", "stats": idx.last_stats()}), flush=True)
+    return 0
+
+
+def _pmc_values(outdir, counter):
+    """{dispatch id: (kernel name, duration ns, summed counter value)}."""
+    import csv
+    import glob
+
+    vals = {}
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] != counter:
+                    continue
+                key = r["Dispatch_Id"]
+                dur = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+                name, _, v = vals.get(key, (r["Kernel_Name"], dur, 0.0))
+                vals[key] = (name, dur, v + float(r["Counter_Value"]))
+    return vals
+
+
+def pmc_traffic(args, tuning) -> dict:
+    """HBM bytes of the full-level filter launch: FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch) in two rocprofv3 --pmc passes over one search, each
+    scaled by what the same counter reads for a device copy of known size."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return {"bytes": None, "note": "rocprofv3 not found"}
+    env = dict(os.environ, TMPDIR="/tmp")
+    me = os.path.abspath(__file__)
+    tune = json.dumps({key: tuning[key] for key in tuning}) if tuning else ""
+    progs = {
+        "filter": [sys.executable, me, "--mode", "pmc-pass", "--precision", args.precision,
+                   "--n", str(args.n), "--nq", str(args.nq), "--d", str(args.d), "--k", str(args.k)]
+                  + (["--tuning", tune] if tune else []),
+        "copy": [sys.executable, me, "--mode", "copy", "--copy-gib", "1", "--steps", "2"],
+    }
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        for what, prog in progs.items():
+            out = tempfile.mkdtemp(prefix="rplsh_pmc_", dir="/tmp")
+            cmd = ["timeout", "-s", "KILL", "180", prof, "--pmc", counter, "--output-format", "csv",
+                   "-d", out, "-o", "p", "--"] + prog
+            log(f"[bench] pmc pass {counter} {what}")
+            r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+            vals = _pmc_values(out, counter)
+            shutil.rmtree(out, ignore_errors=True)
+            if r.returncode != 0 or not vals:
+                return {"bytes": None, "note": f"rocprofv3 pass {counter}/{what} failed (rc {r.returncode})"}
+            if what == "filter":
+                # The full-level launch is the longest filter dispatch.
+                cands = [v for v in vals.values() if "filter" in v[0]]
+                got[(counter, what)] = max(cands, key=lambda v: v[1])[2] if cands else None
+            else:
+                got[(counter, what)] = max(vals.values(), key=lambda v: v[2])[2]
+    copy_bytes = float(1 << 30)
+    f_scale = copy_bytes / (got[("FETCH_SIZE", "copy")] * 1024.0)
+    w_scale = copy_bytes / (got[("WRITE_SIZE", "copy")] * 1024.0)
+    fetch = got[("FETCH_SIZE", "filter")] * 1024.0 * f_scale
+    write = got[("WRITE_SIZE", "filter")] * 1024.0 * w_scale
+    return {"bytes": round(fetch + write), "fetch_bytes": round(fetch), "write_bytes": round(write),
+            "fetch_kib_raw": got[("FETCH_SIZE", "filter")], "write_kib_raw": got[("WRITE_SIZE", "filter")],
+            "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
+            "calibration": "hipMemcpy device-to-device of 1 GiB: scale = 1 GiB / counter bytes"}
 
 
 def copy_probe(args, ca) -> int:
