@@ -11,7 +11,8 @@
 // the same alone and in any mix.
 //
 // Batches of up to graph_max_queries queries replay a hipGraph captured once
-// per (index, k, power-of-two bucket) on the slot: the pipeline up to the
+// per (index, k, power-of-two bucket) on the slot, from the shape's second
+// batch on (a shape seen once runs eagerly): the pipeline up to the
 // failed-query list is launch-free after capture, the live query count is a
 // device scalar written before each replay, and the rare failed queries are
 // resolved eagerly afterwards. Captured graphs hold their own workspaces;
@@ -109,6 +110,9 @@ struct rplsh_server {
   int64_t graph_launches = 0, graph_captures = 0;
   size_t graph_bytes = 0;  // sum over slots of cached graph workspaces
   uint64_t use_clock = 0;
+  // Batches seen per (index, k, bucket): a graph is captured on a key's second
+  // batch, so one-off shapes never pay for a capture.
+  std::unordered_map<uint64_t, int64_t> shape_seen;
 };
 
 namespace {
@@ -213,7 +217,12 @@ int run_batch(rplsh_server* s, Slot& sl, const std::vector<Request*>& batch, std
     int64_t total = 0;
     for (auto* r : batch) total += r->nq;
     const SearchParams p = bf_search_params(b, k, total);
-    const bool graph = s->cfg.use_graphs && v.n > 0 && total <= s->cfg.graph_max_queries;
+    bool graph = s->cfg.use_graphs && v.n > 0 && total <= s->cfg.graph_max_queries;
+    if (graph) {
+      const uint64_t key = ((uint64_t)batch[0]->index << 40) ^ ((uint64_t)k << 20) ^ (uint64_t)bucket_of(total);
+      std::lock_guard<std::mutex> lk(s->mu);
+      graph = ++s->shape_seen[key] >= 2;
+    }
     float* q;
     int32_t* I;
     float* D;

@@ -115,7 +115,8 @@ def test_graph_replays_match_eager(ca, indexes):
         indexes[2].search(ca.gen_uniform(3, 128, 79 + rep, -1, 1), 5)  # eager work between
     st = srv.stats()
     print("graph stats:", st)
-    assert st["graph_launches"] >= 12
+    # a shape's first batch runs eagerly, later ones replay its graph
+    assert st["graph_launches"] >= 10
     assert st["graph_captures"] <= 4  # one per (index, k, bucket) and slot
     srv.close()
 
