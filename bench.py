@@ -193,22 +193,47 @@ def main() -> int:
     if args.cpu_seconds > 0:
         if oracle_mod is None:
             import oracle as oracle_mod
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        # Grow the sample until one run takes at least half the target time.
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)) or 1)
+        # Sample sized so one run takes ~cpu_seconds / 3 (the sizing runs are the
+        # warm-up); the reported time is the median of three runs.
         S = min(nq, threads)
-        for _ in range(4):
+        for _ in range(5):
             t0 = time.perf_counter()
             oracle_mod.search_canonical(x, q[:S], k, threads=threads)
             tc = time.perf_counter() - t0
-            if tc >= 0.5 * args.cpu_seconds or S >= nq:
+            if tc >= args.cpu_seconds / 6.0 or S >= nq:
                 break
-            S = int(min(nq, max(2 * S, S * args.cpu_seconds / max(tc, 1e-6))))
+            S = int(min(nq, max(2 * S, S * args.cpu_seconds / 3.0 / max(tc, 1e-6))))
             S = max(threads, (S // threads) * threads)
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle_mod.search_canonical(x, q[:S], k, threads=threads)
+            runs.append(time.perf_counter() - t0)
+        tc = float(np.median(runs))
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as fh:
+                model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+        except OSError:
+            pass
         cpu = {"value": S / tc, "unit": "QPS", "cores": threads, "kind": "port",
+               "gflops": round(2.0 * S * n * d / tc / 1e9, 1), "cpu_model": model,
+               "affinity_cpus": len(os.sched_getaffinity(0)), "host_cpus": os.cpu_count(),
+               "runs_s": [round(t, 3) for t in runs],
                "sample": f"first {S} of the {nq} queries against the full {n}x{d} database, k={k}, "
                          f"canonical FP32 restatement of searchL2Distance (oracle/oracle.c), "
-                         f"{threads} OpenMP threads, {tc:.1f}s"}
+                         f"{threads} OpenMP threads, median of 3 runs {tc:.1f}s each"}
         log(f"[bench] CPU baseline {cpu['value']:.1f} QPS on {threads} threads ({S} queries, {tc:.1f}s)")
+
+    # Measured ceilings of this device: MFMA (random and constant operands) and
+    # a device-to-device copy.
+    measured = {"mfma_tflops_random": {}, "mfma_tflops_constant": {}}
+    for f in ("bf16", "fp8", "fp32"):
+        measured["mfma_tflops_random"][f] = round(ca.probe_mfma_peak(f, True), 1)
+        measured["mfma_tflops_constant"][f] = round(ca.probe_mfma_peak(f, False), 1)
+    measured["hbm_copy_gbs"] = round(copy_bandwidth(ca, 1 << 30, 3), 1)
+    log(f"[bench] measured peaks {measured}")
 
     form = args.precision
     peak = MI355X_PEAK[form]
@@ -245,9 +270,11 @@ def main() -> int:
             "traffic": traffic.get("bytes"),
             "kernel": f"{KERNEL_OF_VARIANT.get(variant, 'filter')}<{form}> full-database launch",
             "algorithmic_flops_per_launch": 2.0 * nq * n * d,
+            "frac_of_measured_mfma": round(ach / measured["mfma_tflops_random"][form], 4),
             "ms_per_launch": round(head["ms_filter_full"], 4),
         },
         "cpu_baseline": cpu,
+        "measured_peaks": measured,
     }
     for f, r in results.items():
         line[f"{f}_qps"] = round(r["qps"], 1)
@@ -360,6 +387,22 @@ def pmc_traffic(args, tuning) -> dict:
             "fetch_kib_raw": got[("FETCH_SIZE", "filter")], "write_kib_raw": got[("WRITE_SIZE", "filter")],
             "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
             "calibration": "hipMemcpy device-to-device of 1 GiB: scale = 1 GiB / counter bytes"}
+
+
+def copy_bandwidth(ca, nbytes: int, reps: int) -> float:
+    """Read + write GB/s of a device-to-device copy of nbytes."""
+    src = ca.DeviceArray((nbytes // 4,), np.float32)
+    dst = ca.DeviceArray((nbytes // 4,), np.float32)
+    ev0, ev1 = ca.Event(), ca.Event()
+    ca.call("rplsh_memcpy_async", dst.ptr, src.ptr, nbytes, None)
+    ca.call("rplsh_device_sync")
+    ev0.record(0)
+    for _ in range(reps):
+        ca.call("rplsh_memcpy_async", dst.ptr, src.ptr, nbytes, None)
+    ev1.record(0)
+    ev1.synchronize()
+    ms = ev0.elapsed_ms(ev1) / reps
+    return 2.0 * nbytes / (ms / 1000.0) / 1e9
 
 
 def copy_probe(args, ca) -> int:

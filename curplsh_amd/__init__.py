@@ -428,3 +428,13 @@ def gen_mixture(n: int, d: int, centers: int, sigma: float, center_seed: int, po
     call("rplsh_gen_mixture", out.ctypes.data, n, d, centers, sigma, center_seed, point_seed,
          dup_fraction)
     return out
+
+
+def probe_mfma_peak(kind: str = "bf16", random_operands: bool = True) -> float:
+    """Measured MFMA throughput of the current device (TFLOP/s) for "bf16",
+    "fp8" or "fp32": back-to-back MFMAs, no memory traffic; random operands
+    give the power-limited ceiling real data sees."""
+    v = C.c_double(0.0)
+    call("rplsh_probe_mfma_peak", {"bf16": 0, "fp8": 1, "fp32": 2}[kind], 1 if random_operands else 0,
+         C.byref(v))
+    return float(v.value)

@@ -199,6 +199,7 @@ _SIGS = {
     "rplsh_gen_uniform": [_vp, _i64, _i32, _u64, _f32, _f32],
     "rplsh_gen_gaussian": [_vp, _i64, _i32, _u64, _i32],
     "rplsh_gen_mixture": [_vp, _i64, _i32, _i32, _f32, _u64, _u64, C.c_double],
+    "rplsh_probe_mfma_peak": [_i32, _i32, C.POINTER(C.c_double)],
 }
 
 _RESTYPES = {

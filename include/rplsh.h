@@ -320,6 +320,14 @@ rplsh_status rplsh_gen_gaussian(float* out, int64_t n, int d, uint64_t seed, int
 rplsh_status rplsh_gen_mixture(float* out, int64_t n, int d, int centers, float sigma,
                                uint64_t center_seed, uint64_t point_seed, double dup_fraction);
 
+/* ---- measurement ----------------------------------------------------------
+ * No reference counterpart. Measured MFMA throughput of the current device in
+ * TFLOP/s for kind 0 = bf16 (32x32x16), 1 = fp8 e4m3 (32x32x64 scaled),
+ * 2 = fp32 (32x32x2): back-to-back MFMAs on four accumulators per wave, two
+ * waves per SIMD, no memory traffic. random_operands != 0 feeds random finite
+ * values (the realistic, power-limited ceiling), 0 a constant pattern. */
+rplsh_status rplsh_probe_mfma_peak(int kind, int random_operands, double* tflops);
+
 #ifdef __cplusplus
 }
 #endif
