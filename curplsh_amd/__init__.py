@@ -13,6 +13,7 @@ All compute runs in the HIP library; nothing here computes distances.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -438,3 +439,86 @@ def probe_mfma_peak(kind: str = "bf16", random_operands: bool = True) -> float:
     call("rplsh_probe_mfma_peak", {"bf16": 0, "fp8": 1, "fp32": 2}[kind], 1 if random_operands else 0,
          C.byref(v))
     return float(v.value)
+
+
+# ---- datasets (include/dataset.h) -------------------------------------------
+
+_XVECS_ELEM = {np.dtype(np.float32): 4, np.dtype(np.int32): 4, np.dtype(np.uint8): 1}
+
+
+def xvecs_info(path: str, dtype=np.float32) -> tuple[int, int]:
+    """(records, dimension) of a .fvecs / .ivecs / .bvecs file."""
+    num, dim = C.c_int64(0), C.c_int(0)
+    call("rplsh_xvecs_info", os.fsencode(path), _XVECS_ELEM[np.dtype(dtype)], C.byref(num), C.byref(dim))
+    return int(num.value), int(dim.value)
+
+
+def read_xvecs(path: str, dtype=np.float32, first: int = 0, count: int | None = None,
+               device: int | None = None):
+    """Records [first, first + count) as [count, dim]: a numpy array, or a
+    DeviceArray on `device` (streamed through pinned staging)."""
+    dt = np.dtype(dtype)
+    num, dim = xvecs_info(path, dt)
+    if count is None:
+        count = num - first
+    if device is None:
+        out = np.empty((count, dim), dt)
+        call("rplsh_xvecs_read", os.fsencode(path), _XVECS_ELEM[dt], first, count, out.ctypes.data, 0, 0)
+        return out
+    out = DeviceArray((count, dim), dt, device)
+    call("rplsh_xvecs_read", os.fsencode(path), _XVECS_ELEM[dt], first, count, out.ptr, 1, device)
+    return out
+
+
+def read_fvecs(path: str, **kw):
+    return read_xvecs(path, np.float32, **kw)
+
+
+def read_ivecs(path: str, **kw):
+    return read_xvecs(path, np.int32, **kw)
+
+
+def read_bvecs(path: str, **kw):
+    return read_xvecs(path, np.uint8, **kw)
+
+
+def write_xvecs(path: str, a: np.ndarray, append: bool = False) -> None:
+    a = np.ascontiguousarray(a)
+    assert a.ndim == 2 and a.dtype in _XVECS_ELEM, "2-D float32 / int32 / uint8 array"
+    call("rplsh_xvecs_write", os.fsencode(path), _XVECS_ELEM[a.dtype], a.ctypes.data, a.shape[0], a.shape[1],
+         1 if append else 0)
+
+
+def recall_at(result, truth, k: int | None = None, return_missed: bool = False):
+    """Mean |first k of result ∩ first k of truth| / k over the rows (the
+    reference's Dataset::evaluate); inputs are not modified."""
+    r = np.ascontiguousarray(result.to_host() if isinstance(result, DeviceArray) else result, np.int32)
+    t = np.ascontiguousarray(truth.to_host() if isinstance(truth, DeviceArray) else truth, np.int32)
+    k = k or min(r.shape[1], t.shape[1])
+    assert r.shape[0] == t.shape[0]
+    missed = np.empty(r.shape[0], np.int32)
+    v = C.c_double(0.0)
+    call("rplsh_recall_at", r.ctypes.data, r.shape[0], r.shape[1], t.ctypes.data, t.shape[1], k,
+         missed.ctypes.data, C.byref(v))
+    return (float(v.value), missed) if return_missed else float(v.value)
+
+
+class Dataset:
+    """Base, query and ground-truth arrays of a TEXMEX-style dataset
+    (Dataset / DatasetIrisa, dataset.h:49-190): `<dir>/<name>_base.fvecs`,
+    `_query.fvecs`, `_groundtruth.ivecs`. Arrays are numpy, or DeviceArrays
+    when `device` is given."""
+
+    def __init__(self, name: str, datadir: str, device: int | None = None):
+        self.name = name
+        p = lambda s: os.path.join(datadir, f"{name}_{s}")  # noqa: E731
+        self.base = read_fvecs(p("base.fvecs"), device=device)
+        self.query = read_fvecs(p("query.fvecs"), device=device)
+        self.groundtruth = read_ivecs(p("groundtruth.ivecs"))
+        self.dimension = self.base.shape[1]
+        self.num_base, self.num_query = self.base.shape[0], self.query.shape[0]
+        self.gt_k = self.groundtruth.shape[1]
+        assert self.groundtruth.shape[0] == self.num_query and self.query.shape[1] == self.dimension
+
+    def evaluate(self, indices, k: int | None = None, return_missed: bool = False):
+        return recall_at(indices, self.groundtruth, k or self.gt_k, return_missed)

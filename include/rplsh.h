@@ -320,6 +320,27 @@ rplsh_status rplsh_gen_gaussian(float* out, int64_t n, int d, uint64_t seed, int
 rplsh_status rplsh_gen_mixture(float* out, int64_t n, int d, int centers, float sigma,
                                uint64_t center_seed, uint64_t point_seed, double dup_fraction);
 
+/* ---- datasets (include/dataset.h:14-204) -----------------------------------
+ * TEXMEX vector files: records [int32 dim][dim elements], elem_bytes 4 for
+ * .fvecs (float) / .ivecs (int32), 1 for .bvecs (uint8).
+ * info: record count and dimension (0, 0 for an empty file); fails unless the
+ * file is a whole number of records. read: records [first, first+count) packed
+ * as [count][dim] into host memory (on_device = 0) or device memory of
+ * `device` (streamed through pinned staging), checking every record's header
+ * (loadXvecsData, dataset.h:14-44). write: the inverse (saveXvecsData, a TODO
+ * in the reference, dataset.h:46); append != 0 appends. */
+rplsh_status rplsh_xvecs_info(const char* path, int elem_bytes, int64_t* num, int* dim);
+rplsh_status rplsh_xvecs_read(const char* path, int elem_bytes, int64_t first, int64_t count,
+                              void* out, int on_device, int device);
+rplsh_status rplsh_xvecs_write(const char* path, int elem_bytes, const void* data, int64_t num,
+                               int dim, int append);
+/* Dataset::evaluate (dataset.h:135-152): mean over nq queries of
+ * |first k of result row ∩ first k of truth row| / k; host pointers, inputs
+ * left untouched; -1 entries of result never count; missed[q] = k - hits
+ * (optional, the reference's `diff`). */
+rplsh_status rplsh_recall_at(const int32_t* result, int64_t nq, int ld_result, const int32_t* truth,
+                             int ld_truth, int k, int32_t* missed, double* recall);
+
 /* ---- measurement ----------------------------------------------------------
  * No reference counterpart. Measured MFMA throughput of the current device in
  * TFLOP/s for kind 0 = bf16 (32x32x16), 1 = fp8 e4m3 (32x32x64 scaled),

@@ -9,12 +9,19 @@
 //  3. IndexBF with host pointers vs the canonical CPU oracle (bit-exact I, D).
 //  4. IndexBF with device pointers gives the same results; repeated add
 //     equals one add; reset empties the index.
+//  5. DatasetSIFT over files written with helper::saveXvecsData (base,
+//     query, FP64 ground truth): loaded into device memory, searched from
+//     device pointers, evaluate() >= 0.999 (FP32 vs FP64 truth) and the
+//     arrays left as they were.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
 
+#include <unistd.h>
+
+#include "curplsh/dataset.h"
 #include "curplsh/index_bf.h"
 #include "curplsh/internal/norm.h"
 #include "curplsh/internal/search.h"
@@ -118,6 +125,52 @@ static int test_index(DeviceResources& res) {
   return 0;
 }
 
+static int test_dataset(DeviceResources& res) {
+  const int dim = 128, n = 30000, nq = 100, k = 100;
+  std::vector<float> x((size_t)n * dim), q((size_t)nq * dim);
+  CHECK(rplsh_gen_uniform(x.data(), n, dim, 21, 0.0f, 1.0f) == RPLSH_OK);
+  CHECK(rplsh_gen_uniform(q.data(), nq, dim, 22, 0.0f, 1.0f) == RPLSH_OK);
+  std::vector<int32_t> gt((size_t)nq * k);
+  std::vector<double> gd((size_t)nq * k);
+  oracle_search_fp64(x.data(), n, dim, q.data(), nq, k, gt.data(), gd.data(), 0);
+  char dir[] = "/tmp/rplsh_dsXXXXXX";
+  CHECK(mkdtemp(dir) != nullptr);
+  const std::string d(dir);
+  helper::saveXvecsData((d + "/sift_base.fvecs").c_str(), x.data(), n, dim);
+  helper::saveXvecsData((d + "/sift_query.fvecs").c_str(), q.data(), nq, dim);
+  helper::saveXvecsData((d + "/sift_groundtruth.ivecs").c_str(), gt.data(), nq, k);
+  int rc = 0;
+  {
+    DatasetSIFT ds(dir, MemorySpace::Device);
+    CHECK(ds.getNumBase() == n && ds.getNumQuery() == nq && ds.getDimension() == dim &&
+          ds.getGroundTruthK() == k);
+    CHECK(getDeviceForAddress(ds.getBase()) == 0);
+    IndexBF index(&res, dim);
+    index.add(n, ds.getBase());
+    DeviceTensor<int, 2> di({nq, k});
+    DeviceTensor<float, 2> dd({nq, k});
+    index.search(nq, ds.getQuery(), k, di.data(), dd.data());
+    std::vector<int> before((size_t)nq * k), after((size_t)nq * k), diff(nq);
+    Stream s = res.getDefaultStream(0);
+    di.copyTo(before.data(), s);
+    streamSynchronize(s);
+    const float recall = ds.evaluate(di.data(), diff.data());
+    di.copyTo(after.data(), s);
+    streamSynchronize(s);
+    // FP32 vs FP64 ground truth: a boundary near-tie may differ, nothing more.
+    int missed = 0;
+    for (int i = 0; i < nq; ++i) missed += diff[i];
+    if (!(recall >= 0.999f && before == after && missed == (int)std::lround((1.0 - recall) * nq * k))) rc = 1;
+    std::printf("DatasetSIFT recall@%d = %.4f\n", k, recall);
+  }
+  for (const char* f : {"/sift_base.fvecs", "/sift_query.fvecs", "/sift_groundtruth.ivecs"})
+    std::remove((d + f).c_str());
+  rmdir(dir);
+  CHECK(rc == 0);
+  std::printf("Dataset ok\n");
+  return 0;
+}
+
 int main() {
   if (getNumDevices() < 1) {
     std::fprintf(stderr, "no GPU\n");
@@ -128,6 +181,7 @@ int main() {
   if (test_norm(res)) return 1;
   if (test_search_fixture(res)) return 1;
   if (test_index(res)) return 1;
+  if (test_dataset(res)) return 1;
   std::printf("all C++ tests passed\n");
   return 0;
 }
