@@ -19,7 +19,8 @@
 // search.cu:21-42 are not needed: candidates carry global indices).
 // Plain path: certifies that no non-candidate can enter the top k. Precision
 // forms: re-orders the boundary band around the k-th distance by FP64
-// distance.
+// distance, reports the band members' FP64 distances (rounded to FP32) and
+// returns the k results ascending by reported distance, then index.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -421,6 +422,8 @@ __global__ __launch_bounds__(kSelBlock) void rerank_kernel(RerankArgs a) {
   }
 
   // Precision forms: FP64 order inside the band around the k-th distance.
+  bool banded = false;
+  int band0 = 0, band1 = -1;
   if (a.fp64_boundary && valid > k && k > 0) {
     if (threadIdx.x == 0) {
       const float relk = cand_key(s[k - 1]);
@@ -464,7 +467,41 @@ __global__ __launch_bounds__(kSelBlock) void rerank_kernel(RerankArgs a) {
       __syncthreads();
       for (int i = threadIdx.x; i < nb; i += blockDim.x) s[b0 + i] = bw[i];
       __syncthreads();
+      banded = true;
+      band0 = b0;
+      band1 = b1;
     }
+  }
+
+  if (banded) {
+    // The band decided membership by FP64 distance; report its members' FP64
+    // distances (rounded to FP32) and order the k results by the reported
+    // distance, then index, so the list stays ascending.
+    const int Pk = pow2_at_least(k);
+    for (int j = threadIdx.x; j < Pk; j += blockDim.x) {
+      uint64_t w = kEmptyCand;
+      if (j < k && j < valid) {
+        const int32_t idx = cand_idx(s[j]);
+        float dist;
+        if (j >= band0 && j <= band1) {
+          dist = a.exact ? (float)bd[j - band0] : (float)(bd[j - band0] - (double)qn);
+        } else {
+          const float rel = cand_key(s[j]);
+          dist = a.exact ? rel + qn : rel;
+        }
+        w = pack_cand(dist, idx);
+      }
+      s[j] = w;
+    }
+    __syncthreads();
+    bitonic_sort_u64(s, Pk);
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+      const int64_t o = (int64_t)q * k + j;
+      const uint64_t w = s[j];
+      a.out_idx[o] = w == kEmptyCand ? -1 : cand_idx(w);
+      a.out_dist[o] = w == kEmptyCand ? FLT_MAX : cand_key(w);
+    }
+    return;
   }
 
   for (int j = threadIdx.x; j < k; j += blockDim.x) {
