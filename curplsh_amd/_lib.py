@@ -67,6 +67,8 @@ class SearchStats(C.Structure):
         ("ms_select", C.c_float),
         ("ms_rerank", C.c_float),
         ("rows_filter_full", C.c_int64),
+        ("filter_variant", C.c_int32),
+        ("target_wgs", C.c_int32),
     ]
 
 

@@ -272,36 +272,40 @@ __global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) 
 // branch, and it is not reused before the barrier that ends stage s+1.
 constexpr int kQueue = 512;  // entries per survivor queue (power of two)
 
-template <Elem E, int DPAD, bool WIDE>
+template <Elem E, int DPAD, bool WIDE, int NW>
 struct StageShape {
   using G = Geometry<E, DPAD>;
-  // Stage tiles: the double-buffered stage within ~48 KB (wide, two
-  // workgroups per CU) or ~36 KB (narrow, three).
-  static constexpr int kFit = (WIDE ? 48 * 1024 : 36 * 1024) / (2 * G::TILE);
-  static constexpr int SM = kFit > 4 ? 4 : (kFit < 1 ? 1 : kFit);
+  // Stage tiles: the double-buffered stage within ~48 KB (wide, two 4-wave
+  // workgroups per CU), ~36 KB (narrow, three) or ~64 KB (one 8-wave
+  // workgroup per CU), at most 4 tiles.
+  static constexpr int kBudget = NW == 8 ? 64 * 1024 : (WIDE ? 48 * 1024 : 36 * 1024);
+  static constexpr int kMaxSM = 4;  // more tiles per stage need more staging registers
+  static constexpr int kFit = kBudget / (2 * G::TILE);
+  static constexpr int SM = kFit > kMaxSM ? kMaxSM : (kFit < 1 ? 1 : kFit);
   static constexpr int SR = 32 * SM;  // rows per stage
   static constexpr int STAGE = SM * G::TILE;
   static constexpr int LDS = 2 * STAGE + 2 * SR * 8 + 2 * kQueue * 12;
-  static constexpr int WAVES_PER_SIMD = (!WIDE && 3 * LDS <= 160 * 1024) ? 3 : 2;
+  static constexpr int WAVES_PER_SIMD = (NW == 4 && !WIDE && 3 * LDS <= 160 * 1024) ? 3 : 2;
 };
 
-template <Elem E, int DPAD, int TN, bool WIDE>
-__global__ __launch_bounds__(kBlock, (StageShape<E, DPAD, WIDE>::WAVES_PER_SIMD)) void filter_lds_kernel(
+template <Elem E, int DPAD, int TN, bool WIDE, int NW = 4>
+__global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_SIMD)) void filter_lds_kernel(
     FilterArgs a) {
   using G = Geometry<E, DPAD>;
-  using SS = StageShape<E, DPAD, WIDE>;
+  using SS = StageShape<E, DPAD, WIDE, NW>;
+  constexpr int NT = NW * 64;  // threads per workgroup
   constexpr int LB = G::LB;
   constexpr int NC = G::NC;
-  constexpr int BQ = 4 * TN * 32;
+  constexpr int BQ = NW * TN * 32;
   constexpr int SM = SS::SM;
   constexpr int SR = SS::SR;
   constexpr int STAGE = SS::STAGE;
   constexpr int UNITS = STAGE / 16;
-  constexpr int UPT = (UNITS + kBlock - 1) / kBlock;
+  constexpr int UPT = (UNITS + NT - 1) / NT;
   constexpr int TILE_UNITS = G::TILE / 16;
   constexpr int CHUNK_UNITS = G::CHUNK / 16;
   constexpr int LANE_UNITS = LB / 16;
-  static_assert(SR <= kBlock, "one row norm per thread");
+  static_assert(SR <= NT, "one row norm per thread");
 
   __shared__ __attribute__((aligned(16))) char sx[2][STAGE];
   __shared__ __attribute__((aligned(16))) float sn[2][SR];  // -||x||^2/2, -inf past the end
@@ -362,9 +366,9 @@ __global__ __launch_bounds__(kBlock, (StageShape<E, DPAD, WIDE>::WAVES_PER_SIMD)
       const char* src = xb + (j0 >> 5) * G::TILE;
 #pragma unroll
       for (int t = 0; t < UPT; ++t) {
-        const int u = tid + kBlock * t;
+        const int u = tid + NT * t;
         i32x4 v = {0, 0, 0, 0};
-        if (UNITS % kBlock == 0 || u < UNITS) v = *reinterpret_cast<const i32x4*>(src + u * 16);
+        if (UNITS % NT == 0 || u < UNITS) v = *reinterpret_cast<const i32x4*>(src + u * 16);
         pre[t] = v;
       }
       if (tid < SR) {
@@ -375,9 +379,9 @@ __global__ __launch_bounds__(kBlock, (StageShape<E, DPAD, WIDE>::WAVES_PER_SIMD)
     }
 #pragma unroll
     for (int t = 0; t < UPT; ++t) {
-      const int u = tid + kBlock * t;
+      const int u = tid + NT * t;
       i32x4 v = {0, 0, 0, 0};
-      if (UNITS % kBlock == 0 || u < UNITS) {
+      if (UNITS % NT == 0 || u < UNITS) {
         const int mt = u / TILE_UNITS;
         const int w = u - mt * TILE_UNITS;
         const int c = w / CHUNK_UNITS;
@@ -406,8 +410,8 @@ __global__ __launch_bounds__(kBlock, (StageShape<E, DPAD, WIDE>::WAVES_PER_SIMD)
   auto sstore = [&](int buf) {
 #pragma unroll
     for (int t = 0; t < UPT; ++t) {
-      const int u = tid + kBlock * t;
-      if (UNITS % kBlock == 0 || u < UNITS) *reinterpret_cast<i32x4*>(&sx[buf][u * 16]) = pre[t];
+      const int u = tid + NT * t;
+      if (UNITS % NT == 0 || u < UNITS) *reinterpret_cast<i32x4*>(&sx[buf][u * 16]) = pre[t];
     }
     if (tid < SR) {
       sn[buf][tid] = pn;
@@ -431,7 +435,7 @@ __global__ __launch_bounds__(kBlock, (StageShape<E, DPAD, WIDE>::WAVES_PER_SIMD)
   // (at most kQueue since the last flush) to the per-query lists.
   auto flush = [&](int par, unsigned tail) {
     const unsigned n = min(tail - head[par], (unsigned)kQueue);
-    for (unsigned i = tid; i < n; i += kBlock) {
+    for (unsigned i = tid; i < n; i += NT) {
       const unsigned t = (head[par] + i) & (kQueue - 1);
       const int qr = qquery[par][t];
       const int pos = atomicAdd(a.counts + qr, 1);
@@ -920,6 +924,9 @@ void launch_packed_one(int variant, const FilterArgs& a, hipStream_t st) {
     case kFilterLdsNarrow:
       filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, false>(), false><<<grid, kBlock, 0, st>>>(a);
       break;
+    case kFilterLdsWide8:
+      filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, true>(), true, 8><<<grid, 512, 0, st>>>(a);
+      break;
     case kFilterRingWide:
       RPLSH_REQUIRE(a.xnh, "ring filter: C-operand array missing");
       filter_ring_kernel<E, DPAD, lds_tn<E, DPAD, true>(), true><<<grid, kBlock, 0, st>>>(a);
@@ -1090,7 +1097,7 @@ struct EncodePacked {
 }  // namespace
 
 int filter_packed_block_queries(Elem e, int dpad, int variant) {
-  return 4 * 32 * dispatch(e, dpad, TnOf{variant});
+  return (variant == kFilterLdsWide8 ? 8 : 4) * 32 * dispatch(e, dpad, TnOf{variant});
 }
 
 void launch_filter_packed(Elem e, int dpad, int variant, const FilterArgs& a, hipStream_t st) {

@@ -44,6 +44,7 @@ enum FilterVariant : int {
   kFilterDirect = 2,
   kFilterRingWide = 3,
   kFilterRingNarrow = 4,
+  kFilterLdsWide8 = 5,  // LDS wide with 8-wave workgroups (one per CU)
 };
 int filter_packed_block_queries(Elem e, int dpad, int variant);
 void launch_filter_packed(Elem e, int dpad, int variant, const FilterArgs& a, hipStream_t st);

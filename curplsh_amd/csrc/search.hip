@@ -294,6 +294,8 @@ void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& 
     const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
     stats.levels = (int32_t)L.size();
     stats.rows_filter_full = L.back().m;
+    stats.filter_variant = p.variant;
+    stats.target_wgs = (int32_t)p.target_wgs;
     if (pe && pe->on) {
       stats.ms_filter_samples = pe->ms(0, 1);
       stats.ms_filter_full = pe->ms(1, 2);
@@ -532,7 +534,7 @@ SearchParams bf_search_params(const rplsh_bf* b, int k, int64_t nq) {
     if (!plain && best->kprime_factor > 0) p.kp = k * best->kprime_factor;
     if (best->target_wgs > 0) p.target_wgs = best->target_wgs;
     if (best->c_factor > 0) p.c_factor = best->c_factor;
-    if (best->variant >= 0 && best->variant <= 2) p.variant = best->variant;
+    if (best->variant >= kFilterLdsWide && best->variant <= kFilterLdsWide8) p.variant = best->variant;
     if (best->rho > 0) p.rho = best->rho;
   }
   return p;
@@ -1020,7 +1022,7 @@ rplsh_status rplsh_bf_set_tuning(rplsh_bf* b, const rplsh_tune_result* t) {
     RPLSH_REQUIRE(b && t, "NULL argument");
     RPLSH_REQUIRE(t->k > 0 && t->batch_nq > 0, "bad tuning entry");
     RPLSH_REQUIRE(t->kprime_factor >= 0 && (int64_t)t->kprime_factor * t->k <= 4096, "k' too large");
-    RPLSH_REQUIRE(t->variant >= kFilterLdsWide && t->variant <= kFilterRingNarrow, "unknown filter variant %d", t->variant);
+    RPLSH_REQUIRE(t->variant >= kFilterLdsWide && t->variant <= kFilterLdsWide8, "unknown filter variant %d", t->variant);
     RPLSH_REQUIRE(t->rho >= 0 && t->rho <= 256, "bad rho %d", t->rho);
     b->tunes.erase(std::remove_if(b->tunes.begin(), b->tunes.end(),
                                   [&](const rplsh_tune_result& x) {

@@ -198,6 +198,8 @@ typedef struct rplsh_search_stats {
   float ms_select;          /* device time of the last-level top-k' select */
   float ms_rerank;          /* device time of the re-rank */
   int64_t rows_filter_full; /* database rows of the full filter launch */
+  int32_t filter_variant;   /* packed copies: filter variant used (rplsh_tune_result) */
+  int32_t target_wgs;       /* workgroups the filter grid aimed for */
 } rplsh_search_stats;
 rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out);
 
@@ -209,13 +211,14 @@ rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out)
  *    whose recall@k against the plain path's exact result on the sample
  *    reaches target_recall;
  *  - target_wgs: workgroups the filter grid aims for (query tiles x database
- *    slices), from {1024, 2048, 4096, 8192}, by measured batch time;
+ *    slices), from {4096, 8192, 16384}, by measured batch time;
  *  - (c_factor, rho): the last level keeps ~c_factor * k' rows per query (its
  *    candidate queue; capacity 4x that); its threshold is the rho-th smallest
  *    key of a sample with stride c_factor * k' / rho. Pairs from
  *    {(3,auto),(4,16),(2,32),(3,32),(2,64),(8,8)}, by time;
- *  - variant (BF16/FP8): the filter's tile shape -- LDS-staged with wide or
- *    narrow query tiles per wave, or direct register loads -- by time.
+ *  - variant (BF16/FP8): the filter's data movement and tile shape, by time,
+ *    among LDS wide (0), LDS narrow (1) and ring wide (3); direct (2), ring
+ *    narrow (4) and LDS wide with 8-wave workgroups (5) can be installed.
  * target_recall <= 0 keeps the configured k' factor and tunes the rest.
  * The result is installed on the index and used by later searches with the
  * same k (the entry with the nearest batch size). */
@@ -225,7 +228,8 @@ typedef struct rplsh_tune_result {
   int kprime_factor;  /* 0 for the plain path */
   int target_wgs;
   int c_factor;
-  int variant;        /* 0 LDS wide, 1 LDS narrow, 2 direct (BF16/FP8 only) */
+  int variant;        /* 0 LDS wide, 1 LDS narrow, 2 direct, 3 ring wide, 4 ring narrow,
+                         5 LDS wide with 8-wave workgroups (BF16/FP8 only) */
   int rho;            /* rank of the last threshold within its sample (0: automatic) */
   float recall;       /* recall@k of the chosen k' on the sample (1 for plain) */
   float ms_default;   /* batch time with the built-in defaults */

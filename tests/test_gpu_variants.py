@@ -1,4 +1,5 @@
-"""Every filter variant (LDS wide / narrow, direct, ring wide / narrow) and
+"""Every filter variant (LDS wide / narrow / wide with 8-wave workgroups,
+direct, ring wide / narrow) and
 workgroup target computes the same MFMA chain and the same threshold tests,
 so a search returns bit-identical indices and distances under each of them,
 on the sample levels (sample copy, dense first pass) and the full level."""
@@ -7,7 +8,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [0, 1, 2, 3, 4]
+VARIANTS = [0, 1, 2, 3, 4, 5]
 
 
 def tuned(k, nq, variant, tw, c=2, rho=32):
@@ -28,6 +29,7 @@ def test_variants_bit_identical(ca, orc, res, precision, n, d):
             idx.set_tuning(tuned(k, q.shape[0], v, tw))
             I, D = idx.search(q, k)
             st = idx.last_stats()
+            assert st["filter_variant"] == v
             assert st["kprime"] == 4 * k
             if ref is None:
                 ref = (I, D)
