@@ -208,6 +208,24 @@ class IndexBF:
         call("rplsh_bf_create", resources.handle, int(dim), C.byref(cfg), C.byref(self._h))
         self.dim = int(dim)
 
+    def save(self, path: str) -> None:
+        """Configuration, installed tuning entries and the FP32 rows."""
+        call("rplsh_bf_save", self._h, os.fsencode(path))
+
+    @classmethod
+    def load(cls, resources: DeviceResources, path: str, device: int = -1) -> "IndexBF":
+        """A new index from a file written by save(); device < 0: the saved one."""
+        obj = cls.__new__(cls)
+        obj.resources = resources
+        obj._h = C.c_void_p()
+        call("rplsh_bf_load", resources.handle, os.fsencode(path), int(device), C.byref(obj._h))
+        d, dev = C.c_int(0), C.c_int(0)
+        call("rplsh_bf_dimension", obj._h, C.byref(d))
+        call("rplsh_bf_device", obj._h, C.byref(dev))
+        obj.dim = int(d.value)
+        obj.config = IndexBFConfig(device=int(dev.value))
+        return obj
+
     # Index::train / add / search / reset (include/index.h:29-55)
     def train(self, x) -> None:
         n = 0 if x is None else int(x.shape[0])

@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -1087,3 +1088,89 @@ rplsh_status rplsh_bf_clear_tuning(rplsh_bf* b) {
 }
 
 }  // extern "C"
+
+// ---- index files -------------------------------------------------------------
+// Layout (little endian): "RPLSHBF1", u32 version (1), i32 dim, i64 n,
+// rplsh_bf_config, i32 tuning entries, entries (rplsh_tune_result), then the
+// n FP32 rows unpadded [n][dim]. Norms and candidate copies are rebuilt on
+// load (they are functions of the rows).
+namespace {
+constexpr char kIndexMagic[8] = {'R', 'P', 'L', 'S', 'H', 'B', 'F', '1'};
+constexpr uint32_t kIndexVersion = 1;
+}  // namespace
+
+rplsh_status rplsh_bf_save(rplsh_bf* b, const char* path) {
+  return guard([&] {
+    RPLSH_REQUIRE(b && path, "NULL argument");
+    DeviceScope scope(b->cfg.device);
+    hipStream_t st = bf_stream(b);
+    RPLSH_HIP(hipStreamSynchronize(st));
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) fail(RPLSH_ERR_INVALID_ARG, "%s: cannot open for writing", path);
+    bool ok = true;
+    auto put = [&](const void* p, size_t bytes) { ok = ok && std::fwrite(p, 1, bytes, fp) == bytes; };
+    const int32_t dim = b->dim, ntune = (int32_t)b->tunes.size();
+    const int64_t n = b->n;
+    put(kIndexMagic, 8);
+    put(&kIndexVersion, 4);
+    put(&dim, 4);
+    put(&n, 8);
+    put(&b->cfg, sizeof(b->cfg));
+    put(&ntune, 4);
+    for (const auto& t : b->tunes) put(&t, sizeof(t));
+    // Rows, through a host buffer of at most ~64 MB.
+    const int64_t per = std::max<int64_t>(1, (64ll << 20) / ((int64_t)dim * 4));
+    std::vector<float> buf((size_t)std::min<int64_t>(per, std::max<int64_t>(n, 1)) * dim);
+    try {
+      for (int64_t r = 0; r < n && ok; r += per) {
+        const int64_t c = std::min(per, n - r);
+        RPLSH_HIP(hipMemcpy2D(buf.data(), (size_t)dim * 4, b->x + r * b->ldx, (size_t)b->ldx * 4,
+                              (size_t)dim * 4, (size_t)c, hipMemcpyDeviceToHost));
+        put(buf.data(), (size_t)c * dim * 4);
+      }
+    } catch (...) {
+      std::fclose(fp);
+      throw;
+    }
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok) fail(RPLSH_ERR_INVALID_ARG, "%s: write failed", path);
+  });
+}
+
+rplsh_status rplsh_bf_load(rplsh_resources* res, const char* path, int device, rplsh_bf** out) {
+  return guard([&] {
+    RPLSH_REQUIRE(res && path && out, "NULL argument");
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) fail(RPLSH_ERR_INVALID_ARG, "%s: cannot open", path);
+    std::vector<char> head(8);
+    uint32_t version = 0;
+    int32_t dim = 0, ntune = 0;
+    int64_t n = 0;
+    rplsh_bf_config cfg{};
+    bool ok = std::fread(head.data(), 1, 8, fp) == 8 && std::memcmp(head.data(), kIndexMagic, 8) == 0 &&
+              std::fread(&version, 4, 1, fp) == 1 && version == kIndexVersion &&
+              std::fread(&dim, 4, 1, fp) == 1 && std::fread(&n, 8, 1, fp) == 1 &&
+              std::fread(&cfg, sizeof(cfg), 1, fp) == 1 && std::fread(&ntune, 4, 1, fp) == 1 && dim > 0 &&
+              n >= 0 && ntune >= 0 && ntune < (1 << 16);
+    std::vector<rplsh_tune_result> tunes((size_t)std::max(ntune, 0));
+    if (ok && ntune > 0) ok = std::fread(tunes.data(), sizeof(rplsh_tune_result), (size_t)ntune, fp) == (size_t)ntune;
+    std::vector<float> rows;
+    if (ok && n > 0) {
+      rows.resize((size_t)n * dim);
+      ok = std::fread(rows.data(), 4, rows.size(), fp) == rows.size();
+    }
+    std::fclose(fp);
+    if (!ok) fail(RPLSH_ERR_INVALID_ARG, "%s: not an index file of this library (version %u) or truncated", path,
+                  kIndexVersion);
+    if (device >= 0) cfg.device = device;
+    rplsh_bf* b = nullptr;
+    if (rplsh_bf_create(res, dim, &cfg, &b) != RPLSH_OK) fail(RPLSH_ERR_STATE, "%s", rplsh_last_error());
+    if (n > 0 && rplsh_bf_add(b, n, rows.data()) != RPLSH_OK) {
+      const std::string msg = rplsh_last_error();
+      rplsh_bf_destroy(b);
+      fail(RPLSH_ERR_STATE, "%s", msg.c_str());
+    }
+    b->tunes = tunes;
+    *out = b;
+  });
+}

@@ -98,6 +98,25 @@ IndexBF::~IndexBF() {
   if (handle_) rplsh_bf_destroy(handle_);
 }
 
+void IndexBF::save(const char* path) { checkStatus(rplsh_bf_save(handle_, path), "IndexBF::save"); }
+
+void IndexBF::load(const char* path) {
+  rplsh_bf* fresh = nullptr;
+  checkStatus(rplsh_bf_load(resources_->handle(), path, device_, &fresh), "IndexBF::load");
+  int d = 0;
+  int64_t n = 0;
+  rplsh_bf_dimension(fresh, &d);
+  if (d != dimension_) {
+    rplsh_bf_destroy(fresh);
+    host_assert(d == dimension_);
+  }
+  checkStatus(rplsh_bf_set_precision(fresh, (int)config_.precision, config_.kprimeFactor), "IndexBF::load");
+  rplsh_bf_destroy(handle_);
+  handle_ = fresh;
+  checkStatus(rplsh_bf_num_data(handle_, &n), "IndexBF::load");
+  numData_ = (int)n;
+}
+
 // Nothing to learn (the reference's intent, index_bf.cpp:13-15).
 void IndexBF::train(int, const float*) {}
 

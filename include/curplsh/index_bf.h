@@ -63,6 +63,14 @@ class IndexBF : public Index {
   /// Switches the candidate precision (re-encodes the resident vectors).
   void setPrecision(Precision p, int kprimeFactor = 0);
 
+  /// Writes the rows and installed tuning entries to a file (no reference
+  /// counterpart; the reference has no persistence).
+  void save(const char* path);
+
+  /// Replaces the rows and tuning entries by those of a file written by
+  /// save() (same dimension); this object's device and precision apply.
+  void load(const char* path);
+
   /// Counters of the last search.
   rplsh_search_stats lastStats() const;
 

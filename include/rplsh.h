@@ -173,6 +173,12 @@ rplsh_status rplsh_bf_search(rplsh_bf* index, int64_t nq, const float* queries, 
                              int32_t* indices, float* distances);
 /* IndexBF::reset (index_bf.cpp:17-24). */
 rplsh_status rplsh_bf_reset(rplsh_bf* index);
+/* Index files (no reference counterpart: the reference lists save/load as
+ * missing). save: configuration, installed tuning entries and the FP32 rows;
+ * load: a new index on `device` (< 0: the saved one) with the same rows,
+ * configuration and tuning; norms and candidate copies are rebuilt. */
+rplsh_status rplsh_bf_save(rplsh_bf* index, const char* path);
+rplsh_status rplsh_bf_load(rplsh_resources* res, const char* path, int device, rplsh_bf** out);
 rplsh_status rplsh_bf_num_data(const rplsh_bf* index, int64_t* n);
 rplsh_status rplsh_bf_dimension(const rplsh_bf* index, int* d);
 rplsh_status rplsh_bf_device(const rplsh_bf* index, int* device);

@@ -8,7 +8,8 @@
 //     searchL2Distance with relative distances: top-10 [1,0,2,...,9].
 //  3. IndexBF with host pointers vs the canonical CPU oracle (bit-exact I, D).
 //  4. IndexBF with device pointers gives the same results; repeated add
-//     equals one add; reset empties the index.
+//     equals one add; save + load into a fresh object gives the same
+//     results; reset empties the index.
 //  5. DatasetSIFT over files written with helper::saveXvecsData (base,
 //     query, FP64 ground truth): loaded into device memory, searched from
 //     device pointers, evaluate() >= 0.999 (FP32 vs FP64 truth) and the
@@ -116,6 +117,22 @@ static int test_index(DeviceResources& res) {
   streamSynchronize(s);
   CHECK(I2 == I);
   CHECK(std::memcmp(D2.data(), D.data(), D.size() * 4) == 0);
+
+  // Index file round trip into a fresh object.
+  char path[] = "/tmp/rplsh_indexXXXXXX";
+  const int fd = mkstemp(path);
+  CHECK(fd >= 0);
+  close(fd);
+  index.save(path);
+  IndexBF other(&res, dim);
+  other.load(path);
+  std::remove(path);
+  CHECK(other.getNumData() == n);
+  std::vector<int> I3((size_t)nq * k);
+  std::vector<float> D3((size_t)nq * k);
+  other.search(nq, q.data(), k, I3.data(), D3.data());
+  CHECK(I3 == I);
+  CHECK(std::memcmp(D3.data(), D.data(), D.size() * 4) == 0);
 
   index.reset();
   CHECK(index.getNumData() == 0);

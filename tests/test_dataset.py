@@ -115,3 +115,15 @@ def test_dataset_irisa_layout(ca, tmp_path):
     assert (ds.num_base, ds.num_query, ds.dimension, ds.gt_k) == (2000, 20, 32, 10)
     assert np.array_equal(ds.base, base) and np.array_equal(ds.query, query)
     assert ds.evaluate(gt) == 1.0
+
+
+def test_index_file_rejects_foreign_files(ca, tmp_path):
+    """rplsh_bf_load checks the header before touching a device."""
+    p = str(tmp_path / "not_an_index.bin")
+    ca.write_xvecs(p, np.ones((3, 4), np.float32))
+    with pytest.raises(ca.RplshError, match="not an index file"):
+        ca.IndexBF.load(_NoResources(), p)
+
+
+class _NoResources:
+    handle = 1  # never dereferenced: the header check fails first
