@@ -14,7 +14,8 @@
 // rerank_kernel: one workgroup per query. Recomputes every candidate's
 // relative distance in the canonical FP32 order
 //     dot = fmaf chain over j,  rel = fmaf(-2, dot, ||x||^2)
-// sorts by (rel, index), keeps k, and writes  D = rel + ||q||^2
+// (rows gathered cooperatively through LDS, score_rows_staged), sorts by
+// (rel, index), keeps k, and writes  D = rel + ||q||^2
 // (broadcast.cu:15-38 folded into the write; index offsets of
 // search.cu:21-42 are not needed: candidates carry global indices).
 // Plain path: certifies that no non-candidate can enter the top k. Precision
@@ -57,7 +58,7 @@ __device__ inline void bitonic_sort_u64(uint64_t* s, int P) {
   }
 }
 
-__device__ inline int pow2_at_least(int v) {
+__host__ __device__ inline int pow2_at_least(int v) {
   int p = 1;
   while (p < v) p <<= 1;
   return p;
@@ -330,66 +331,237 @@ __global__ __launch_bounds__(64 * kWaveSelQ) void select_wave_kernel(SelectArgs 
 }
 
 // Canonical FP32 relative distance of query sq (in LDS) to row idx.
+// Loads of one row are issued kRowBatch float4 at a time before the
+// arithmetic consumes them; the order over j is unchanged.
+constexpr int kRowBatch = 8;
+
+// Canonical relative distance of one row computed by one lane: 16-byte loads
+// kRowBatch at a time when rows are 16-byte aligned, else scalar loads.
 __device__ inline float canonical_rel(const float* __restrict__ x, int64_t ldx,
                                       const float* __restrict__ xnorm, int d, const float* sq,
                                       int32_t idx, bool vec4) {
   const float* xr = x + (int64_t)idx * ldx;
+  const float xn = xnorm[idx];
   float s = 0.0f;
   if (vec4) {
-    for (int j = 0; j < d; j += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(xr + j);
-      s = fmaf(sq[j + 0], v.x, s);
-      s = fmaf(sq[j + 1], v.y, s);
-      s = fmaf(sq[j + 2], v.z, s);
-      s = fmaf(sq[j + 3], v.w, s);
+    for (int j0 = 0; j0 < d; j0 += 4 * kRowBatch) {
+      float4 v[kRowBatch];
+#pragma unroll
+      for (int t = 0; t < kRowBatch; ++t)
+        if (j0 + 4 * t < d) v[t] = *reinterpret_cast<const float4*>(xr + j0 + 4 * t);
+#pragma unroll
+      for (int t = 0; t < kRowBatch; ++t) {
+        if (j0 + 4 * t < d) {
+          const int j = j0 + 4 * t;
+          s = fmaf(sq[j + 0], v[t].x, s);
+          s = fmaf(sq[j + 1], v[t].y, s);
+          s = fmaf(sq[j + 2], v[t].z, s);
+          s = fmaf(sq[j + 3], v[t].w, s);
+        }
+      }
     }
   } else {
     for (int j = 0; j < d; ++j) s = fmaf(sq[j], xr[j], s);
   }
-  return fmaf(-2.0f, s, xnorm[idx]);
+  return fmaf(-2.0f, s, xn);
+}
+
+// Rows a wave stages per step of score_rows_staged, and the staging row
+// stride in floats (padding keeps the four chain lanes on different banks).
+constexpr int kRowGroup = 4;
+constexpr int kSelWaves = kSelBlock / 64;
+constexpr int kStagedMinQueries = 16;
+__host__ __device__ inline int stage_ld(int d) { return d + 4; }
+
+// Scores candidates [0, kp) of one query into s[] as (canonical rel, index)
+// words; empty candidates give kEmptyCand. Rows need 16-byte alignment and
+// d <= 128 * NP. Each wave owns a contiguous slice of the list and walks it
+// in steps of kRowGroup rows: half a wave loads one row, 16 B per lane (NP
+// loads per lane and row), so a load instruction touches two rows (a
+// random-row gather is limited by the rows one instruction touches, not by
+// bytes); the rows go through a double-buffered LDS stage, and while
+// kRowGroup lanes run the sequential FMA chains from LDS the next step's rows
+// are in flight. The loads are unconditional (absent rows read a present
+// row of the same chunk, parts past d re-read the last part) so that no
+// branch around them forces a wait.
+// One step's rows in registers: row slots half (r0) and 2 + half (r1), parts
+// p0 = lane % 32 and (NP == 2) p1 = p0 + 32, clamped to the row.
+struct StageRegs {
+  float4 r0p0, r0p1, r1p0, r1p1;
+};
+
+template <int NP>
+__device__ __forceinline__ StageRegs stage_load(const RerankArgs& a, int32_t rowi, int half, int p0, int p1,
+                                                int g) {
+  StageRegs v;
+  const int r = g * kRowGroup + half;
+  const float* x0 = a.x + (int64_t)__shfl(rowi, r & 63, 64) * a.ldx;
+  const float* x1 = a.x + (int64_t)__shfl(rowi, (r + 2) & 63, 64) * a.ldx;
+  v.r0p0 = *reinterpret_cast<const float4*>(x0 + 4 * p0);
+  v.r1p0 = *reinterpret_cast<const float4*>(x1 + 4 * p0);
+  if constexpr (NP == 2) {
+    v.r0p1 = *reinterpret_cast<const float4*>(x0 + 4 * p1);
+    v.r1p1 = *reinterpret_cast<const float4*>(x1 + 4 * p1);
+  }
+  return v;
+}
+
+template <int NP>
+__device__ void score_rows_staged(const RerankArgs& a, const float* sq, const uint64_t* src,
+                                  int kp, uint64_t* s, float* stage, float* sxn) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int d = a.d, ldr = stage_ld(d), parts = d >> 2;
+  float* const wstage = stage + (size_t)w * 2 * kRowGroup * ldr;
+  const int per = (kp + kSelWaves - 1) / kSelWaves;
+  const int c0 = min(kp, w * per), c1 = min(kp, c0 + per);
+  const int half = lane >> 5;
+  const int p0 = min(lane & 31, parts - 1), p1 = min((lane & 31) + 32, parts - 1);
+  const bool w0 = (lane & 31) < parts, w1 = NP == 2 && (lane & 31) + 32 < parts;
+  for (int cb = c0; cb < c1; cb += 64) {
+    const int nc = min(64, c1 - cb);
+    const uint64_t word = lane < nc ? src[cb + lane] : kEmptyCand;
+    const int32_t myidx = word != kEmptyCand ? cand_idx(word) : -1;
+    const uint64_t present = __ballot(myidx >= 0);
+    if (present == 0) {
+      if (lane < nc) s[cb + lane] = kEmptyCand;
+      continue;
+    }
+    const int32_t any_row = __shfl(myidx, __ffsll((unsigned long long)present) - 1, 64);
+    const int32_t rowi = myidx >= 0 ? myidx : any_row;
+    // The chunk's norms wait here, before any row load is in flight: read
+    // inside the loop, a register load would wait for the prefetched rows too.
+    sxn[w * 64 + lane] = a.xnorm[rowi];
+    const int ng = (nc + kRowGroup - 1) / kRowGroup;
+    StageRegs v = stage_load<NP>(a, rowi, half, p0, p1, 0);
+    for (int g = 0; g < ng; ++g) {
+      float* const rw = wstage + (g & 1) * kRowGroup * ldr;
+      float* const s0 = rw + half * ldr;
+      float* const s1 = rw + (2 + half) * ldr;
+      if (w0) {
+        *reinterpret_cast<float4*>(s0 + 4 * p0) = v.r0p0;
+        *reinterpret_cast<float4*>(s1 + 4 * p0) = v.r1p0;
+      }
+      if (w1) {
+        *reinterpret_cast<float4*>(s0 + 4 * p1) = v.r0p1;
+        *reinterpret_cast<float4*>(s1 + 4 * p1) = v.r1p1;
+      }
+      if (g + 1 < ng) v = stage_load<NP>(a, rowi, half, p0, p1, g + 1);
+      wave_sync();
+      const int r = g * kRowGroup + lane;
+      const int idx = __shfl(myidx, r & 63, 64);
+      if (lane < kRowGroup && r < nc) {
+        const float xn = sxn[w * 64 + r];
+        uint64_t out = kEmptyCand;
+        if (idx >= 0) {
+          const float* xr = rw + lane * ldr;
+          float acc = 0.0f;
+#pragma unroll 4
+          for (int j = 0; j < d; j += 4) {
+            const float4 t = *reinterpret_cast<const float4*>(xr + j);
+            acc = fmaf(sq[j + 0], t.x, acc);
+            acc = fmaf(sq[j + 1], t.y, acc);
+            acc = fmaf(sq[j + 2], t.z, acc);
+            acc = fmaf(sq[j + 3], t.w, acc);
+          }
+          out = pack_cand(fmaf(-2.0f, acc, xn), idx);
+        }
+        s[cb + r] = out;
+      }
+    }
+    wave_sync();
+  }
 }
 
 // sum_j (q_j - x_j)^2 in double, sequential j, no contraction: the same
-// arithmetic as the FP64 oracle.
+// arithmetic as the FP64 oracle. Loads batched as in canonical_rel.
 __device__ inline double fp64_dist(const float* __restrict__ x, int64_t ldx, int d,
-                                   const float* sq, int32_t idx) {
+                                   const float* sq, int32_t idx, bool vec4) {
   const float* xr = x + (int64_t)idx * ldx;
   double s = 0.0;
-  for (int j = 0; j < d; ++j) {
-    const double t = __dsub_rn((double)sq[j], (double)xr[j]);
-    s = __dadd_rn(s, __dmul_rn(t, t));
+  if (vec4) {
+    for (int j0 = 0; j0 < d; j0 += 4 * kRowBatch) {
+      float4 v[kRowBatch];
+#pragma unroll
+      for (int t = 0; t < kRowBatch; ++t)
+        if (j0 + 4 * t < d) v[t] = *reinterpret_cast<const float4*>(xr + j0 + 4 * t);
+#pragma unroll
+      for (int t = 0; t < kRowBatch; ++t) {
+        if (j0 + 4 * t < d) {
+          const int j = j0 + 4 * t;
+          const float e[4] = {v[t].x, v[t].y, v[t].z, v[t].w};
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const double dt = __dsub_rn((double)sq[j + u], (double)e[u]);
+            s = __dadd_rn(s, __dmul_rn(dt, dt));
+          }
+        }
+      }
+    }
+  } else {
+    for (int j = 0; j < d; ++j) {
+      const double t = __dsub_rn((double)sq[j], (double)xr[j]);
+      s = __dadd_rn(s, __dmul_rn(t, t));
+    }
   }
   return s;
 }
 
 constexpr int kMaxBand = 512;
 
+// Dynamic LDS, sized by the launch to the candidate list: s[P] words, then
+// one region used first as the row stage of score_rows_staged (when rows are
+// 16-byte aligned) and afterwards, for the FP64 band, as bw[B] words, bd[B]
+// doubles and bi[B] ints, B = min(kMaxBand, P).
+__host__ __device__ inline int rerank_band_cap(int P) { return P < kMaxBand ? P : kMaxBand; }
+__host__ __device__ inline int rerank_words(int P) { return P < 2 ? 2 : P; }
+__host__ __device__ inline size_t rerank_lds_bytes(int P, bool band, int d, bool staged) {
+  const size_t band_bytes = band ? (size_t)rerank_band_cap(P) * (8 + 8 + 4) : 0;
+  const size_t stage_bytes = staged ? (size_t)kSelWaves * 2 * kRowGroup * stage_ld(d) * 4 : 0;
+  return (size_t)rerank_words(P) * 8 + (band_bytes > stage_bytes ? band_bytes : stage_bytes);
+}
+__host__ __device__ inline bool rerank_staged(const float* x, int64_t ldx, int d) {
+  return d >= 4 && d % 4 == 0 && ldx % 4 == 0 && ((uintptr_t)x & 15) == 0;
+}
+
+// NP = 0: one lane per candidate (rows not 16-byte aligned, or a batch too
+// small to fill the device, where the staged walk's serial steps per wave
+// cost more than they save); NP = 1, 2: score_rows_staged<NP> (d <= 128,
+// d <= 256).
+template <int NP>
 __global__ __launch_bounds__(kSelBlock) void rerank_kernel(RerankArgs a) {
-  __shared__ uint64_t s[kMaxSortK];
+  extern __shared__ __attribute__((aligned(16))) uint64_t rerank_lds[];
   __shared__ float sq[256];
-  __shared__ double bd[kMaxBand];
-  __shared__ int bi[kMaxBand];
-  __shared__ uint64_t bw[kMaxBand];
+  __shared__ float sxn[kSelWaves * 64];
   __shared__ int sh[4];
+  uint64_t* s = rerank_lds;
   const int q = blockIdx.x;
   if (q >= a.nq || (a.nq_dev && q >= *a.nq_dev)) return;
   const int d = a.d;
   for (int j = threadIdx.x; j < d; j += blockDim.x) sq[j] = a.q[(int64_t)q * a.ldq + j];
   __syncthreads();
-  const bool vec4 = (d % 4 == 0) && (a.ldx % 4 == 0);
+  const bool vec4 = rerank_staged(a.x, a.ldx, d);  // 16-byte loads also in the FP64 band
   const int kp = a.kp;
   const int P = pow2_at_least(kp);
+  const int band_cap = rerank_band_cap(P);
+  uint64_t* bw = s + rerank_words(P);  // 16-byte aligned
+  double* bd = reinterpret_cast<double*>(bw + band_cap);
+  int* bi = reinterpret_cast<int*>(bd + band_cap);
   const uint64_t* src = a.topk + (int64_t)q * kp;
-  for (int i = threadIdx.x; i < P; i += blockDim.x) {
-    uint64_t w = kEmptyCand;
-    if (i < kp) {
-      const uint64_t c = src[i];
-      if (c != kEmptyCand) {
-        const int32_t idx = cand_idx(c);
-        w = pack_cand(canonical_rel(a.x, a.ldx, a.xnorm, d, sq, idx, vec4), idx);
+  if constexpr (NP > 0) {
+    score_rows_staged<NP>(a, sq, src, kp, s, reinterpret_cast<float*>(bw), sxn);
+    for (int i = kp + threadIdx.x; i < P; i += blockDim.x) s[i] = kEmptyCand;
+  } else {
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+      uint64_t w = kEmptyCand;
+      if (i < kp) {
+        const uint64_t c = src[i];
+        if (c != kEmptyCand) {
+          const int32_t idx = cand_idx(c);
+          w = pack_cand(canonical_rel(a.x, a.ldx, a.xnorm, d, sq, idx, vec4), idx);
+        }
       }
+      s[i] = w;
     }
-    s[i] = w;
   }
   __syncthreads();
   bitonic_sort_u64(s, P);
@@ -429,8 +601,8 @@ __global__ __launch_bounds__(kSelBlock) void rerank_kernel(RerankArgs a) {
       const float relk = cand_key(s[k - 1]);
       const float E = a.band_coeff * (qn + 2.0f * a.xn_max) + 4.0f * 5.96e-8f * (fabsf(relk) + qn);
       int b0 = k - 1, b1 = k - 1;
-      while (b0 > 0 && cand_key(s[b0 - 1]) >= relk - 2.0f * E && (k - 1) - (b0 - 1) < kMaxBand / 2) --b0;
-      while (b1 + 1 < valid && cand_key(s[b1 + 1]) <= relk + 2.0f * E && (b1 + 1) - b0 < kMaxBand) ++b1;
+      while (b0 > 0 && cand_key(s[b0 - 1]) >= relk - 2.0f * E && (k - 1) - (b0 - 1) < band_cap / 2) --b0;
+      while (b1 + 1 < valid && cand_key(s[b1 + 1]) <= relk + 2.0f * E && (b1 + 1) - b0 < band_cap) ++b1;
       sh[1] = b0;
       sh[2] = b1;
     }
@@ -440,7 +612,7 @@ __global__ __launch_bounds__(kSelBlock) void rerank_kernel(RerankArgs a) {
     if (nb > 1 && b1 >= k) {
       for (int i = threadIdx.x; i < nb; i += blockDim.x) {
         const int32_t idx = cand_idx(s[b0 + i]);
-        bd[i] = fp64_dist(a.x, a.ldx, d, sq, idx);
+        bd[i] = fp64_dist(a.x, a.ldx, d, sq, idx, vec4);
         bi[i] = b0 + i;
       }
       __syncthreads();
@@ -586,7 +758,15 @@ void launch_rerank(const RerankArgs& a, hipStream_t st) {
   if (a.nq <= 0) return;
   if (a.kp > kMaxSortK) fail(RPLSH_ERR_UNSUPPORTED, "re-rank list limited to %d", kMaxSortK);
   if (a.d > 256) fail(RPLSH_ERR_UNSUPPORTED, "re-rank dimension limited to 256");
-  rerank_kernel<<<a.nq, kSelBlock, 0, st>>>(a);
+  const int P = pow2_at_least(a.kp < 1 ? 1 : a.kp);
+  const bool staged = rerank_staged(a.x, a.ldx, a.d) && a.nq >= kStagedMinQueries;
+  const size_t lds = rerank_lds_bytes(P, a.fp64_boundary != 0, a.d, staged);
+  if (!staged)
+    rerank_kernel<0><<<a.nq, kSelBlock, lds, st>>>(a);
+  else if (a.d <= 128)
+    rerank_kernel<1><<<a.nq, kSelBlock, lds, st>>>(a);
+  else
+    rerank_kernel<2><<<a.nq, kSelBlock, lds, st>>>(a);
   RPLSH_HIP(hipGetLastError());
 }
 
