@@ -25,7 +25,7 @@ __all__ = [
     "DeviceResources", "DeviceArray", "IndexBF", "IndexBFConfig", "RplshError",
     "search_l2_distance", "compute_l2_norm", "encode_bf16", "encode_e4m3",
     "gen_uniform", "gen_gaussian", "gen_mixture", "device_count", "Event", "load",
-    "PREC_FP32", "PREC_BF16", "PREC_FP8", "Server",
+    "PREC_FP32", "PREC_BF16", "PREC_FP8", "Server", "IndexRPLSH", "IndexRPLSHConfig",
 ]
 
 _PREC = {"fp32": PREC_FP32, "plain": PREC_FP32, "bf16": PREC_BF16, "fp8": PREC_FP8,
@@ -191,6 +191,120 @@ class IndexBFConfig:
         c.kprime_factor = self.kprime_factor
         c.fp64_boundary = int(self.fp64_boundary)
         return c
+
+
+@dataclass
+class IndexRPLSHConfig:
+    """rplsh_lsh_config: random-projection LSH in front of the exact re-rank.
+
+    tables x bits: hash tables and code bits per table; probes: extra
+    single-bit-flip buckets per table (query-directed); max_candidates: bucket
+    entries taken per query before de-duplication (<= 4096); seed: projection
+    seed (standard normal, the library's generator)."""
+
+    device: int = 0
+    tables: int = 16
+    bits: int = 16
+    probes: int = 0
+    max_candidates: int = 1024
+    seed: int = 1234
+
+    def to_c(self) -> _lib.LshConfig:
+        c = _lib.LshConfig()
+        for f, _ in c._fields_:
+            setattr(c, f, int(getattr(self, f)))
+        return c
+
+
+class IndexRPLSH:
+    """Approximate kNN: random-projection LSH candidates, exact re-rank.
+
+    The "RPLSH" step of the reference's name, which the reference does not
+    implement (its only piece is the Gaussian projection initialiser,
+    src/internal/matrix_ops.cu:71-76). search() returns exact squared L2
+    distances of the candidates, ascending, ties by lower index, padded with
+    -1 / FLT_MAX when a query has fewer than k distinct candidates."""
+
+    def __init__(self, resources: DeviceResources, dim: int, config: IndexRPLSHConfig | None = None):
+        self.config = config or IndexRPLSHConfig()
+        self.resources = resources
+        self._h = C.c_void_p()
+        cfg = self.config.to_c()
+        call("rplsh_lsh_create", resources.handle, int(dim), C.byref(cfg), C.byref(self._h))
+        self.dim = int(dim)
+
+    def train(self, x) -> None:
+        """No training: the projections depend on the seed only."""
+
+    def add(self, x) -> None:
+        if isinstance(x, np.ndarray):
+            x = np.ascontiguousarray(x, dtype=np.float32)
+        n = int(x.shape[0])
+        if n and x.shape[1] != self.dim:
+            raise ValueError(f"expected dim {self.dim}, got {x.shape[1]}")
+        call("rplsh_lsh_add", self._h, n, _ptr(x) if n else None)
+
+    def search(self, q, k: int, indices=None, distances=None):
+        if isinstance(q, np.ndarray):
+            q = np.ascontiguousarray(q, dtype=np.float32)
+        nq = int(q.shape[0])
+        if indices is None:
+            indices = np.empty((nq, k), np.int32)
+        if distances is None:
+            distances = np.empty((nq, k), np.float32)
+        if nq:
+            call("rplsh_lsh_search", self._h, nq, _ptr(q), int(k), _ptr(indices), _ptr(distances))
+        return indices, distances
+
+    def reset(self) -> None:
+        call("rplsh_lsh_reset", self._h)
+
+    @property
+    def num_data(self) -> int:
+        n = C.c_int64()
+        call("rplsh_lsh_num_data", self._h, C.byref(n))
+        return n.value
+
+    def last_stats(self) -> dict:
+        s = _lib.LshStats()
+        call("rplsh_lsh_last_stats", self._h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def projections(self) -> np.ndarray:
+        """W [tables*bits][dim] (float32)."""
+        c = self.config
+        w = np.empty((c.tables * c.bits, self.dim), np.float32)
+        call("rplsh_lsh_projections", self._h, w.ctypes.data)
+        return w
+
+    def tables(self):
+        """(codes in row order, sorted codes, rows in bucket order), each [tables][n]."""
+        n, L = self.num_data, self.config.tables
+        codes = np.empty((L, n), np.uint32)
+        srt = np.empty((L, n), np.uint32)
+        perm = np.empty((L, n), np.int32)
+        call("rplsh_lsh_tables", self._h, codes.ctypes.data, srt.ctypes.data, perm.ctypes.data)
+        return codes, srt, perm
+
+    def hash_queries(self, q):
+        """(codes [tables][nq] uint32, dots [nq][tables*bits] float32)."""
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        nq, c = q.shape[0], self.config
+        codes = np.empty((c.tables, nq), np.uint32)
+        marg = np.empty((nq, c.tables * c.bits), np.float32)
+        call("rplsh_lsh_hash_queries", self._h, nq, q.ctypes.data, codes.ctypes.data, marg.ctypes.data)
+        return codes, marg
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            call("rplsh_lsh_destroy", self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class IndexBF:

@@ -137,6 +137,32 @@ _u64 = C.c_uint64
 _f32 = C.c_float
 _pf = C.POINTER(C.c_float)
 
+class LshConfig(C.Structure):
+    """rplsh_lsh_config: the random-projection LSH front end."""
+
+    _fields_ = [
+        ("device", C.c_int),
+        ("tables", C.c_int),
+        ("bits", C.c_int),
+        ("probes", C.c_int),
+        ("max_candidates", C.c_int),
+        ("seed", C.c_uint64),
+    ]
+
+
+class LshStats(C.Structure):
+    """rplsh_lsh_stats: counters of the last IndexRPLSH search."""
+
+    _fields_ = [
+        ("queries", C.c_int64),
+        ("raw_candidates", C.c_int64),
+        ("distinct_candidates", C.c_int64),
+        ("ms_total", C.c_float),
+        ("ms_candidates", C.c_float),
+        ("ms_rerank", C.c_float),
+    ]
+
+
 # name -> argtypes (restype is always c_int status unless listed in _RESTYPES)
 _SIGS = {
     "rplsh_last_error": [],
@@ -208,6 +234,18 @@ _SIGS = {
     "rplsh_xvecs_read": [C.c_char_p, _i32, _i64, _i64, _vp, _i32, _i32],
     "rplsh_xvecs_write": [C.c_char_p, _i32, _vp, _i64, _i32, _i32],
     "rplsh_recall_at": [_vp, _i64, _i32, _vp, _i32, _i32, _vp, C.POINTER(C.c_double)],
+    "rplsh_lsh_config_default": [C.POINTER(LshConfig)],
+    "rplsh_lsh_create": [_vp, _i32, C.POINTER(LshConfig), C.POINTER(_vp)],
+    "rplsh_lsh_destroy": [_vp],
+    "rplsh_lsh_add": [_vp, _i64, _vp],
+    "rplsh_lsh_reset": [_vp],
+    "rplsh_lsh_search": [_vp, _i64, _vp, _i32, _vp, _vp],
+    "rplsh_lsh_last_stats": [_vp, C.POINTER(LshStats)],
+    "rplsh_lsh_num_data": [_vp, C.POINTER(_i64)],
+    "rplsh_lsh_get_config": [_vp, C.POINTER(LshConfig)],
+    "rplsh_lsh_projections": [_vp, _vp],
+    "rplsh_lsh_tables": [_vp, _vp, _vp, _vp],
+    "rplsh_lsh_hash_queries": [_vp, _i64, _vp, _vp, _vp],
 }
 
 _RESTYPES = {
@@ -215,6 +253,7 @@ _RESTYPES = {
     "rplsh_version": C.c_char_p,
     "rplsh_bf_config_default": None,
     "rplsh_server_config_default": None,
+    "rplsh_lsh_config_default": None,
 }
 
 _lib = None

@@ -1,0 +1,638 @@
+// Random-projection LSH front end (IndexRPLSH): the "RPLSH" step the
+// reference's name promises but does not implement. Its only LSH piece is the
+// Gaussian projection-matrix initialiser (src/internal/matrix_ops.cu:71-76,
+// curandGenerateNormal with mean / stddev / seed); everything after it is
+// this library's own design, placed in front of the exact re-rank of the
+// brute-force path.
+//
+// Index (add):
+//   W [L*K][d]   standard normal projections from the library's seeded
+//                generator (row r = table r / K, bit r % K)
+//   code_t(x)    = sum_b [m_{t,b}(x) >= 0] << b,  m_{t,b}(x) = canonical fmaf
+//                chain over j of W[t*K+b][j] * x[j] (as the distance dot)
+//   table t      rows stably sorted by code_t (ties: ascending row), as
+//                sorted codes + row permutation, so a bucket is a range
+// Search, per query:
+//   probes       bucket code_t(q) and, for p = 1..P, code_t(q) with the bit of
+//                the p-th smallest |m_{t,b}(q)| flipped (ties: lower bit;
+//                query-directed multi-probe, Lv et al., VLDB 2007)
+//   candidates   buckets concatenated in (probe, table) order, rows in bucket
+//                order, cut after B entries, then de-duplicated
+//   result       canonical FP32 re-rank of the distinct candidates (the
+//                plain path's rerank_kernel): k smallest by (rel, index),
+//                D = rel + ||q||^2; fewer than k candidates pad with -1/FLT_MAX
+//
+// Kernels: lsh_hash_kernel (VALU, one thread per (row, table), the K chains
+// of a table interleaved so x is read once per table), lsh_gather_kernel (one
+// workgroup per query: probe codes, bucket ranges by binary search, a scan,
+// the gather, a bitonic sort and de-duplication in LDS). The table build is a
+// stable LSD radix sort per table (hipCUB) at add time, off the search path.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace rplsh {
+namespace {
+
+constexpr int kLshMaxBits = 24;
+constexpr int kLshMaxEntries = 1024;  // (probes + 1) * tables
+constexpr int kLshMaxBudget = 4096;   // the re-rank's longest list
+constexpr int kLshBlock = 256;
+
+// ---------------------------------------------------------------- hashing
+
+// Codes (and, for queries, margins) of rows [0, n) for table blockIdx.y.
+// Rows are 16-byte aligned with ldx % 4 == 0 (zero padding past d is not
+// read). codes[t * ldc + i]; margins[i * L * K + t * K + b] when not null.
+__global__ __launch_bounds__(kLshBlock) void lsh_hash_kernel(const float* __restrict__ x, int64_t ldx,
+                                                             int64_t n, int d, const float* __restrict__ w,
+                                                             int L, int K, uint32_t* __restrict__ codes,
+                                                             int64_t ldc, float* __restrict__ margins) {
+  extern __shared__ float wt[];  // [K][d] of table t
+  const int t = blockIdx.y;
+  for (int e = threadIdx.x; e < K * d; e += blockDim.x) wt[e] = w[(int64_t)t * K * d + e];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* xr = x + i * ldx;
+  float acc[kLshMaxBits];
+#pragma unroll
+  for (int b = 0; b < kLshMaxBits; ++b) acc[b] = 0.0f;
+  for (int j0 = 0; j0 < d; j0 += 4) {
+    const float4 v = *reinterpret_cast<const float4*>(xr + j0);
+    const float xv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = j0 + u;
+      if (j < d) {
+#pragma unroll
+        for (int b = 0; b < kLshMaxBits; ++b)
+          if (b < K) acc[b] = fmaf(wt[b * d + j], xv[u], acc[b]);
+      }
+    }
+  }
+  uint32_t code = 0;
+#pragma unroll
+  for (int b = 0; b < kLshMaxBits; ++b)
+    if (b < K && acc[b] >= 0.0f) code |= 1u << b;
+  codes[(int64_t)t * ldc + i] = code;
+  if (margins) {
+    float* mr = margins + i * (int64_t)L * K + (int64_t)t * K;
+#pragma unroll
+    for (int b = 0; b < kLshMaxBits; ++b)
+      if (b < K) mr[b] = acc[b];
+  }
+}
+
+__global__ void iota_kernel(int32_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (int32_t)i;
+}
+
+// --------------------------------------------------------------- gathering
+
+struct LshGatherArgs {
+  const uint32_t* sorted;  // [L][ld] sorted codes of each table
+  const int32_t* perm;     // [L][ld] rows in bucket order
+  int64_t ld;              // table stride
+  int64_t n;               // rows in each table
+  const uint32_t* qcodes;  // [L][nq]
+  const float* qmarg;      // [nq][L*K]
+  int nq, L, K, probes, budget;
+  uint64_t* cand;          // [nq][budget] words (index only; key ignored by the re-rank)
+  unsigned long long* raw_total;
+  unsigned long long* distinct_total;
+};
+
+__device__ inline int64_t lower_bound_u32(const uint32_t* a, int64_t n, uint32_t v) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Sort key of a probe candidate bit: |margin| as an ordered bit pattern
+// (NaN after +inf), then the bit index.
+__device__ inline uint64_t probe_key(float m, int b) {
+  return ((uint64_t)__float_as_uint(fabsf(m)) << 5) | (uint64_t)b;
+}
+
+__global__ __launch_bounds__(kLshBlock) void lsh_gather_kernel(LshGatherArgs a) {
+  __shared__ uint32_t pcode[kLshMaxEntries];
+  __shared__ int32_t elo[kLshMaxEntries];
+  __shared__ int32_t ecnt[kLshMaxEntries];
+  __shared__ int32_t eoff[kLshMaxEntries + 1];
+  __shared__ int32_t part[kLshBlock];
+  __shared__ int32_t buf[kLshMaxBudget];
+  __shared__ int32_t sh_distinct;
+  const int q = blockIdx.x;
+  if (q >= a.nq) return;
+  const int L = a.L, K = a.K, P = a.probes, B = a.budget;
+  const int E = (P + 1) * L;
+
+  // 1. probe codes of every table, entry e = p * L + t
+  for (int t = threadIdx.x; t < L; t += blockDim.x) {
+    const uint32_t c = a.qcodes[(int64_t)t * a.nq + q];
+    pcode[t] = c;
+    const float* m = a.qmarg + (int64_t)q * L * K + (int64_t)t * K;
+    uint32_t used = 0;
+    for (int p = 1; p <= P; ++p) {
+      uint64_t best = ~0ull;
+      for (int b = 0; b < K; ++b) {
+        if (used >> b & 1u) continue;
+        const uint64_t key = probe_key(m[b], b);
+        if (key < best) best = key;
+      }
+      const int bb = (int)(best & 31u);
+      used |= 1u << bb;
+      pcode[p * L + t] = c ^ (1u << bb);
+    }
+  }
+  __syncthreads();
+
+  // 2. bucket ranges; counts clamped to B (only the first B entries matter)
+  for (int e = threadIdx.x; e < E; e += blockDim.x) {
+    const int t = e % L;
+    const uint32_t* s = a.sorted + (int64_t)t * a.ld;
+    const uint32_t c = pcode[e];
+    const int64_t lo = lower_bound_u32(s, a.n, c);
+    const int64_t hi = c == 0xffffffffu ? a.n : lower_bound_u32(s, a.n, c + 1);
+    elo[e] = (int32_t)lo;
+    ecnt[e] = (int32_t)(hi - lo < B ? hi - lo : B);
+  }
+  __syncthreads();
+
+  // 3. exclusive scan of the counts: each thread sums a run of entries, a
+  //    scan over the threads' sums, then the runs are rewritten
+  const int per = (E + blockDim.x - 1) / blockDim.x;
+  const int e0 = threadIdx.x * per, e1 = min(E, e0 + per);
+  int run = 0;
+  for (int e = e0; e < e1; ++e) run += ecnt[e];
+  part[threadIdx.x] = run;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    const int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int acc = part[threadIdx.x] - run;  // exclusive prefix of this run
+  for (int e = e0; e < e1; ++e) {
+    eoff[e] = acc;
+    acc += ecnt[e];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) eoff[E] = part[blockDim.x - 1];
+  __syncthreads();
+  const int R = min(eoff[E], B);
+
+  // 4. gather the first R entries of the concatenation
+  for (int j = threadIdx.x; j < R; j += blockDim.x) {
+    int lo = 0, hi = E;  // last e with eoff[e] <= j
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (eoff[mid] <= j) lo = mid; else hi = mid;
+    }
+    const int t = lo % L;
+    buf[j] = a.perm[(int64_t)t * a.ld + elo[lo] + (j - eoff[lo])];
+  }
+  int P2 = 1;
+  while (P2 < R) P2 <<= 1;
+  for (int j = R + threadIdx.x; j < P2; j += blockDim.x) buf[j] = INT_MAX;
+  if (threadIdx.x == 0) sh_distinct = 0;
+  __syncthreads();
+
+  // 5. bitonic sort of buf[0, P2)
+  for (int k = 2; k <= P2; k <<= 1) {
+    for (int s = k >> 1; s > 0; s >>= 1) {
+      for (int i = threadIdx.x; i < (P2 >> 1); i += blockDim.x) {
+        const int lo = ((i & ~(s - 1)) << 1) | (i & (s - 1));
+        const int hi = lo + s;
+        const bool up = (lo & k) == 0;
+        const int x = buf[lo], y = buf[hi];
+        if ((x > y) == up) {
+          buf[lo] = y;
+          buf[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // 6. distinct rows as candidate words, the rest empty
+  int mine = 0;
+  uint64_t* out = a.cand + (int64_t)q * B;
+  for (int j = threadIdx.x; j < B; j += blockDim.x) {
+    uint64_t wv = kEmptyCand;
+    if (j < R && (j == 0 || buf[j - 1] != buf[j])) {
+      wv = pack_cand(0.0f, buf[j]);
+      ++mine;
+    }
+    out[j] = wv;
+  }
+  if (mine) atomicAdd(&sh_distinct, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(a.raw_total, (unsigned long long)R);
+    atomicAdd(a.distinct_total, (unsigned long long)sh_distinct);
+  }
+}
+
+}  // namespace
+}  // namespace rplsh
+
+using namespace rplsh;
+
+struct rplsh_lsh {
+  rplsh_resources* res = nullptr;
+  int dim = 0;
+  rplsh_lsh_config cfg{};
+  int64_t n = 0, capacity = 0, ldx = 0;
+  float* x = nullptr;         // [capacity][ldx], zero padded
+  float* xn = nullptr;        // [capacity]
+  float* w = nullptr;         // [L*K][dim]
+  uint32_t* codes = nullptr;  // [L][capacity] codes in row order
+  uint32_t* sorted = nullptr; // [L][capacity]
+  int32_t* perm = nullptr;    // [L][capacity]
+  DevBuf iota, sort_tmp, qstage, qpad, qn, qcodes, qmarg, cand, idev, ddev, counters, status, relk, fp64;
+  rplsh_lsh_stats last{};
+};
+
+namespace {
+
+hipStream_t lsh_stream(rplsh_lsh* h) {
+  void* s = nullptr;
+  if (rplsh_resources_default_stream(h->res, h->cfg.device, &s) != RPLSH_OK)
+    fail(RPLSH_ERR_STATE, "default stream: %s", rplsh_last_error());
+  return (hipStream_t)s;
+}
+
+void lsh_free(rplsh_lsh* h) {
+  for (void* p : {(void*)h->x, (void*)h->xn, (void*)h->codes, (void*)h->sorted, (void*)h->perm})
+    if (p) RPLSH_HIP(hipFree(p));
+  h->x = h->xn = nullptr;
+  h->codes = h->sorted = nullptr;
+  h->perm = nullptr;
+  h->n = h->capacity = 0;
+}
+
+void launch_hash(const rplsh_lsh* h, const float* rows, int64_t n, uint32_t* codes, int64_t ldc,
+                 float* margins, hipStream_t st) {
+  if (n <= 0) return;
+  const int L = h->cfg.tables, K = h->cfg.bits;
+  const dim3 grid((unsigned)div_up(n, kLshBlock), (unsigned)L);
+  const size_t lds = (size_t)K * h->dim * sizeof(float);
+  lsh_hash_kernel<<<grid, kLshBlock, lds, st>>>(rows, h->ldx, n, h->dim, h->w, L, K, codes, ldc, margins);
+  RPLSH_HIP(hipGetLastError());
+}
+
+// Queries (host or device, [nq][dim]) as device rows [nq][ldx], zero padded.
+float* stage_queries(rplsh_lsh* h, int64_t nq, const float* queries, hipStream_t st) {
+  const float* src = queries;
+  if (pointer_device(queries) != h->cfg.device) {
+    float* t = h->qstage.as<float>((size_t)nq * h->dim);
+    RPLSH_HIP(hipMemcpyAsync(t, queries, (size_t)nq * h->dim * sizeof(float), hipMemcpyDefault, st));
+    src = t;
+  }
+  float* qp = h->qpad.as<float>((size_t)nq * h->ldx);
+  launch_pad_rows(src, nq, h->dim, h->dim, (int)h->ldx, qp, st);
+  return qp;
+}
+
+// Sorts every table's (code, row) pairs of rows [0, n) by code, stably.
+void build_tables(rplsh_lsh* h, hipStream_t st) {
+  const int L = h->cfg.tables;
+  int32_t* rows = h->iota.as<int32_t>((size_t)h->n);
+  iota_kernel<<<(unsigned)std::min<int64_t>(div_up(h->n, 256), 65536), 256, 0, st>>>(rows, h->n);
+  RPLSH_HIP(hipGetLastError());
+  size_t tmp = 0;
+  RPLSH_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, h->codes, h->sorted, rows, h->perm,
+                                               (int)h->n, 0, h->cfg.bits, st));
+  void* tp = h->sort_tmp.ensure(tmp);
+  for (int t = 0; t < L; ++t) {
+    const int64_t o = (int64_t)t * h->capacity;
+    size_t tb = tmp;
+    RPLSH_HIP(hipcub::DeviceRadixSort::SortPairs(tp, tb, h->codes + o, h->sorted + o, rows, h->perm + o,
+                                                 (int)h->n, 0, h->cfg.bits, st));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void rplsh_lsh_config_default(rplsh_lsh_config* cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->device = 0;
+  cfg->tables = 16;
+  cfg->bits = 16;
+  cfg->probes = 0;
+  cfg->max_candidates = 1024;
+  cfg->seed = 1234;
+}
+
+rplsh_status rplsh_lsh_create(rplsh_resources* res, int dim, const rplsh_lsh_config* cfg, rplsh_lsh** out) {
+  return guard([&] {
+    RPLSH_REQUIRE(out, "out is NULL");
+    RPLSH_REQUIRE(res, "resources is NULL");
+    RPLSH_REQUIRE(dim > 0 && dim <= 256, "dimension must be in [1, 256] (got %d)", dim);
+    rplsh_lsh_config c;
+    rplsh_lsh_config_default(&c);
+    if (cfg) c = *cfg;
+    int count = 0;
+    rplsh_device_count(&count);
+    RPLSH_REQUIRE(c.device >= 0 && c.device < count, "device %d out of range [0, %d)", c.device, count);
+    RPLSH_REQUIRE(c.tables >= 1 && c.bits >= 1 && c.bits <= kLshMaxBits,
+                  "tables >= 1 and bits in [1, %d] required", kLshMaxBits);
+    RPLSH_REQUIRE(c.probes >= 0 && c.probes <= c.bits, "probes must be in [0, bits]");
+    RPLSH_REQUIRE((int64_t)(c.probes + 1) * c.tables <= kLshMaxEntries,
+                  "(probes + 1) * tables must be <= %d", kLshMaxEntries);
+    RPLSH_REQUIRE(c.max_candidates >= 1 && c.max_candidates <= kLshMaxBudget,
+                  "max_candidates must be in [1, %d]", kLshMaxBudget);
+    if (rplsh_resources_init_device(res, c.device) != RPLSH_OK)
+      fail(RPLSH_ERR_STATE, "%s", rplsh_last_error());
+    DeviceScope scope(c.device);
+    auto* h = new rplsh_lsh();
+    h->res = res;
+    h->dim = dim;
+    h->cfg = c;
+    h->ldx = round_up(dim, 4);
+    // matrixRandomInitialize (matrix_ops.cu:71-76) with mean 0, stddev 1: the
+    // signs are what matter, so the scale is left at 1.
+    const int64_t rows = (int64_t)c.tables * c.bits;
+    std::vector<float> wh((size_t)rows * dim);
+    if (rplsh_gen_gaussian(wh.data(), rows, dim, c.seed, 0) != RPLSH_OK) {
+      delete h;
+      fail(RPLSH_ERR_STATE, "projection generator: %s", rplsh_last_error());
+    }
+    try {
+      RPLSH_HIP(hipMalloc(&h->w, wh.size() * sizeof(float)));
+      RPLSH_HIP(hipMemcpy(h->w, wh.data(), wh.size() * sizeof(float), hipMemcpyHostToDevice));
+    } catch (...) {
+      if (h->w) (void)hipFree(h->w);
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+rplsh_status rplsh_lsh_destroy(rplsh_lsh* h) {
+  return guard([&] {
+    if (!h) return;
+    DeviceScope scope(h->cfg.device);
+    (void)hipDeviceSynchronize();
+    lsh_free(h);
+    if (h->w) RPLSH_HIP(hipFree(h->w));
+    delete h;
+  });
+}
+
+rplsh_status rplsh_lsh_reset(rplsh_lsh* h) {
+  return guard([&] {
+    RPLSH_REQUIRE(h, "index is NULL");
+    DeviceScope scope(h->cfg.device);
+    RPLSH_HIP(hipStreamSynchronize(lsh_stream(h)));
+    lsh_free(h);
+  });
+}
+
+rplsh_status rplsh_lsh_add(rplsh_lsh* h, int64_t num, const float* data) {
+  return guard([&] {
+    RPLSH_REQUIRE(h, "index is NULL");
+    if (num <= 0) return;
+    RPLSH_REQUIRE(data, "data is NULL");
+    RPLSH_REQUIRE(h->n + num < (1ll << 31), "index size limited to 2^31 rows");
+    DeviceScope scope(h->cfg.device);
+    hipStream_t st = lsh_stream(h);
+    const int L = h->cfg.tables;
+    const int64_t need = h->n + num;
+    if (need > h->capacity) {
+      const int64_t cap = h->capacity == 0 ? need : std::max(need, h->capacity + h->capacity / 2);
+      float *nx = nullptr, *nn = nullptr;
+      uint32_t *nc = nullptr, *ns = nullptr;
+      int32_t* np = nullptr;
+      RPLSH_HIP(hipMalloc(&nx, (size_t)cap * h->ldx * sizeof(float)));
+      RPLSH_HIP(hipMalloc(&nn, (size_t)cap * sizeof(float)));
+      RPLSH_HIP(hipMalloc(&nc, (size_t)cap * L * sizeof(uint32_t)));
+      RPLSH_HIP(hipMalloc(&ns, (size_t)cap * L * sizeof(uint32_t)));
+      RPLSH_HIP(hipMalloc(&np, (size_t)cap * L * sizeof(int32_t)));
+      if (h->n > 0) {
+        RPLSH_HIP(hipMemcpyAsync(nx, h->x, (size_t)h->n * h->ldx * sizeof(float), hipMemcpyDeviceToDevice, st));
+        RPLSH_HIP(hipMemcpyAsync(nn, h->xn, (size_t)h->n * sizeof(float), hipMemcpyDeviceToDevice, st));
+        RPLSH_HIP(hipMemcpy2DAsync(nc, cap * sizeof(uint32_t), h->codes, h->capacity * sizeof(uint32_t),
+                                   h->n * sizeof(uint32_t), L, hipMemcpyDeviceToDevice, st));
+      }
+      RPLSH_HIP(hipStreamSynchronize(st));
+      const int64_t n_keep = h->n;
+      lsh_free(h);
+      h->x = nx;
+      h->xn = nn;
+      h->codes = nc;
+      h->sorted = ns;
+      h->perm = np;
+      h->capacity = cap;
+      h->n = n_keep;
+    }
+    float* dst = h->x + h->n * h->ldx;
+    if (h->ldx == h->dim) {
+      RPLSH_HIP(hipMemcpyAsync(dst, data, (size_t)num * h->dim * sizeof(float), hipMemcpyDefault, st));
+    } else {
+      RPLSH_HIP(hipMemsetAsync(dst, 0, (size_t)num * h->ldx * sizeof(float), st));
+      RPLSH_HIP(hipMemcpy2DAsync(dst, h->ldx * sizeof(float), data, h->dim * sizeof(float),
+                                 h->dim * sizeof(float), num, hipMemcpyDefault, st));
+    }
+    launch_l2norm(dst, num, h->dim, h->ldx, h->xn + h->n, 1, st);
+    launch_hash(h, dst, num, h->codes + h->n, h->capacity, nullptr, st);
+    h->n = need;
+    build_tables(h, st);
+    RPLSH_HIP(hipStreamSynchronize(st));
+  });
+}
+
+rplsh_status rplsh_lsh_search(rplsh_lsh* h, int64_t nq, const float* queries, int k, int32_t* indices,
+                              float* distances) {
+  return guard([&] {
+    RPLSH_REQUIRE(h, "index is NULL");
+    if (nq <= 0) return;
+    RPLSH_REQUIRE(queries && indices && distances, "NULL pointer");
+    RPLSH_REQUIRE(k > 0, "k must be > 0 (got %d)", k);
+    RPLSH_REQUIRE(nq < (1ll << 31) && (int64_t)nq * k < (1ll << 40), "batch too large");
+    DeviceScope scope(h->cfg.device);
+    hipStream_t st = lsh_stream(h);
+    const int dev = h->cfg.device;
+    const int L = h->cfg.tables, K = h->cfg.bits, B = h->cfg.max_candidates;
+    rplsh_lsh_stats stats{};
+    stats.queries = nq;
+    hipEvent_t ev[4];
+    for (auto& e : ev) RPLSH_HIP(hipEventCreate(&e));
+    RPLSH_HIP(hipEventRecord(ev[0], st));
+
+    const bool i_dev = pointer_device(indices) == dev;
+    const bool d_dev = pointer_device(distances) == dev;
+    int32_t* id = i_dev ? indices : h->idev.as<int32_t>((size_t)nq * k);
+    float* dd = d_dev ? distances : h->ddev.as<float>((size_t)nq * k);
+    unsigned long long* ctr = h->counters.as<unsigned long long>(2);
+    RPLSH_HIP(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned long long), st));
+
+    if (h->n == 0) {
+      launch_fill_i32(id, nq * k, -1, st);
+      launch_fill_f32(dd, nq * k, FLT_MAX, st);
+      RPLSH_HIP(hipEventRecord(ev[1], st));
+      RPLSH_HIP(hipEventRecord(ev[2], st));
+    } else {
+      // queries: padded rows (16-byte aligned, zero padding), norms, codes and margins
+      float* qp = stage_queries(h, nq, queries, st);
+      float* qn = h->qn.as<float>((size_t)nq);
+      launch_l2norm(qp, nq, h->dim, h->ldx, qn, 1, st);
+      uint32_t* qc = h->qcodes.as<uint32_t>((size_t)nq * L);
+      float* qm = h->qmarg.as<float>((size_t)nq * L * K);
+      launch_hash(h, qp, nq, qc, nq, qm, st);
+      // candidates
+      uint64_t* cand = h->cand.as<uint64_t>((size_t)nq * B);
+      LshGatherArgs ga{};
+      ga.sorted = h->sorted;
+      ga.perm = h->perm;
+      ga.ld = h->capacity;
+      ga.n = h->n;
+      ga.qcodes = qc;
+      ga.qmarg = qm;
+      ga.nq = (int)nq;
+      ga.L = L;
+      ga.K = K;
+      ga.probes = h->cfg.probes;
+      ga.budget = B;
+      ga.cand = cand;
+      ga.raw_total = ctr;
+      ga.distinct_total = ctr + 1;
+      lsh_gather_kernel<<<(unsigned)nq, kLshBlock, 0, st>>>(ga);
+      RPLSH_HIP(hipGetLastError());
+      RPLSH_HIP(hipEventRecord(ev[1], st));
+      // exact re-rank of the distinct candidates (no certificate, no FP64 band)
+      int* status = h->status.as<int>((size_t)nq);
+      float* relk = h->relk.as<float>((size_t)nq);
+      int* fp64c = h->fp64.as<int>(1);
+      launch_fill_i32(status, nq, 0, st);
+      launch_fill_i32(fp64c, 1, 0, st);
+      RerankArgs ra{};
+      ra.x = h->x;
+      ra.ldx = h->ldx;
+      ra.xnorm = h->xn;
+      ra.d = h->dim;
+      ra.q = qp;
+      ra.ldq = h->ldx;
+      ra.qnorm = qn;
+      ra.topk = cand;
+      ra.kp = B;
+      ra.k = k;
+      ra.nq = (int)nq;
+      ra.exact = 1;
+      ra.bound = nullptr;
+      ra.fp64_boundary = 0;
+      ra.out_idx = id;
+      ra.out_dist = dd;
+      ra.status = status;
+      ra.relk_out = relk;
+      ra.fp64_count = fp64c;
+      launch_rerank(ra, st);
+      RPLSH_HIP(hipEventRecord(ev[2], st));
+    }
+    if (!i_dev) RPLSH_HIP(hipMemcpyAsync(indices, id, (size_t)nq * k * sizeof(int32_t), hipMemcpyDefault, st));
+    if (!d_dev) RPLSH_HIP(hipMemcpyAsync(distances, dd, (size_t)nq * k * sizeof(float), hipMemcpyDefault, st));
+    unsigned long long hc[2] = {0, 0};
+    RPLSH_HIP(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
+    RPLSH_HIP(hipEventRecord(ev[3], st));
+    RPLSH_HIP(hipEventSynchronize(ev[3]));
+    float ms = 0.0f;
+    RPLSH_HIP(hipEventElapsedTime(&ms, ev[0], ev[3]));
+    stats.ms_total = ms;
+    RPLSH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    stats.ms_candidates = ms;
+    RPLSH_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
+    stats.ms_rerank = ms;
+    stats.raw_candidates = (int64_t)hc[0];
+    stats.distinct_candidates = (int64_t)hc[1];
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    h->last = stats;
+  });
+}
+
+rplsh_status rplsh_lsh_last_stats(const rplsh_lsh* h, rplsh_lsh_stats* out) {
+  return guard([&] {
+    RPLSH_REQUIRE(h && out, "NULL argument");
+    *out = h->last;
+  });
+}
+
+rplsh_status rplsh_lsh_num_data(const rplsh_lsh* h, int64_t* n) {
+  return guard([&] {
+    RPLSH_REQUIRE(h && n, "NULL argument");
+    *n = h->n;
+  });
+}
+
+rplsh_status rplsh_lsh_get_config(const rplsh_lsh* h, rplsh_lsh_config* out) {
+  return guard([&] {
+    RPLSH_REQUIRE(h && out, "NULL argument");
+    *out = h->cfg;
+  });
+}
+
+rplsh_status rplsh_lsh_projections(const rplsh_lsh* h, float* out) {
+  return guard([&] {
+    RPLSH_REQUIRE(h && out, "NULL argument");
+    DeviceScope scope(h->cfg.device);
+    RPLSH_HIP(hipMemcpy(out, h->w, (size_t)h->cfg.tables * h->cfg.bits * h->dim * sizeof(float),
+                        hipMemcpyDeviceToHost));
+  });
+}
+
+rplsh_status rplsh_lsh_tables(const rplsh_lsh* h, uint32_t* codes, uint32_t* sorted, int32_t* perm) {
+  return guard([&] {
+    RPLSH_REQUIRE(h, "index is NULL");
+    DeviceScope scope(h->cfg.device);
+    const size_t row = (size_t)h->n;
+    for (int t = 0; t < h->cfg.tables && row > 0; ++t) {
+      const size_t o = (size_t)t * h->capacity;
+      if (codes) RPLSH_HIP(hipMemcpy(codes + t * row, h->codes + o, row * 4, hipMemcpyDeviceToHost));
+      if (sorted) RPLSH_HIP(hipMemcpy(sorted + t * row, h->sorted + o, row * 4, hipMemcpyDeviceToHost));
+      if (perm) RPLSH_HIP(hipMemcpy(perm + t * row, h->perm + o, row * 4, hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+rplsh_status rplsh_lsh_hash_queries(rplsh_lsh* h, int64_t nq, const float* queries, uint32_t* codes,
+                                    float* margins) {
+  return guard([&] {
+    RPLSH_REQUIRE(h && queries && codes, "NULL argument");
+    if (nq <= 0) return;
+    DeviceScope scope(h->cfg.device);
+    hipStream_t st = lsh_stream(h);
+    const int L = h->cfg.tables, K = h->cfg.bits;
+    float* qp = stage_queries(h, nq, queries, st);
+    uint32_t* qc = h->qcodes.as<uint32_t>((size_t)nq * L);
+    float* qm = h->qmarg.as<float>((size_t)nq * L * K);
+    launch_hash(h, qp, nq, qc, nq, qm, st);
+    RPLSH_HIP(hipMemcpyAsync(codes, qc, (size_t)nq * L * sizeof(uint32_t), hipMemcpyDefault, st));
+    if (margins)
+      RPLSH_HIP(hipMemcpyAsync(margins, qm, (size_t)nq * L * K * sizeof(float), hipMemcpyDefault, st));
+    RPLSH_HIP(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"

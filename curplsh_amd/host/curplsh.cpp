@@ -6,6 +6,7 @@
 #include "curplsh/device_resources.h"
 #include "curplsh/index.h"
 #include "curplsh/index_bf.h"
+#include "curplsh/index_rplsh.h"
 #include "curplsh/internal/assertions.h"
 #include "curplsh/internal/norm.h"
 #include "curplsh/internal/search.h"
@@ -146,6 +147,54 @@ rplsh_search_stats IndexBF::lastStats() const {
   rplsh_search_stats s;
   std::memset(&s, 0, sizeof(s));
   checkStatus(rplsh_bf_last_stats(handle_, &s), "IndexBF::lastStats");
+  return s;
+}
+
+// ------------------------------------------------------------ IndexRPLSH
+
+IndexRPLSH::IndexRPLSH(DeviceResources* resources, int dim, IndexRPLSHConfig config)
+    : Index(resources, dim, config), config_(config) {
+  rplsh_lsh_config c;
+  rplsh_lsh_config_default(&c);
+  c.device = config.device;
+  c.tables = config.numTables;
+  c.bits = config.numBits;
+  c.probes = config.numProbes;
+  c.max_candidates = config.maxCandidates;
+  c.seed = config.seed;
+  host_assert(config.memorySpace == MemorySpace::Device);
+  checkStatus(rplsh_lsh_create(resources->handle(), dim, &c, &handle_), "IndexRPLSH");
+  isTrained_ = true;
+}
+
+IndexRPLSH::~IndexRPLSH() {
+  if (handle_) rplsh_lsh_destroy(handle_);
+}
+
+void IndexRPLSH::train(int, const float*) {}
+
+void IndexRPLSH::add(int num, const float* data) {
+  if (num <= 0) return;
+  checkStatus(rplsh_lsh_add(handle_, num, data), "IndexRPLSH::add");
+  int64_t n = 0;
+  checkStatus(rplsh_lsh_num_data(handle_, &n), "IndexRPLSH::add");
+  numData_ = (int)n;
+}
+
+void IndexRPLSH::search(int num, const float* queries, int k, int* indices, float* distances) {
+  if (num <= 0) return;
+  checkStatus(rplsh_lsh_search(handle_, num, queries, k, indices, distances), "IndexRPLSH::search");
+}
+
+void IndexRPLSH::reset() {
+  checkStatus(rplsh_lsh_reset(handle_), "IndexRPLSH::reset");
+  numData_ = 0;
+}
+
+rplsh_lsh_stats IndexRPLSH::lastStats() const {
+  rplsh_lsh_stats s;
+  std::memset(&s, 0, sizeof(s));
+  checkStatus(rplsh_lsh_last_stats(handle_, &s), "IndexRPLSH::lastStats");
   return s;
 }
 

@@ -252,6 +252,73 @@ rplsh_status rplsh_bf_autotune(rplsh_bf* index, const float* sample, int64_t nsa
 rplsh_status rplsh_bf_set_tuning(rplsh_bf* index, const rplsh_tune_result* t);
 rplsh_status rplsh_bf_clear_tuning(rplsh_bf* index);
 
+/* --------------------------------------------------------- IndexRPLSH -- */
+/* Random-projection LSH front end before the exact re-rank: the "RPLSH" step
+ * of the reference's name, which the reference does not implement (its only
+ * piece is the Gaussian projection initialiser matrixRandomInitialize,
+ * src/internal/matrix_ops.cu:71-76). Approximate: a row is a candidate only
+ * if it shares a probed bucket with the query.
+ *  - projections W [tables*bits][dim]: standard normal, from the library's
+ *    seeded generator (rplsh_gen_gaussian(W, tables*bits, dim, seed, 0));
+ *  - code of x in table t: bit b set iff the canonical FP32 dot (fmaf chain
+ *    over j, as the distance dot) of W[t*bits+b] and x is >= 0;
+ *  - per table, rows stably sorted by code (a bucket is a range, rows
+ *    ascending within it);
+ *  - a query probes, in each table, its own bucket and then, for p = 1..probes,
+ *    the bucket with the bit of its p-th smallest |dot| flipped (ties: lower
+ *    bit); buckets are concatenated in (probe, table) order, cut after
+ *    max_candidates entries and de-duplicated;
+ *  - the distinct candidates are re-ranked exactly (canonical FP32, as the
+ *    plain path): k smallest by (distance, index), padded with -1 / FLT_MAX. */
+typedef struct rplsh_lsh_config {
+  int device;
+  int tables;          /* hash tables, >= 1 */
+  int bits;            /* bits per table code, 1..24 */
+  int probes;          /* extra single-bit-flip buckets per table, 0..bits;
+                          (probes + 1) * tables <= 1024 */
+  int max_candidates;  /* bucket entries taken per query before de-duplication, 1..4096 */
+  uint64_t seed;       /* projection seed */
+} rplsh_lsh_config;
+
+/* Fills the defaults: device 0, 16 tables x 16 bits, no extra probes,
+ * 1024 candidates, seed 1234. */
+void rplsh_lsh_config_default(rplsh_lsh_config* cfg);
+
+typedef struct rplsh_lsh rplsh_lsh;
+
+typedef struct rplsh_lsh_stats {
+  int64_t queries;
+  int64_t raw_candidates;       /* bucket entries taken (sum over queries) */
+  int64_t distinct_candidates;  /* after de-duplication: rows re-ranked */
+  float ms_total;               /* device time of the call */
+  float ms_candidates;          /* query hashing and candidate gathering */
+  float ms_rerank;              /* exact re-rank */
+} rplsh_lsh_stats;
+
+rplsh_status rplsh_lsh_create(rplsh_resources* res, int dim, const rplsh_lsh_config* cfg,
+                              rplsh_lsh** out);
+rplsh_status rplsh_lsh_destroy(rplsh_lsh* index);
+/* Appends rows (host or device pointer), hashes them and rebuilds the tables. */
+rplsh_status rplsh_lsh_add(rplsh_lsh* index, int64_t n, const float* x);
+rplsh_status rplsh_lsh_reset(rplsh_lsh* index);
+/* k approximate nearest neighbours with exact squared L2 distances,
+ * ascending, ties by lower index; host or device pointers. */
+rplsh_status rplsh_lsh_search(rplsh_lsh* index, int64_t nq, const float* queries, int k,
+                              int32_t* indices, float* distances);
+rplsh_status rplsh_lsh_last_stats(const rplsh_lsh* index, rplsh_lsh_stats* out);
+rplsh_status rplsh_lsh_num_data(const rplsh_lsh* index, int64_t* n);
+rplsh_status rplsh_lsh_get_config(const rplsh_lsh* index, rplsh_lsh_config* out);
+/* W [tables*bits][dim] into host memory. */
+rplsh_status rplsh_lsh_projections(const rplsh_lsh* index, float* out);
+/* Host copies [tables][n] of the codes in row order, the sorted codes and the
+ * rows in bucket order; any of the three may be NULL. */
+rplsh_status rplsh_lsh_tables(const rplsh_lsh* index, uint32_t* codes, uint32_t* sorted,
+                              int32_t* perm);
+/* Codes [tables][nq] and (if not NULL) dots [nq][tables*bits] of queries
+ * (host or device; outputs host or device). */
+rplsh_status rplsh_lsh_hash_queries(rplsh_lsh* index, int64_t nq, const float* queries,
+                                    uint32_t* codes, float* margins);
+
 /* ----------------------------------------------------------- serving -- */
 /* Continuous-batching request loop over resident indexes (no reference
  * counterpart). `slots` worker threads each own a HIP stream and a workspace;

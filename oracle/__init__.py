@@ -18,13 +18,20 @@ reference repository):
                     recall and distance error for every mode)
   recall            Dataset::evaluate, include/dataset.h:135-152
   bf16 / e4m3       encoders from IEEE-754 and the OCP OFP8 spec
+  lsh_hash /        the library's random-projection LSH front end (IndexRPLSH);
+  lsh_search        the reference has no LSH step (only the Gaussian
+                    initialiser of src/internal/matrix_ops.cu:71-76), so this
+                    restates the library's own specification
 
 Pinning (tests/test_oracle.py): closed-form fixtures of test/test_search.cpp:17-36
 and test/test_norm.cpp:15-22, exact rational arithmetic on tiny inputs, numpy
 float64 brute force, torch's bfloat16 / float8_e4m3fn conversions, and the
-exhaustive e4m3 code table.
+exhaustive e4m3 code table. LSH (tests/test_oracle_lsh.py): float64 signs of
+the projections, the all-buckets case against search_canonical, and the
+candidate sets rebuilt by set membership and by an explicit bucket
+concatenation in numpy.
 """
 from .oracle import (  # noqa: F401
-    LIB_PATH, bf16_rne, bf16_to_float, e4m3_rne_sat, e4m3_to_float, load, norms_canonical,
-    pair_dist_fp64, recall, search_canonical, search_fp64,
+    LIB_PATH, bf16_rne, bf16_to_float, e4m3_rne_sat, e4m3_to_float, load, lsh_hash, lsh_search,
+    norms_canonical, pair_dist_fp64, recall, search_canonical, search_fp64,
 )

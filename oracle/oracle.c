@@ -400,3 +400,149 @@ float oracle_e4m3_to_float(uint8_t b) {
   double v = e == 0 ? ldexp((double)m, -9) : ldexp(1.0 + m / 8.0, e - 7);
   return (float)(s ? -v : v);
 }
+
+/* ------------------------------------------------ random-projection LSH -- */
+/* Restatement of the library's LSH front end (IndexRPLSH). The reference
+ * has no LSH step to follow (its only piece is the Gaussian projection
+ * initialiser, src/internal/matrix_ops.cu:71-76); this is the specification
+ * the GPU path implements, written as plainly as possible. */
+
+void oracle_lsh_hash(const float* x, int64_t n, int d, const float* w, int L, int K,
+                     uint32_t* codes, float* margins) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const float* xr = x + i * (int64_t)d;
+    for (int t = 0; t < L; ++t) {
+      uint32_t c = 0;
+      for (int b = 0; b < K; ++b) {
+        const float s = oracle_dot_canonical(w + ((int64_t)t * K + b) * d, xr, d);
+        if (s >= 0.0f) c |= 1u << b;
+        if (margins) margins[i * (int64_t)L * K + (int64_t)t * K + b] = s;
+      }
+      codes[(int64_t)t * n + i] = c;
+    }
+  }
+}
+
+typedef struct {
+  uint32_t code;
+  int32_t row;
+} code_row;
+
+static int code_row_cmp(const void* a, const void* b) {
+  const code_row* x = (const code_row*)a;
+  const code_row* y = (const code_row*)b;
+  if (x->code != y->code) return x->code < y->code ? -1 : 1;
+  return (x->row > y->row) - (x->row < y->row);
+}
+
+/* Order of the probe bits: |dot| as its IEEE bit pattern (NaN after +inf),
+ * then the bit index. */
+static uint64_t probe_key(float m, int b) {
+  return ((uint64_t)f2u(fabsf(m)) << 5) | (uint64_t)b;
+}
+
+void oracle_lsh_search(const float* x, int64_t n, int d, const float* w, int L, int K, int probes,
+                       int budget, const float* q, int64_t nq, int k, int32_t* out_idx,
+                       float* out_dist, int64_t* raw_total, int64_t* distinct_total) {
+  if (raw_total) *raw_total = 0;
+  if (distinct_total) *distinct_total = 0;
+  if (nq <= 0 || k <= 0) return;
+  if (n <= 0) {
+    for (int64_t i = 0; i < nq * (int64_t)k; ++i) {
+      out_idx[i] = -1;
+      out_dist[i] = FLT_MAX;
+    }
+    return;
+  }
+  uint32_t* codes = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n * L);
+  oracle_lsh_hash(x, n, d, w, L, K, codes, NULL);
+  code_row* tab = (code_row*)malloc(sizeof(code_row) * (size_t)n * L);
+  for (int t = 0; t < L; ++t) {
+    for (int64_t i = 0; i < n; ++i) {
+      tab[(int64_t)t * n + i].code = codes[(int64_t)t * n + i];
+      tab[(int64_t)t * n + i].row = (int32_t)i;
+    }
+    qsort(tab + (int64_t)t * n, (size_t)n, sizeof(code_row), code_row_cmp);
+  }
+  float* xn = (float*)malloc(sizeof(float) * (size_t)n);
+  oracle_norms_canonical(x, n, d, xn);
+  float* qn = (float*)malloc(sizeof(float) * (size_t)nq);
+  oracle_norms_canonical(q, nq, d, qn);
+  uint32_t* qc = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)nq * L);
+  float* qm = (float*)malloc(sizeof(float) * (size_t)nq * L * K);
+  oracle_lsh_hash(q, nq, d, w, L, K, qc, qm);
+  int64_t raw = 0, distinct = 0;
+
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : raw, distinct)
+  for (int64_t qi = 0; qi < nq; ++qi) {
+    int32_t* buf = (int32_t*)malloc(sizeof(int32_t) * (size_t)(budget > 0 ? budget : 1));
+    int R = 0;
+    for (int p = 0; p <= probes && R < budget; ++p) {
+      for (int t = 0; t < L && R < budget; ++t) {
+        uint32_t code = qc[(int64_t)t * nq + qi];
+        if (p > 0) {
+          /* the bit of the p-th smallest key among the K bits */
+          const float* m = qm + qi * (int64_t)L * K + (int64_t)t * K;
+          uint64_t keys[32];
+          for (int b = 0; b < K; ++b) keys[b] = probe_key(m[b], b);
+          for (int a = 1; a < K; ++a) { /* insertion sort, ascending */
+            uint64_t v = keys[a];
+            int j = a - 1;
+            while (j >= 0 && keys[j] > v) {
+              keys[j + 1] = keys[j];
+              --j;
+            }
+            keys[j + 1] = v;
+          }
+          code ^= 1u << (int)(keys[p - 1] & 31u);
+        }
+        const code_row* s = tab + (int64_t)t * n;
+        int64_t lo = 0, hi = n;
+        while (lo < hi) { /* first entry with code >= probe */
+          int64_t mid = (lo + hi) / 2;
+          if (s[mid].code < code) lo = mid + 1; else hi = mid;
+        }
+        for (int64_t r = lo; r < n && s[r].code == code && R < budget; ++r) buf[R++] = s[r].row;
+      }
+    }
+    raw += R;
+    qsort(buf, (size_t)R, sizeof(int32_t), i32_cmp);
+    kv32* h = (kv32*)malloc(sizeof(kv32) * (size_t)k);
+    for (int j = 0; j < k; ++j) {
+      h[j].key = INFINITY;
+      h[j].idx = INT32_MAX;
+    }
+    int nd = 0;
+    const float* qv = q + qi * (int64_t)d;
+    for (int j = 0; j < R; ++j) {
+      if (j > 0 && buf[j] == buf[j - 1]) continue;
+      ++nd;
+      const int32_t r = buf[j];
+      const float rel = fmaf(-2.0f, oracle_dot_canonical(qv, x + (int64_t)r * d, d), xn[r]);
+      heap32_push(h, k, rel, r);
+    }
+    distinct += nd;
+    qsort(h, (size_t)k, sizeof(kv32), kv32_cmp);
+    for (int j = 0; j < k; ++j) {
+      const int64_t o = qi * (int64_t)k + j;
+      if (j < nd) {
+        out_idx[o] = h[j].idx;
+        out_dist[o] = h[j].key + qn[qi];
+      } else {
+        out_idx[o] = -1;
+        out_dist[o] = FLT_MAX;
+      }
+    }
+    free(h);
+    free(buf);
+  }
+  if (raw_total) *raw_total = raw;
+  if (distinct_total) *distinct_total = distinct;
+  free(codes);
+  free(tab);
+  free(xn);
+  free(qn);
+  free(qc);
+  free(qm);
+}

@@ -1,0 +1,131 @@
+"""IndexRPLSH (random-projection LSH front end + exact re-rank) on the GPU
+against the oracle's LSH restatement: projections, codes, dots and bucket
+tables bit-exact; search results (indices and distance bits) and candidate
+counts equal to the oracle's on the same seeded data, across table / bit /
+probe / budget shapes, ragged dimensions, small batches and padding."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def make(ca, res, d, **kw):
+    return ca.IndexRPLSH(res, d, ca.IndexRPLSHConfig(**kw))
+
+
+def test_projections_are_seeded_gaussians(ca, res):
+    idx = make(ca, res, 40, tables=3, bits=5, seed=77)
+    assert np.array_equal(idx.projections(), ca.gen_gaussian(15, 40, 77))
+
+
+def test_codes_tables_and_query_dots_bitexact(ca, orc, res):
+    n, d, L, K = 5000, 33, 4, 9
+    x = ca.gen_uniform(n, d, 1, -1, 1)
+    q = ca.gen_uniform(50, d, 2, -1, 1)
+    idx = make(ca, res, d, tables=L, bits=K, seed=5)
+    idx.add(x)
+    w = idx.projections()
+    codes, srt, perm = idx.tables()
+    c_ref, _ = orc.lsh_hash(x, w, L, K)
+    assert np.array_equal(codes, c_ref)
+    for t in range(L):
+        order = np.lexsort((np.arange(n), c_ref[t]))  # by code, then row
+        assert np.array_equal(perm[t], order)
+        assert np.array_equal(srt[t], c_ref[t][order])
+    qc, qm = idx.hash_queries(q)
+    qc_ref, qm_ref = orc.lsh_hash(q, w, L, K)
+    assert np.array_equal(qc, qc_ref)
+    assert np.array_equal(bits(qm), bits(qm_ref))
+
+
+CASES = [  # n, d, nq, k, tables, bits, probes, budget
+    (20000, 64, 200, 10, 8, 12, 0, 1024),
+    (20000, 64, 200, 10, 8, 12, 3, 1024),
+    (30000, 100, 97, 100, 16, 10, 2, 4096),
+    (5000, 33, 40, 10, 4, 6, 1, 64),      # budget cuts the bucket concatenation
+    (3000, 200, 20, 20, 6, 8, 4, 512),    # d > 128
+    (4000, 96, 7, 10, 8, 10, 2, 300),     # batch under 16 queries
+    (2000, 17, 30, 50, 2, 14, 0, 100),    # many queries with < k candidates: padding
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"n{c[0]}d{c[1]}q{c[2]}k{c[3]}L{c[4]}K{c[5]}P{c[6]}B{c[7]}"
+                                              for c in CASES])
+def test_search_bitexact_vs_oracle(ca, orc, res, case):
+    n, d, nq, k, L, K, P, B = case
+    x = ca.gen_mixture(n, d, 32, 0.2, 3, 4)
+    q = ca.gen_mixture(nq, d, 32, 0.2, 3, 5)
+    idx = make(ca, res, d, tables=L, bits=K, probes=P, max_candidates=B, seed=11)
+    idx.add(x)
+    I, D = idx.search(q, k)
+    Io, Do, raw, distinct = orc.lsh_search(x, idx.projections(), L, K, P, B, q, k)
+    bad = np.nonzero((I != Io).any(1) | (bits(D) != bits(Do)).any(1))[0]
+    assert bad.size == 0, f"{bad.size} rows differ, first {bad[:5]}"
+    st = idx.last_stats()
+    assert st["raw_candidates"] == raw and st["distinct_candidates"] == distinct
+    assert st["queries"] == nq
+
+
+def test_all_buckets_equals_plain_brute_force(ca, res):
+    x = ca.gen_uniform(3000, 48, 7, -1, 1)
+    q = ca.gen_uniform(64, 48, 8, -1, 1)
+    idx = make(ca, res, 48, tables=1, bits=1, probes=1, max_candidates=4096, seed=3)
+    idx.add(x)
+    I, D = idx.search(q, 25)
+    bf = ca.IndexBF(res, 48)
+    bf.add(x)
+    Ib, Db = bf.search(q, 25)
+    assert np.array_equal(I, Ib) and np.array_equal(bits(D), bits(Db))
+    assert idx.last_stats()["distinct_candidates"] == 64 * 3000
+
+
+def test_incremental_add_device_pointers_and_reset(ca, res):
+    d, k = 64, 10
+    x = ca.gen_mixture(12000, d, 16, 0.3, 1, 2)
+    q = ca.gen_mixture(100, d, 16, 0.3, 1, 3)
+    cfg = dict(tables=6, bits=10, probes=2, max_candidates=800, seed=9)
+    one = make(ca, res, d, **cfg)
+    one.add(x)
+    I1, D1 = one.search(q, k)
+    two = make(ca, res, d, **cfg)
+    two.add(x[:5000])
+    two.add(ca.DeviceArray.from_host(x[5000:]))
+    assert two.num_data == 12000
+    dq = ca.DeviceArray.from_host(q)
+    dI, dD = ca.DeviceArray((100, k), np.int32), ca.DeviceArray((100, k), np.float32)
+    two.search(dq, k, dI, dD)
+    assert np.array_equal(dI.to_host(), I1) and np.array_equal(bits(dD.to_host()), bits(D1))
+    two.reset()
+    assert two.num_data == 0
+    I0, D0 = two.search(q, k)
+    assert (I0 == -1).all() and (D0 == np.finfo(np.float32).max).all()
+    two.add(x)
+    I2, _ = two.search(q, k)
+    assert np.array_equal(I2, I1)
+
+
+def test_recall_on_clustered_data(ca, orc, res):
+    """Not a parity check: the front end must find most true neighbours on
+    clustered data (256 centres, sigma 0.1, d = 128), and more with probes."""
+    x = ca.gen_mixture(200_000, 128, 256, 0.1, 2, 3)
+    q = ca.gen_mixture(500, 128, 256, 0.1, 2, 30)
+    Io, _ = orc.search_canonical(x, q, 10)
+    rec = {}
+    for P in (0, 3):
+        idx = make(ca, res, 128, tables=16, bits=14, probes=P, max_candidates=2048, seed=1)
+        idx.add(x)
+        I, _ = idx.search(q, 10)
+        rec[P] = orc.recall(I, Io)
+        print("probes", P, "recall@10", rec[P], idx.last_stats())
+    assert rec[3] >= rec[0] and rec[3] >= 0.8
+
+
+def test_invalid_configs_rejected(ca, res):
+    for kw in (dict(bits=0), dict(bits=25), dict(probes=17, bits=16), dict(max_candidates=5000),
+               dict(tables=100, probes=10, bits=12), dict(tables=0)):
+        with pytest.raises(ca.RplshError):
+            make(ca, res, 16, **kw)

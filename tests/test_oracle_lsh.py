@@ -1,0 +1,141 @@
+"""Pins the LSH restatement in the oracle (oracle_lsh_hash / oracle_lsh_search)
+against formulations that do not share its code:
+
+* codes: signs of float64 projections (numpy matmul), on every bit whose
+  float64 dot is far from zero; dots within the FP32 forward-error bound;
+* probing every bucket (1 bit, 1 extra probe) with an unlimited budget must
+  reproduce the canonical brute-force search exactly (itself pinned to the
+  golden vectors);
+* unlimited budget: the candidate set is {rows sharing a probed bucket},
+  computed by set membership over numpy codes, and the result is the exact
+  search restricted to that set;
+* limited budget: the first B entries of the bucket concatenation, built in
+  numpy from the codes, then de-duplicated.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+
+def _data(n, nq, d, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    q = rng.standard_normal((nq, d)).astype(np.float32)
+    return x, q
+
+
+def _w(L, K, d, seed):
+    return np.random.default_rng(seed).standard_normal((L * K, d)).astype(np.float32)
+
+
+def _probe_codes(codes_q, marg_q, L, K, P):
+    """[nq][P+1][L] probe codes from numpy: bit order by (|dot|, bit)."""
+    nq = codes_q.shape[1]
+    out = np.empty((nq, P + 1, L), np.uint64)
+    for i in range(nq):
+        for t in range(L):
+            c = int(codes_q[t, i])
+            out[i, 0, t] = c
+            m = np.abs(marg_q[i, t * K:(t + 1) * K].astype(np.float64))
+            order = np.lexsort((np.arange(K), m))  # by |m|, then bit
+            for p in range(1, P + 1):
+                out[i, p, t] = c ^ (1 << int(order[p - 1]))
+    return out
+
+
+def test_codes_match_float64_signs():
+    L, K, d = 3, 7, 33
+    x, _ = _data(2000, 1, d, 1)
+    w = _w(L, K, d, 2)
+    codes, marg = orc.lsh_hash(x, w, L, K)
+    m64 = x.astype(np.float64) @ w.astype(np.float64).T  # [n][L*K]
+    scale = np.abs(x).astype(np.float64) @ np.abs(w).astype(np.float64).T
+    bound = (d + 2) * 2.0 ** -24 * scale
+    assert np.all(np.abs(marg - m64) <= bound)
+    for t in range(L):
+        for b in range(K):
+            col = t * K + b
+            sure = np.abs(m64[:, col]) > bound[:, col]
+            bit = (codes[t] >> np.uint32(b)) & np.uint32(1)
+            assert np.array_equal(bit[sure].astype(bool), m64[sure, col] >= 0), (t, b)
+
+
+def test_all_buckets_is_brute_force():
+    x, q = _data(700, 40, 24, 3)
+    w = _w(1, 1, 24, 4)
+    for k in (1, 10, 50):
+        I, D, raw, distinct = orc.lsh_search(x, w, 1, 1, 1, 4096, q, k)
+        Io, Do = orc.search_canonical(x, q, k)
+        assert np.array_equal(I, Io) and np.array_equal(D.view(np.uint32), Do.view(np.uint32))
+        assert raw == 40 * 700 and distinct == 40 * 700
+    # three tables, every bucket of each: each row appears three times before de-duplication
+    w3 = _w(3, 1, 24, 5)
+    I, D, raw, distinct = orc.lsh_search(x[:500], w3, 3, 1, 1, 4096, q, 10)
+    Io, Do = orc.search_canonical(x[:500], q, 10)
+    assert np.array_equal(I, Io) and np.array_equal(D.view(np.uint32), Do.view(np.uint32))
+    assert raw == 40 * 1500 and distinct == 40 * 500
+
+
+@pytest.mark.parametrize("L,K,P", [(4, 6, 0), (4, 6, 2), (8, 3, 3), (2, 10, 5)])
+def test_unlimited_budget_is_bucket_union(L, K, P):
+    x, q = _data(400, 30, 16, 10 + L + K + P)
+    w = _w(L, K, 16, 20 + P)
+    codes, _ = orc.lsh_hash(x, w, L, K)
+    cq, mq = orc.lsh_hash(q, w, L, K)
+    probes = _probe_codes(cq, mq, L, K, P)
+    k = 8
+    I, D, raw, distinct = orc.lsh_search(x, w, L, K, P, 4096, q, k)
+    tot_distinct = 0
+    for i in range(q.shape[0]):
+        member = np.zeros(x.shape[0], bool)
+        for t in range(L):
+            member |= np.isin(codes[t].astype(np.uint64), probes[i, :, t])
+        rows = np.nonzero(member)[0]
+        entries = sum(int(np.isin(codes[t].astype(np.uint64), probes[i, :, t]).sum()) for t in range(L))
+        assert entries <= 4096  # nothing was cut
+        tot_distinct += rows.size
+        if rows.size == 0:
+            assert (I[i] == -1).all()
+            continue
+        Io, Do = orc.search_canonical(x[rows], q[i:i + 1], min(k, rows.size))
+        m = min(k, rows.size)
+        assert np.array_equal(I[i, :m], rows[Io[0]]), i
+        assert np.array_equal(D[i, :m].view(np.uint32), Do[0].view(np.uint32)), i
+        assert (I[i, m:] == -1).all() and (D[i, m:] == np.finfo(np.float32).max).all()
+    assert distinct == tot_distinct
+
+
+@pytest.mark.parametrize("budget", [1, 7, 64, 300])
+def test_budget_cuts_the_concatenation(budget):
+    L, K, P = 5, 4, 2
+    x, q = _data(800, 12, 20, 30)
+    w = _w(L, K, 20, 31)
+    codes, _ = orc.lsh_hash(x, w, L, K)
+    cq, mq = orc.lsh_hash(q, w, L, K)
+    probes = _probe_codes(cq, mq, L, K, P)
+    k = 5
+    I, D, raw, distinct = orc.lsh_search(x, w, L, K, P, budget, q, k)
+    raw_ref = dist_ref = 0
+    for i in range(q.shape[0]):
+        seq = []
+        for p in range(P + 1):
+            for t in range(L):
+                seq.extend(np.nonzero(codes[t].astype(np.uint64) == probes[i, p, t])[0].tolist())
+        seq = seq[:budget]
+        raw_ref += len(seq)
+        rows = np.unique(np.array(seq, np.int64))
+        dist_ref += rows.size
+        m = min(k, rows.size)
+        if m:
+            Io, Do = orc.search_canonical(x[rows], q[i:i + 1], m)
+            assert np.array_equal(I[i, :m], rows[Io[0]]), i
+            assert np.array_equal(D[i, :m].view(np.uint32), Do[0].view(np.uint32)), i
+        assert (I[i, m:] == -1).all()
+    assert raw == raw_ref and distinct == dist_ref
+
+
+def test_empty_index_pads():
+    _, q = _data(1, 3, 8, 40)
+    I, D, raw, distinct = orc.lsh_search(np.zeros((0, 8), np.float32), _w(2, 3, 8, 41), 2, 3, 1, 10, q, 4)
+    assert (I == -1).all() and (D == np.finfo(np.float32).max).all() and raw == 0 and distinct == 0
