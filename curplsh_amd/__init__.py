@@ -200,7 +200,9 @@ class IndexRPLSHConfig:
     tables x bits: hash tables and code bits per table; probes: extra
     single-bit-flip buckets per table (query-directed); max_candidates: bucket
     entries taken per query before de-duplication (<= 4096); seed: projection
-    seed (standard normal, the library's generator)."""
+    seed (standard normal, the library's generator); center: hyperplane
+    offsets at the projections' medians over the first add's first 65536 rows
+    (L2 proximity) instead of through the origin (angle)."""
 
     device: int = 0
     tables: int = 16
@@ -208,6 +210,7 @@ class IndexRPLSHConfig:
     probes: int = 0
     max_candidates: int = 1024
     seed: int = 1234
+    center: bool = True
 
     def to_c(self) -> _lib.LshConfig:
         c = _lib.LshConfig()
@@ -277,6 +280,13 @@ class IndexRPLSH:
         call("rplsh_lsh_projections", self._h, w.ctypes.data)
         return w
 
+    def offsets(self) -> np.ndarray:
+        """Hyperplane offsets [tables*bits] (float32)."""
+        c = self.config
+        o = np.empty(c.tables * c.bits, np.float32)
+        call("rplsh_lsh_offsets", self._h, o.ctypes.data)
+        return o
+
     def tables(self):
         """(codes in row order, sorted codes, rows in bucket order), each [tables][n]."""
         n, L = self.num_data, self.config.tables
@@ -287,7 +297,7 @@ class IndexRPLSH:
         return codes, srt, perm
 
     def hash_queries(self, q):
-        """(codes [tables][nq] uint32, dots [nq][tables*bits] float32)."""
+        """(codes [tables][nq] uint32, dot - offset [nq][tables*bits] float32)."""
         q = np.ascontiguousarray(q, dtype=np.float32)
         nq, c = q.shape[0], self.config
         codes = np.empty((c.tables, nq), np.uint32)

@@ -147,6 +147,7 @@ class LshConfig(C.Structure):
         ("probes", C.c_int),
         ("max_candidates", C.c_int),
         ("seed", C.c_uint64),
+        ("center", C.c_int),
     ]
 
 
@@ -245,6 +246,7 @@ _SIGS = {
     "rplsh_lsh_get_config": [_vp, C.POINTER(LshConfig)],
     "rplsh_lsh_projections": [_vp, _vp],
     "rplsh_lsh_tables": [_vp, _vp, _vp, _vp],
+    "rplsh_lsh_offsets": [_vp, _vp],
     "rplsh_lsh_hash_queries": [_vp, _i64, _vp, _vp, _vp],
 }
 

@@ -8,14 +8,19 @@
 // Index (add):
 //   W [L*K][d]   standard normal projections from the library's seeded
 //                generator (row r = table r / K, bit r % K)
-//   code_t(x)    = sum_b [m_{t,b}(x) >= 0] << b,  m_{t,b}(x) = canonical fmaf
-//                chain over j of W[t*K+b][j] * x[j] (as the distance dot)
+//   code_t(x)    = sum_b [s_{t,b}(x) >= o_{t,b}] << b,  s_{t,b}(x) = canonical
+//                fmaf chain over j of W[t*K+b][j] * x[j] (as the distance dot)
+//   offsets o    0 (hyperplanes through the origin: angular hashing), or, with
+//                `center`, each projection's median over a sample of the first
+//                add (the first min(n, 65536) rows; element m/2 of the m sorted
+//                dots), so every bit halves the data and hyperplanes cut
+//                through clusters: collisions then follow L2 proximity
 //   table t      rows stably sorted by code_t (ties: ascending row), as
 //                sorted codes + row permutation, so a bucket is a range
 // Search, per query:
 //   probes       bucket code_t(q) and, for p = 1..P, code_t(q) with the bit of
-//                the p-th smallest |m_{t,b}(q)| flipped (ties: lower bit;
-//                query-directed multi-probe, Lv et al., VLDB 2007)
+//                the p-th smallest |s_{t,b}(q) - o_{t,b}| flipped (ties: lower
+//                bit; query-directed multi-probe, Lv et al., VLDB 2007)
 //   candidates   buckets concatenated in (probe, table) order, rows in bucket
 //                order, cut after B entries, then de-duplicated
 //   result       canonical FP32 re-rank of the distinct candidates (the
@@ -50,16 +55,22 @@ constexpr int kLshMaxBits = 24;
 constexpr int kLshMaxEntries = 1024;  // (probes + 1) * tables
 constexpr int kLshMaxBudget = 4096;   // the re-rank's longest list
 constexpr int kLshBlock = 256;
+constexpr int64_t kLshCenterSample = 65536;
 
 // ---------------------------------------------------------------- hashing
 
-// Codes (and, for queries, margins) of rows [0, n) for table blockIdx.y.
-// Rows are 16-byte aligned with ldx % 4 == 0 (zero padding past d is not
-// read). codes[t * ldc + i]; margins[i * L * K + t * K + b] when not null.
+// Codes (and margins) of rows [0, n) for table blockIdx.y. Rows are 16-byte
+// aligned with ldx % 4 == 0 (zero padding past d is not read). Bit b of
+// codes[t * ldc + i] is s >= off[t * K + b]; margins, when not null, hold
+// s - off at margins[i * mrow + (t * K + b) * mcol] (queries: mrow = L * K,
+// mcol = 1; the transposed sample of the offset computation: mrow = 1,
+// mcol = rows). codes may be null.
 __global__ __launch_bounds__(kLshBlock) void lsh_hash_kernel(const float* __restrict__ x, int64_t ldx,
                                                              int64_t n, int d, const float* __restrict__ w,
-                                                             int L, int K, uint32_t* __restrict__ codes,
-                                                             int64_t ldc, float* __restrict__ margins) {
+                                                             const float* __restrict__ off, int L, int K,
+                                                             uint32_t* __restrict__ codes, int64_t ldc,
+                                                             float* __restrict__ margins, int64_t mrow,
+                                                             int64_t mcol) {
   extern __shared__ float wt[];  // [K][d] of table t
   const int t = blockIdx.y;
   for (int e = threadIdx.x; e < K * d; e += blockDim.x) wt[e] = w[(int64_t)t * K * d + e];
@@ -83,17 +94,28 @@ __global__ __launch_bounds__(kLshBlock) void lsh_hash_kernel(const float* __rest
       }
     }
   }
+  const float* o = off + (int64_t)t * K;
   uint32_t code = 0;
 #pragma unroll
   for (int b = 0; b < kLshMaxBits; ++b)
-    if (b < K && acc[b] >= 0.0f) code |= 1u << b;
-  codes[(int64_t)t * ldc + i] = code;
+    if (b < K && acc[b] >= o[b]) code |= 1u << b;
+  if (codes) codes[(int64_t)t * ldc + i] = code;
   if (margins) {
-    float* mr = margins + i * (int64_t)L * K + (int64_t)t * K;
 #pragma unroll
     for (int b = 0; b < kLshMaxBits; ++b)
-      if (b < K) mr[b] = acc[b];
+      if (b < K) margins[i * mrow + ((int64_t)t * K + b) * mcol] = acc[b] - o[b];
   }
+}
+
+// off[r] = sorted[r * m + m / 2] for r < nr.
+__global__ void lsh_median_kernel(const float* __restrict__ sorted, int64_t m, int nr, float* __restrict__ off) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nr) off[r] = sorted[(int64_t)r * m + m / 2];
+}
+
+__global__ void lsh_segment_offsets_kernel(int* __restrict__ seg, int nr, int m) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r <= nr) seg[r] = r * m;
 }
 
 __global__ void iota_kernel(int32_t* p, int64_t n) {
@@ -265,10 +287,12 @@ struct rplsh_lsh {
   float* x = nullptr;         // [capacity][ldx], zero padded
   float* xn = nullptr;        // [capacity]
   float* w = nullptr;         // [L*K][dim]
+  float* off = nullptr;       // [L*K] hyperplane offsets (0, or medians set at the first add)
+  bool off_set = false;
   uint32_t* codes = nullptr;  // [L][capacity] codes in row order
   uint32_t* sorted = nullptr; // [L][capacity]
   int32_t* perm = nullptr;    // [L][capacity]
-  DevBuf iota, sort_tmp, qstage, qpad, qn, qcodes, qmarg, cand, idev, ddev, counters, status, relk, fp64;
+  DevBuf iota, sort_tmp, ctr_dots, ctr_sorted, ctr_seg, qstage, qpad, qn, qcodes, qmarg, cand, idev, ddev, counters, status, relk, fp64;
   rplsh_lsh_stats last{};
 };
 
@@ -290,13 +314,41 @@ void lsh_free(rplsh_lsh* h) {
   h->n = h->capacity = 0;
 }
 
+// Offsets back to zero (reset, create): with `center` the next add sets them.
+void clear_offsets(rplsh_lsh* h) {
+  RPLSH_HIP(hipMemset(h->off, 0, (size_t)h->cfg.tables * h->cfg.bits * sizeof(float)));
+  h->off_set = false;
+}
+
 void launch_hash(const rplsh_lsh* h, const float* rows, int64_t n, uint32_t* codes, int64_t ldc,
-                 float* margins, hipStream_t st) {
+                 float* margins, int64_t mrow, int64_t mcol, hipStream_t st) {
   if (n <= 0) return;
   const int L = h->cfg.tables, K = h->cfg.bits;
   const dim3 grid((unsigned)div_up(n, kLshBlock), (unsigned)L);
   const size_t lds = (size_t)K * h->dim * sizeof(float);
-  lsh_hash_kernel<<<grid, kLshBlock, lds, st>>>(rows, h->ldx, n, h->dim, h->w, L, K, codes, ldc, margins);
+  lsh_hash_kernel<<<grid, kLshBlock, lds, st>>>(rows, h->ldx, n, h->dim, h->w, h->off, L, K, codes, ldc,
+                                                margins, mrow, mcol);
+  RPLSH_HIP(hipGetLastError());
+}
+
+// With `center`: offsets = medians of the projections over the first
+// min(n, kLshCenterSample) rows (off must be zero when this runs).
+void set_offsets(rplsh_lsh* h, const float* rows, int64_t n, hipStream_t st) {
+  const int nr = h->cfg.tables * h->cfg.bits;
+  const int64_t m = std::min<int64_t>(n, kLshCenterSample);
+  float* dots = h->ctr_dots.as<float>((size_t)nr * m);
+  float* sorted = h->ctr_sorted.as<float>((size_t)nr * m);
+  int* seg = h->ctr_seg.as<int>((size_t)nr + 1);
+  launch_hash(h, rows, m, nullptr, 0, dots, 1, m, st);  // dots[r * m + i]
+  lsh_segment_offsets_kernel<<<(unsigned)div_up(nr + 1, 256), 256, 0, st>>>(seg, nr, (int)m);
+  RPLSH_HIP(hipGetLastError());
+  size_t tmp = 0;
+  RPLSH_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tmp, dots, sorted, (int)(nr * m), nr, seg,
+                                                       seg + 1, 0, 32, st));
+  void* tp = h->sort_tmp.ensure(tmp);
+  RPLSH_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(tp, tmp, dots, sorted, (int)(nr * m), nr, seg, seg + 1,
+                                                       0, 32, st));
+  lsh_median_kernel<<<(unsigned)div_up(nr, 256), 256, 0, st>>>(sorted, m, nr, h->off);
   RPLSH_HIP(hipGetLastError());
 }
 
@@ -344,6 +396,7 @@ void rplsh_lsh_config_default(rplsh_lsh_config* cfg) {
   cfg->probes = 0;
   cfg->max_candidates = 1024;
   cfg->seed = 1234;
+  cfg->center = 1;
 }
 
 rplsh_status rplsh_lsh_create(rplsh_resources* res, int dim, const rplsh_lsh_config* cfg, rplsh_lsh** out) {
@@ -383,8 +436,11 @@ rplsh_status rplsh_lsh_create(rplsh_resources* res, int dim, const rplsh_lsh_con
     try {
       RPLSH_HIP(hipMalloc(&h->w, wh.size() * sizeof(float)));
       RPLSH_HIP(hipMemcpy(h->w, wh.data(), wh.size() * sizeof(float), hipMemcpyHostToDevice));
+      RPLSH_HIP(hipMalloc(&h->off, (size_t)rows * sizeof(float)));
+      clear_offsets(h);
     } catch (...) {
       if (h->w) (void)hipFree(h->w);
+      if (h->off) (void)hipFree(h->off);
       delete h;
       throw;
     }
@@ -399,6 +455,7 @@ rplsh_status rplsh_lsh_destroy(rplsh_lsh* h) {
     (void)hipDeviceSynchronize();
     lsh_free(h);
     if (h->w) RPLSH_HIP(hipFree(h->w));
+    if (h->off) RPLSH_HIP(hipFree(h->off));
     delete h;
   });
 }
@@ -409,6 +466,7 @@ rplsh_status rplsh_lsh_reset(rplsh_lsh* h) {
     DeviceScope scope(h->cfg.device);
     RPLSH_HIP(hipStreamSynchronize(lsh_stream(h)));
     lsh_free(h);
+    clear_offsets(h);
   });
 }
 
@@ -458,7 +516,9 @@ rplsh_status rplsh_lsh_add(rplsh_lsh* h, int64_t num, const float* data) {
                                  h->dim * sizeof(float), num, hipMemcpyDefault, st));
     }
     launch_l2norm(dst, num, h->dim, h->ldx, h->xn + h->n, 1, st);
-    launch_hash(h, dst, num, h->codes + h->n, h->capacity, nullptr, st);
+    if (h->cfg.center && !h->off_set) set_offsets(h, dst, num, st);
+    h->off_set = true;
+    launch_hash(h, dst, num, h->codes + h->n, h->capacity, nullptr, 0, 0, st);
     h->n = need;
     build_tables(h, st);
     RPLSH_HIP(hipStreamSynchronize(st));
@@ -502,7 +562,7 @@ rplsh_status rplsh_lsh_search(rplsh_lsh* h, int64_t nq, const float* queries, in
       launch_l2norm(qp, nq, h->dim, h->ldx, qn, 1, st);
       uint32_t* qc = h->qcodes.as<uint32_t>((size_t)nq * L);
       float* qm = h->qmarg.as<float>((size_t)nq * L * K);
-      launch_hash(h, qp, nq, qc, nq, qm, st);
+      launch_hash(h, qp, nq, qc, nq, qm, (int64_t)L * K, 1, st);
       // candidates
       uint64_t* cand = h->cand.as<uint64_t>((size_t)nq * B);
       LshGatherArgs ga{};
@@ -602,6 +662,14 @@ rplsh_status rplsh_lsh_projections(const rplsh_lsh* h, float* out) {
   });
 }
 
+rplsh_status rplsh_lsh_offsets(const rplsh_lsh* h, float* out) {
+  return guard([&] {
+    RPLSH_REQUIRE(h && out, "NULL argument");
+    DeviceScope scope(h->cfg.device);
+    RPLSH_HIP(hipMemcpy(out, h->off, (size_t)h->cfg.tables * h->cfg.bits * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
 rplsh_status rplsh_lsh_tables(const rplsh_lsh* h, uint32_t* codes, uint32_t* sorted, int32_t* perm) {
   return guard([&] {
     RPLSH_REQUIRE(h, "index is NULL");
@@ -627,7 +695,7 @@ rplsh_status rplsh_lsh_hash_queries(rplsh_lsh* h, int64_t nq, const float* queri
     float* qp = stage_queries(h, nq, queries, st);
     uint32_t* qc = h->qcodes.as<uint32_t>((size_t)nq * L);
     float* qm = h->qmarg.as<float>((size_t)nq * L * K);
-    launch_hash(h, qp, nq, qc, nq, qm, st);
+    launch_hash(h, qp, nq, qc, nq, qm, (int64_t)L * K, 1, st);
     RPLSH_HIP(hipMemcpyAsync(codes, qc, (size_t)nq * L * sizeof(uint32_t), hipMemcpyDefault, st));
     if (margins)
       RPLSH_HIP(hipMemcpyAsync(margins, qm, (size_t)nq * L * K * sizeof(float), hipMemcpyDefault, st));

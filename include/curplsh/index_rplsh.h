@@ -8,7 +8,8 @@
 // shape, train / add / search / reset, host or device pointers.
 //   * projections: tables x bits standard-normal rows from the seed;
 //   * a row's code in a table: one bit per projection, set iff the canonical
-//     FP32 dot with the row is >= 0;
+//     FP32 dot with the row is >= the projection's offset (its median over
+//     the first add, or 0 with center = false);
 //   * a query probes its own bucket in every table and, per extra probe, the
 //     bucket with its least-certain bit flipped; the first maxCandidates
 //     bucket entries (in probe, table order) are de-duplicated and re-ranked
@@ -23,7 +24,8 @@
 namespace curplsh {
 
 struct IndexRPLSHConfig : public IndexConfig {
-  IndexRPLSHConfig() : numTables(16), numBits(16), numProbes(0), maxCandidates(1024), seed(1234) {}
+  IndexRPLSHConfig()
+      : numTables(16), numBits(16), numProbes(0), maxCandidates(1024), seed(1234), center(true) {}
   /// Hash tables.
   int numTables;
   /// Code bits per table (1..24).
@@ -34,6 +36,9 @@ struct IndexRPLSHConfig : public IndexConfig {
   int maxCandidates;
   /// Seed of the projection matrix.
   unsigned long long seed;
+  /// Hyperplanes through the projections' medians over the first add (L2
+  /// proximity) rather than through the origin (angle).
+  bool center;
 };
 
 class IndexRPLSH : public Index {

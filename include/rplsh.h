@@ -261,12 +261,15 @@ rplsh_status rplsh_bf_clear_tuning(rplsh_bf* index);
  *  - projections W [tables*bits][dim]: standard normal, from the library's
  *    seeded generator (rplsh_gen_gaussian(W, tables*bits, dim, seed, 0));
  *  - code of x in table t: bit b set iff the canonical FP32 dot (fmaf chain
- *    over j, as the distance dot) of W[t*bits+b] and x is >= 0;
+ *    over j, as the distance dot) of W[t*bits+b] and x is >= the offset
+ *    o[t*bits+b]: 0 (hyperplanes through the origin), or with `center` the
+ *    projection's median over the first min(n, 65536) rows of the first add
+ *    (element m/2 of the m sorted dots), set again after a reset;
  *  - per table, rows stably sorted by code (a bucket is a range, rows
  *    ascending within it);
  *  - a query probes, in each table, its own bucket and then, for p = 1..probes,
- *    the bucket with the bit of its p-th smallest |dot| flipped (ties: lower
- *    bit); buckets are concatenated in (probe, table) order, cut after
+ *    the bucket with the bit of its p-th smallest |dot - offset| flipped
+ *    (ties: lower bit); buckets are concatenated in (probe, table) order, cut after
  *    max_candidates entries and de-duplicated;
  *  - the distinct candidates are re-ranked exactly (canonical FP32, as the
  *    plain path): k smallest by (distance, index), padded with -1 / FLT_MAX. */
@@ -278,10 +281,11 @@ typedef struct rplsh_lsh_config {
                           (probes + 1) * tables <= 1024 */
   int max_candidates;  /* bucket entries taken per query before de-duplication, 1..4096 */
   uint64_t seed;       /* projection seed */
+  int center;          /* 1: median offsets (L2 proximity), 0: through the origin (angle) */
 } rplsh_lsh_config;
 
 /* Fills the defaults: device 0, 16 tables x 16 bits, no extra probes,
- * 1024 candidates, seed 1234. */
+ * 1024 candidates, seed 1234, median offsets. */
 void rplsh_lsh_config_default(rplsh_lsh_config* cfg);
 
 typedef struct rplsh_lsh rplsh_lsh;
@@ -310,12 +314,14 @@ rplsh_status rplsh_lsh_num_data(const rplsh_lsh* index, int64_t* n);
 rplsh_status rplsh_lsh_get_config(const rplsh_lsh* index, rplsh_lsh_config* out);
 /* W [tables*bits][dim] into host memory. */
 rplsh_status rplsh_lsh_projections(const rplsh_lsh* index, float* out);
+/* Offsets o [tables*bits] into host memory. */
+rplsh_status rplsh_lsh_offsets(const rplsh_lsh* index, float* out);
 /* Host copies [tables][n] of the codes in row order, the sorted codes and the
  * rows in bucket order; any of the three may be NULL. */
 rplsh_status rplsh_lsh_tables(const rplsh_lsh* index, uint32_t* codes, uint32_t* sorted,
                               int32_t* perm);
-/* Codes [tables][nq] and (if not NULL) dots [nq][tables*bits] of queries
- * (host or device; outputs host or device). */
+/* Codes [tables][nq] and (if not NULL) margins dot - offset
+ * [nq][tables*bits] of queries (host or device; outputs host or device). */
 rplsh_status rplsh_lsh_hash_queries(rplsh_lsh* index, int64_t nq, const float* queries,
                                     uint32_t* codes, float* margins);
 

@@ -407,8 +407,8 @@ float oracle_e4m3_to_float(uint8_t b) {
  * initialiser, src/internal/matrix_ops.cu:71-76); this is the specification
  * the GPU path implements, written as plainly as possible. */
 
-void oracle_lsh_hash(const float* x, int64_t n, int d, const float* w, int L, int K,
-                     uint32_t* codes, float* margins) {
+void oracle_lsh_hash(const float* x, int64_t n, int d, const float* w, const float* off, int L,
+                     int K, uint32_t* codes, float* margins) {
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i) {
     const float* xr = x + i * (int64_t)d;
@@ -416,8 +416,9 @@ void oracle_lsh_hash(const float* x, int64_t n, int d, const float* w, int L, in
       uint32_t c = 0;
       for (int b = 0; b < K; ++b) {
         const float s = oracle_dot_canonical(w + ((int64_t)t * K + b) * d, xr, d);
-        if (s >= 0.0f) c |= 1u << b;
-        if (margins) margins[i * (int64_t)L * K + (int64_t)t * K + b] = s;
+        const float o = off ? off[(int64_t)t * K + b] : 0.0f;
+        if (s >= o) c |= 1u << b;
+        if (margins) margins[i * (int64_t)L * K + (int64_t)t * K + b] = s - o;
       }
       codes[(int64_t)t * n + i] = c;
     }
@@ -442,9 +443,10 @@ static uint64_t probe_key(float m, int b) {
   return ((uint64_t)f2u(fabsf(m)) << 5) | (uint64_t)b;
 }
 
-void oracle_lsh_search(const float* x, int64_t n, int d, const float* w, int L, int K, int probes,
-                       int budget, const float* q, int64_t nq, int k, int32_t* out_idx,
-                       float* out_dist, int64_t* raw_total, int64_t* distinct_total) {
+void oracle_lsh_search(const float* x, int64_t n, int d, const float* w, const float* off, int L,
+                       int K, int probes, int budget, const float* q, int64_t nq, int k,
+                       int32_t* out_idx, float* out_dist, int64_t* raw_total,
+                       int64_t* distinct_total) {
   if (raw_total) *raw_total = 0;
   if (distinct_total) *distinct_total = 0;
   if (nq <= 0 || k <= 0) return;
@@ -456,7 +458,7 @@ void oracle_lsh_search(const float* x, int64_t n, int d, const float* w, int L, 
     return;
   }
   uint32_t* codes = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n * L);
-  oracle_lsh_hash(x, n, d, w, L, K, codes, NULL);
+  oracle_lsh_hash(x, n, d, w, off, L, K, codes, NULL);
   code_row* tab = (code_row*)malloc(sizeof(code_row) * (size_t)n * L);
   for (int t = 0; t < L; ++t) {
     for (int64_t i = 0; i < n; ++i) {
@@ -471,7 +473,7 @@ void oracle_lsh_search(const float* x, int64_t n, int d, const float* w, int L, 
   oracle_norms_canonical(q, nq, d, qn);
   uint32_t* qc = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)nq * L);
   float* qm = (float*)malloc(sizeof(float) * (size_t)nq * L * K);
-  oracle_lsh_hash(q, nq, d, w, L, K, qc, qm);
+  oracle_lsh_hash(q, nq, d, w, off, L, K, qc, qm);
   int64_t raw = 0, distinct = 0;
 
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : raw, distinct)

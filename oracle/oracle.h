@@ -74,21 +74,23 @@ float oracle_e4m3_to_float(uint8_t b);
 
 /* Random-projection LSH front end (the library's IndexRPLSH; the reference
  * has no LSH step, only the Gaussian initialiser of matrix_ops.cu:71-76).
- * Codes: bit b of table t set iff oracle_dot_canonical(W[t*K+b], x) >= 0;
- * codes[t*n + i], margins[i*L*K + t*K + b] (the dots) when not NULL. */
-void oracle_lsh_hash(const float* x, int64_t n, int d, const float* w, int L, int K,
-                     uint32_t* codes, float* margins);
+ * Codes: bit b of table t set iff oracle_dot_canonical(W[t*K+b], x) >=
+ * off[t*K+b] (off NULL: 0); codes[t*n + i], margins[i*L*K + t*K + b] =
+ * dot - off when not NULL. */
+void oracle_lsh_hash(const float* x, int64_t n, int d, const float* w, const float* off, int L,
+                     int K, uint32_t* codes, float* margins);
 
 /* LSH search: tables = rows stably sorted by code; per query, buckets of
  * (probe p = 0..probes, table t) in that order -- p = 0 the query's code,
- * p >= 1 the code with the bit of the p-th smallest (|dot| bits, bit) key
- * flipped -- concatenated and cut after `budget` entries, de-duplicated,
+ * p >= 1 the code with the bit of the p-th smallest (|dot - off| bits, bit)
+ * key flipped -- concatenated and cut after `budget` entries, de-duplicated,
  * then rel = fmaf(-2, dot, ||x||^2) over the distinct rows, k smallest by
  * (rel, index), D = rel + ||q||^2, padded with -1 / FLT_MAX. raw_total /
  * distinct_total (may be NULL): entries taken and distinct rows, summed. */
-void oracle_lsh_search(const float* x, int64_t n, int d, const float* w, int L, int K, int probes,
-                       int budget, const float* q, int64_t nq, int k, int32_t* out_idx,
-                       float* out_dist, int64_t* raw_total, int64_t* distinct_total);
+void oracle_lsh_search(const float* x, int64_t n, int d, const float* w, const float* off, int L,
+                       int K, int probes, int budget, const float* q, int64_t nq, int k,
+                       int32_t* out_idx, float* out_dist, int64_t* raw_total,
+                       int64_t* distinct_total);
 
 #ifdef __cplusplus
 }

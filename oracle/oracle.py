@@ -45,10 +45,10 @@ def load() -> C.CDLL:
     lib.oracle_e4m3_rne_sat.restype = C.c_uint8
     lib.oracle_e4m3_to_float.argtypes = [C.c_uint8]
     lib.oracle_e4m3_to_float.restype = C.c_float
-    lib.oracle_lsh_hash.argtypes = [_vp, _i64, _i32, _vp, _i32, _i32, _vp, _vp]
+    lib.oracle_lsh_hash.argtypes = [_vp, _i64, _i32, _vp, _vp, _i32, _i32, _vp, _vp]
     lib.oracle_lsh_hash.restype = None
-    lib.oracle_lsh_search.argtypes = [_vp, _i64, _i32, _vp, _i32, _i32, _i32, _i32, _vp, _i64,
-                                      _i32, _vp, _vp, _vp, _vp]
+    lib.oracle_lsh_search.argtypes = [_vp, _i64, _i32, _vp, _vp, _i32, _i32, _i32, _i32, _vp,
+                                      _i64, _i32, _vp, _vp, _vp, _vp]
     lib.oracle_lsh_search.restype = None
     _lib = lib
     return lib
@@ -125,29 +125,34 @@ def e4m3_to_float(b: int) -> float:
     return float(load().oracle_e4m3_to_float(int(b)))
 
 
-def lsh_hash(x: np.ndarray, w: np.ndarray, L: int, K: int):
-    """LSH codes [L][n] (uint32) and dots [n][L*K] of the rows of x."""
-    x, w = _f32(x), _f32(w)
+def _off(off):
+    return None if off is None else _f32(off)
+
+
+def lsh_hash(x: np.ndarray, w: np.ndarray, L: int, K: int, off: np.ndarray | None = None):
+    """LSH codes [L][n] (uint32) and margins dot - off [n][L*K] of the rows of x."""
+    x, w, o = _f32(x), _f32(w), _off(off)
     n, d = x.shape
     codes = np.empty((L, n), np.uint32)
     marg = np.empty((n, L * K), np.float32)
-    load().oracle_lsh_hash(x.ctypes.data, n, d, w.ctypes.data, L, K, codes.ctypes.data,
-                           marg.ctypes.data)
+    load().oracle_lsh_hash(x.ctypes.data, n, d, w.ctypes.data, o.ctypes.data if o is not None else None,
+                           L, K, codes.ctypes.data, marg.ctypes.data)
     return codes, marg
 
 
 def lsh_search(x: np.ndarray, w: np.ndarray, L: int, K: int, probes: int, budget: int,
-               q: np.ndarray, k: int):
+               q: np.ndarray, k: int, off: np.ndarray | None = None):
     """LSH search restatement: (indices, distances, raw entries, distinct rows)."""
     q = _f32(q)
     d = q.shape[1]
     x = _f32(x) if x is not None and x.size else np.zeros((0, d), np.float32)
-    w = _f32(w)
+    w, o = _f32(w), _off(off)
     nq = q.shape[0]
     I = np.empty((nq, k), np.int32)
     D = np.empty((nq, k), np.float32)
     raw, dist = C.c_int64(0), C.c_int64(0)
-    load().oracle_lsh_search(x.ctypes.data, x.shape[0], d, w.ctypes.data, L, K, probes, budget,
+    load().oracle_lsh_search(x.ctypes.data, x.shape[0], d, w.ctypes.data,
+                             o.ctypes.data if o is not None else None, L, K, probes, budget,
                              q.ctypes.data, nq, k, I.ctypes.data, D.ctypes.data, C.byref(raw),
                              C.byref(dist))
     return I, D, raw.value, dist.value

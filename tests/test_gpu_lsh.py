@@ -1,6 +1,6 @@
 """IndexRPLSH (random-projection LSH front end + exact re-rank) on the GPU
-against the oracle's LSH restatement: projections, codes, dots and bucket
-tables bit-exact; search results (indices and distance bits) and candidate
+against the oracle's LSH restatement: projections, median offsets, codes,
+margins and bucket tables bit-exact; search results (indices and distance bits) and candidate
 counts equal to the oracle's on the same seeded data, across table / bit /
 probe / budget shapes, ragged dimensions, small batches and padding."""
 import numpy as np
@@ -22,47 +22,60 @@ def test_projections_are_seeded_gaussians(ca, res):
     assert np.array_equal(idx.projections(), ca.gen_gaussian(15, 40, 77))
 
 
-def test_codes_tables_and_query_dots_bitexact(ca, orc, res):
+@pytest.mark.parametrize("center", [True, False])
+def test_codes_tables_and_query_dots_bitexact(ca, orc, res, center):
     n, d, L, K = 5000, 33, 4, 9
-    x = ca.gen_uniform(n, d, 1, -1, 1)
-    q = ca.gen_uniform(50, d, 2, -1, 1)
-    idx = make(ca, res, d, tables=L, bits=K, seed=5)
+    x = ca.gen_uniform(n, d, 1, -1, 1) + np.float32(0.5)
+    q = ca.gen_uniform(50, d, 2, -1, 1) + np.float32(0.5)
+    idx = make(ca, res, d, tables=L, bits=K, seed=5, center=center)
     idx.add(x)
     w = idx.projections()
+    off = idx.offsets()
+    if center:  # element n // 2 of each projection's sorted canonical dots
+        _, dots = orc.lsh_hash(x, w, L, K)
+        assert np.array_equal(off, np.sort(dots, axis=0)[n // 2])
+    else:
+        assert (off == 0).all()
     codes, srt, perm = idx.tables()
-    c_ref, _ = orc.lsh_hash(x, w, L, K)
+    c_ref, _ = orc.lsh_hash(x, w, L, K, off)
     assert np.array_equal(codes, c_ref)
     for t in range(L):
         order = np.lexsort((np.arange(n), c_ref[t]))  # by code, then row
         assert np.array_equal(perm[t], order)
         assert np.array_equal(srt[t], c_ref[t][order])
     qc, qm = idx.hash_queries(q)
-    qc_ref, qm_ref = orc.lsh_hash(q, w, L, K)
+    qc_ref, qm_ref = orc.lsh_hash(q, w, L, K, off)
     assert np.array_equal(qc, qc_ref)
     assert np.array_equal(bits(qm), bits(qm_ref))
 
 
-CASES = [  # n, d, nq, k, tables, bits, probes, budget
-    (20000, 64, 200, 10, 8, 12, 0, 1024),
-    (20000, 64, 200, 10, 8, 12, 3, 1024),
-    (30000, 100, 97, 100, 16, 10, 2, 4096),
-    (5000, 33, 40, 10, 4, 6, 1, 64),      # budget cuts the bucket concatenation
-    (3000, 200, 20, 20, 6, 8, 4, 512),    # d > 128
-    (4000, 96, 7, 10, 8, 10, 2, 300),     # batch under 16 queries
-    (2000, 17, 30, 50, 2, 14, 0, 100),    # many queries with < k candidates: padding
+CASES = [  # n, d, nq, k, tables, bits, probes, budget, center
+    (20000, 64, 200, 10, 8, 12, 0, 1024, 1),
+    (20000, 64, 200, 10, 8, 12, 3, 1024, 1),
+    (20000, 64, 200, 10, 8, 12, 3, 1024, 0),
+    (30000, 100, 97, 100, 16, 10, 2, 4096, 1),
+    (5000, 33, 40, 10, 4, 6, 1, 64, 1),      # budget cuts the bucket concatenation
+    (3000, 200, 20, 20, 6, 8, 4, 512, 1),    # d > 128
+    (4000, 96, 7, 10, 8, 10, 2, 300, 1),     # batch under 16 queries
+    (2000, 17, 30, 50, 2, 14, 0, 100, 1),    # many queries with < k candidates: padding
+    (70000, 32, 64, 10, 4, 20, 2, 512, 1),   # median over a 65536-row sample
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=[f"n{c[0]}d{c[1]}q{c[2]}k{c[3]}L{c[4]}K{c[5]}P{c[6]}B{c[7]}"
-                                              for c in CASES])
+                                              f"c{c[8]}" for c in CASES])
 def test_search_bitexact_vs_oracle(ca, orc, res, case):
-    n, d, nq, k, L, K, P, B = case
+    n, d, nq, k, L, K, P, B, center = case
     x = ca.gen_mixture(n, d, 32, 0.2, 3, 4)
     q = ca.gen_mixture(nq, d, 32, 0.2, 3, 5)
-    idx = make(ca, res, d, tables=L, bits=K, probes=P, max_candidates=B, seed=11)
+    idx = make(ca, res, d, tables=L, bits=K, probes=P, max_candidates=B, seed=11, center=bool(center))
     idx.add(x)
+    off = idx.offsets()
+    if center and n > 65536:
+        _, dots = orc.lsh_hash(x[:65536], idx.projections(), L, K)
+        assert np.array_equal(off, np.sort(dots, axis=0)[65536 // 2])
     I, D = idx.search(q, k)
-    Io, Do, raw, distinct = orc.lsh_search(x, idx.projections(), L, K, P, B, q, k)
+    Io, Do, raw, distinct = orc.lsh_search(x, idx.projections(), L, K, P, B, q, k, off)
     bad = np.nonzero((I != Io).any(1) | (bits(D) != bits(Do)).any(1))[0]
     assert bad.size == 0, f"{bad.size} rows differ, first {bad[:5]}"
     st = idx.last_stats()
@@ -88,10 +101,10 @@ def test_incremental_add_device_pointers_and_reset(ca, res):
     x = ca.gen_mixture(12000, d, 16, 0.3, 1, 2)
     q = ca.gen_mixture(100, d, 16, 0.3, 1, 3)
     cfg = dict(tables=6, bits=10, probes=2, max_candidates=800, seed=9)
-    one = make(ca, res, d, **cfg)
+    one = make(ca, res, d, center=False, **cfg)
     one.add(x)
     I1, D1 = one.search(q, k)
-    two = make(ca, res, d, **cfg)
+    two = make(ca, res, d, center=False, **cfg)
     two.add(x[:5000])
     two.add(ca.DeviceArray.from_host(x[5000:]))
     assert two.num_data == 12000
@@ -106,6 +119,16 @@ def test_incremental_add_device_pointers_and_reset(ca, res):
     two.add(x)
     I2, _ = two.search(q, k)
     assert np.array_equal(I2, I1)
+    # centred: offsets come from the first add (after a reset, the next one)
+    c1 = make(ca, res, d, center=True, **cfg)
+    c1.add(x[:5000])
+    off = c1.offsets()
+    c1.add(x[5000:])
+    assert np.array_equal(c1.offsets(), off)
+    c1.reset()
+    assert (c1.offsets() == 0).all()
+    c1.add(x[5000:])
+    assert not np.array_equal(c1.offsets(), off)
 
 
 def test_recall_on_clustered_data(ca, orc, res):

@@ -1,8 +1,9 @@
 """Pins the LSH restatement in the oracle (oracle_lsh_hash / oracle_lsh_search)
 against formulations that do not share its code:
 
-* codes: signs of float64 projections (numpy matmul), on every bit whose
-  float64 dot is far from zero; dots within the FP32 forward-error bound;
+* codes: signs of float64 projections minus the offsets (numpy matmul), on
+  every bit whose float64 value is far from its offset; margins within the
+  FP32 forward-error bound;
 * probing every bucket (1 bit, 1 extra probe) with an unlimited budget must
   reproduce the canonical brute-force search exactly (itself pinned to the
   golden vectors);
@@ -44,14 +45,17 @@ def _probe_codes(codes_q, marg_q, L, K, P):
     return out
 
 
-def test_codes_match_float64_signs():
+@pytest.mark.parametrize("centered", [False, True])
+def test_codes_match_float64_signs(centered):
     L, K, d = 3, 7, 33
     x, _ = _data(2000, 1, d, 1)
     w = _w(L, K, d, 2)
-    codes, marg = orc.lsh_hash(x, w, L, K)
-    m64 = x.astype(np.float64) @ w.astype(np.float64).T  # [n][L*K]
-    scale = np.abs(x).astype(np.float64) @ np.abs(w).astype(np.float64).T
-    bound = (d + 2) * 2.0 ** -24 * scale
+    off = np.random.default_rng(3).standard_normal(L * K).astype(np.float32) if centered else None
+    codes, marg = orc.lsh_hash(x, w, L, K, off)
+    o64 = np.zeros(L * K) if off is None else off.astype(np.float64)
+    m64 = x.astype(np.float64) @ w.astype(np.float64).T - o64  # [n][L*K]
+    scale = np.abs(x).astype(np.float64) @ np.abs(w).astype(np.float64).T + np.abs(o64)
+    bound = (d + 3) * 2.0 ** -24 * scale
     assert np.all(np.abs(marg - m64) <= bound)
     for t in range(L):
         for b in range(K):
@@ -77,15 +81,25 @@ def test_all_buckets_is_brute_force():
     assert raw == 40 * 1500 and distinct == 40 * 500
 
 
-@pytest.mark.parametrize("L,K,P", [(4, 6, 0), (4, 6, 2), (8, 3, 3), (2, 10, 5)])
-def test_unlimited_budget_is_bucket_union(L, K, P):
+def _medians(x, w, L, K):
+    """Offsets at the projections' medians over x (float64 ranks of the FP32 dots)."""
+    _, m = orc.lsh_hash(x, w, L, K)
+    return np.sort(m, axis=0)[x.shape[0] // 2].astype(np.float32)
+
+
+@pytest.mark.parametrize("L,K,P,centered", [(4, 6, 0, False), (4, 6, 2, False), (8, 3, 3, False),
+                                            (2, 10, 5, False), (4, 6, 2, True), (2, 9, 4, True)])
+def test_unlimited_budget_is_bucket_union(L, K, P, centered):
     x, q = _data(400, 30, 16, 10 + L + K + P)
+    x = x + np.float32(2.0)  # away from the origin
+    q = q + np.float32(2.0)
     w = _w(L, K, 16, 20 + P)
-    codes, _ = orc.lsh_hash(x, w, L, K)
-    cq, mq = orc.lsh_hash(q, w, L, K)
+    off = _medians(x, w, L, K) if centered else None
+    codes, _ = orc.lsh_hash(x, w, L, K, off)
+    cq, mq = orc.lsh_hash(q, w, L, K, off)
     probes = _probe_codes(cq, mq, L, K, P)
     k = 8
-    I, D, raw, distinct = orc.lsh_search(x, w, L, K, P, 4096, q, k)
+    I, D, raw, distinct = orc.lsh_search(x, w, L, K, P, 4096, q, k, off)
     tot_distinct = 0
     for i in range(q.shape[0]):
         member = np.zeros(x.shape[0], bool)
@@ -111,11 +125,12 @@ def test_budget_cuts_the_concatenation(budget):
     L, K, P = 5, 4, 2
     x, q = _data(800, 12, 20, 30)
     w = _w(L, K, 20, 31)
-    codes, _ = orc.lsh_hash(x, w, L, K)
-    cq, mq = orc.lsh_hash(q, w, L, K)
+    off = _medians(x, w, L, K)
+    codes, _ = orc.lsh_hash(x, w, L, K, off)
+    cq, mq = orc.lsh_hash(q, w, L, K, off)
     probes = _probe_codes(cq, mq, L, K, P)
     k = 5
-    I, D, raw, distinct = orc.lsh_search(x, w, L, K, P, budget, q, k)
+    I, D, raw, distinct = orc.lsh_search(x, w, L, K, P, budget, q, k, off)
     raw_ref = dist_ref = 0
     for i in range(q.shape[0]):
         seq = []
@@ -133,6 +148,22 @@ def test_budget_cuts_the_concatenation(budget):
             assert np.array_equal(D[i, :m].view(np.uint32), Do[0].view(np.uint32)), i
         assert (I[i, m:] == -1).all()
     assert raw == raw_ref and distinct == dist_ref
+
+
+def test_median_offsets_balance_every_bit():
+    """With median offsets each bit splits the rows in half (ties aside)."""
+    L, K = 4, 8
+    x, _ = _data(1001, 1, 12, 50)
+    x = x * np.float32(0.1) + np.float32(3.0)  # a tight cluster far from the origin
+    w = _w(L, K, 12, 51)
+    c0, _ = orc.lsh_hash(x, w, L, K)
+    c1, _ = orc.lsh_hash(x, w, L, K, _medians(x, w, L, K))
+    for t in range(L):
+        for b in range(K):
+            ones = int(((c1[t] >> np.uint32(b)) & np.uint32(1)).sum())
+            assert ones == 1001 - 1001 // 2, (t, b, ones)  # rows >= the median
+    # through the origin the cluster mostly lands in few buckets
+    assert len(np.unique(c0[0])) < len(np.unique(c1[0]))
 
 
 def test_empty_index_pads():
