@@ -14,6 +14,8 @@
 //     query, FP64 ground truth): loaded into device memory, searched from
 //     device pointers, evaluate() >= 0.999 (FP32 vs FP64 truth) and the
 //     arrays left as they were.
+//  6. IndexRPLSH (LSH front end) vs the oracle's LSH restatement with the
+//     index's projections and offsets: bit-exact I and D; reset empties it.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -24,6 +26,7 @@
 
 #include "curplsh/dataset.h"
 #include "curplsh/index_bf.h"
+#include "curplsh/index_rplsh.h"
 #include "curplsh/internal/norm.h"
 #include "curplsh/internal/search.h"
 #include "../../oracle/oracle.h"
@@ -188,6 +191,40 @@ static int test_dataset(DeviceResources& res) {
   return 0;
 }
 
+static int test_rplsh(DeviceResources& res) {
+  const int n = 20000, dim = 48, nq = 64, k = 10;
+  std::vector<float> x((size_t)n * dim), q((size_t)nq * dim);
+  CHECK(rplsh_gen_mixture(x.data(), n, dim, 16, 0.3f, 5, 6, 0.0) == RPLSH_OK);
+  CHECK(rplsh_gen_mixture(q.data(), nq, dim, 16, 0.3f, 5, 7, 0.0) == RPLSH_OK);
+  IndexRPLSHConfig cfg;
+  cfg.numTables = 8;
+  cfg.numBits = 12;
+  cfg.numProbes = 2;
+  cfg.maxCandidates = 1024;
+  IndexRPLSH index(&res, dim, cfg);
+  index.add(n, x.data());
+  CHECK(index.getNumData() == n);
+  std::vector<float> w((size_t)cfg.numTables * cfg.numBits * dim), off((size_t)cfg.numTables * cfg.numBits);
+  CHECK(rplsh_lsh_projections(index.handle(), w.data()) == RPLSH_OK);
+  CHECK(rplsh_lsh_offsets(index.handle(), off.data()) == RPLSH_OK);
+  std::vector<int> I((size_t)nq * k), Io((size_t)nq * k);
+  std::vector<float> D((size_t)nq * k), Do((size_t)nq * k);
+  index.search(nq, q.data(), k, I.data(), D.data());
+  int64_t raw = 0, distinct = 0;
+  oracle_lsh_search(x.data(), n, dim, w.data(), off.data(), cfg.numTables, cfg.numBits, cfg.numProbes,
+                    cfg.maxCandidates, q.data(), nq, k, Io.data(), Do.data(), &raw, &distinct);
+  CHECK(I == Io);
+  CHECK(std::memcmp(D.data(), Do.data(), D.size() * sizeof(float)) == 0);
+  const rplsh_lsh_stats st = index.lastStats();
+  CHECK(st.raw_candidates == raw && st.distinct_candidates == distinct);
+  index.reset();
+  CHECK(index.getNumData() == 0);
+  index.search(nq, q.data(), k, I.data(), D.data());
+  for (int v : I) CHECK(v == -1);
+  std::printf("IndexRPLSH ok (%.1f distinct candidates per query)\n", (double)distinct / nq);
+  return 0;
+}
+
 int main() {
   if (getNumDevices() < 1) {
     std::fprintf(stderr, "no GPU\n");
@@ -199,6 +236,7 @@ int main() {
   if (test_search_fixture(res)) return 1;
   if (test_index(res)) return 1;
   if (test_dataset(res)) return 1;
+  if (test_rplsh(res)) return 1;
   std::printf("all C++ tests passed\n");
   return 0;
 }
