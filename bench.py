@@ -602,10 +602,12 @@ def rplsh_front_end(args, ca) -> int:
         workloads[name] = {"n": n, "brute_force": brute, "configs": rows}
         del x, q
     last = list(workloads)[-1]
-    best = max(workloads[last]["configs"], key=lambda r: (r["recall_at_10"] >= 0.9, r["qps"]))
+    # the fastest config with recall@10 >= 0.9, else the highest-recall one
+    best = max(workloads[last]["configs"],
+               key=lambda r: (r["recall_at_10"] >= 0.9, r["qps"] if r["recall_at_10"] >= 0.9 else r["recall_at_10"]))
     line = {
         "metric": "IndexRPLSH QPS, k=10 (LSH candidates + exact re-rank), fastest config "
-                  "with recall@10 >= 0.9",
+                  "with recall@10 >= 0.9 (else the highest-recall config)",
         "value": best["qps"],
         "unit": "QPS",
         "n_gpus": 1,

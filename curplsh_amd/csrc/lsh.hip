@@ -27,10 +27,12 @@
 //                plain path's rerank_kernel): k smallest by (rel, index),
 //                D = rel + ||q||^2; fewer than k candidates pad with -1/FLT_MAX
 //
-// Kernels: lsh_hash_kernel (VALU, one thread per (row, table), the K chains
-// of a table interleaved so x is read once per table), lsh_gather_kernel (one
+// Kernels: lsh_hash_kernel (VALU: a workgroup stages 64 rows in LDS, each
+// wave hashes a quarter of the tables for them, a lane's K chains of a table
+// interleaved, projections through the scalar cache), lsh_gather_kernel (one
 // workgroup per query: probe codes, bucket ranges by binary search, a scan,
-// the gather, a bitonic sort and de-duplication in LDS). The table build is a
+// the gather, a bitonic sort and de-duplication in LDS; bucket searches start
+// from a directory over the codes' top 16 bits). The table build is a
 // stable LSD radix sort per table (hipCUB) at add time, off the search path.
 #include <hip/hip_runtime.h>
 
@@ -59,51 +61,65 @@ constexpr int64_t kLshCenterSample = 65536;
 
 // ---------------------------------------------------------------- hashing
 
-// Codes (and margins) of rows [0, n) for table blockIdx.y. Rows are 16-byte
-// aligned with ldx % 4 == 0 (zero padding past d is not read). Bit b of
-// codes[t * ldc + i] is s >= off[t * K + b]; margins, when not null, hold
-// s - off at margins[i * mrow + (t * K + b) * mcol] (queries: mrow = L * K,
-// mcol = 1; the transposed sample of the offset computation: mrow = 1,
-// mcol = rows). codes may be null.
-__global__ __launch_bounds__(kLshBlock) void lsh_hash_kernel(const float* __restrict__ x, int64_t ldx,
-                                                             int64_t n, int d, const float* __restrict__ w,
-                                                             const float* __restrict__ off, int L, int K,
-                                                             uint32_t* __restrict__ codes, int64_t ldc,
-                                                             float* __restrict__ margins, int64_t mrow,
-                                                             int64_t mcol) {
-  extern __shared__ float wt[];  // [K][d] of table t
-  const int t = blockIdx.y;
-  for (int e = threadIdx.x; e < K * d; e += blockDim.x) wt[e] = w[(int64_t)t * K * d + e];
-  __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float* xr = x + i * ldx;
-  float acc[kLshMaxBits];
-#pragma unroll
-  for (int b = 0; b < kLshMaxBits; ++b) acc[b] = 0.0f;
-  for (int j0 = 0; j0 < d; j0 += 4) {
-    const float4 v = *reinterpret_cast<const float4*>(xr + j0);
-    const float xv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int j = j0 + u;
-      if (j < d) {
-#pragma unroll
-        for (int b = 0; b < kLshMaxBits; ++b)
-          if (b < K) acc[b] = fmaf(wt[b * d + j], xv[u], acc[b]);
-      }
-    }
+constexpr int kHashRows = 64;   // rows a workgroup stages (one per lane)
+constexpr int kHashWaves = 4;
+// Chains a hash lane runs per table: K rounded up to a multiple of 8.
+inline int hash_width(int K) { return (K + 7) / 8 * 8; }
+
+// Codes (and margins) of rows [0, n) for every table. A workgroup stages
+// kHashRows rows in LDS with coalesced loads (row stride d + 1 floats: lanes
+// on different banks), then wave w hashes tables w, w + 4, ... of them, one
+// row per lane; the projections come from wj[t][j][0..KB) (j-major, zero
+// padded from K to KB = K rounded up to 8) at uniform addresses, i.e. through
+// the scalar cache, so each FMA takes its projection value as a scalar
+// operand and no bit needs a branch. With gridDim.y > 1 (few rows, e.g. a
+// query batch) the tables are spread over blockIdx.y as well. Bit b of codes[t * ldc + i] is
+// s >= off[t * K + b], s the canonical fmaf chain over j of W[t * K + b][j] *
+// x[j]; margins, when not null, hold s - off at
+// margins[i * mrow + (t * K + b) * mcol] (queries: mrow = L * K, mcol = 1;
+// the transposed sample of the offset computation: mrow = 1, mcol = rows).
+// codes may be null.
+template <int KB>
+__global__ __launch_bounds__(kHashRows * kHashWaves) void lsh_hash_kernel(
+    const float* __restrict__ x, int64_t ldx, int64_t n, int d, const float* __restrict__ wj,
+    const float* __restrict__ off, int L, int K, uint32_t* __restrict__ codes, int64_t ldc,
+    float* __restrict__ margins, int64_t mrow, int64_t mcol) {
+  extern __shared__ float xs[];  // [kHashRows][d + 1]
+  const int64_t r0 = (int64_t)blockIdx.x * kHashRows;
+  const int rows = (int)(n - r0 < kHashRows ? n - r0 : kHashRows);
+  const int ld = d + 1;
+  for (int e = threadIdx.x; e < rows * d; e += blockDim.x) {
+    const int r = e / d, j = e - r * d;
+    xs[r * ld + j] = x[(r0 + r) * ldx + j];
   }
-  const float* o = off + (int64_t)t * K;
-  uint32_t code = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (lane >= rows) return;
+  const float* xr = xs + lane * ld;
+  const int64_t i = r0 + lane;
+  for (int t = blockIdx.y * kHashWaves + wave; t < L; t += gridDim.y * kHashWaves) {
+    const float* wt = wj + (int64_t)t * d * KB;
+    float acc[KB];
 #pragma unroll
-  for (int b = 0; b < kLshMaxBits; ++b)
-    if (b < K && acc[b] >= o[b]) code |= 1u << b;
-  if (codes) codes[(int64_t)t * ldc + i] = code;
-  if (margins) {
+    for (int b = 0; b < KB; ++b) acc[b] = 0.0f;
+    for (int j = 0; j < d; ++j) {
+      const float xv = xr[j];
+      const float* wr = wt + j * KB;
 #pragma unroll
-    for (int b = 0; b < kLshMaxBits; ++b)
-      if (b < K) margins[i * mrow + ((int64_t)t * K + b) * mcol] = acc[b] - o[b];
+      for (int b = 0; b < KB; ++b) acc[b] = fmaf(wr[b], xv, acc[b]);
+    }
+    const float* o = off + (int64_t)t * K;
+    uint32_t code = 0;
+#pragma unroll
+    for (int b = 0; b < KB; ++b)
+      if (b < K && acc[b] >= o[b]) code |= 1u << b;
+    if (codes) codes[(int64_t)t * ldc + i] = code;
+    if (margins) {
+#pragma unroll
+      for (int b = 0; b < KB; ++b)
+        if (b < K) margins[i * mrow + ((int64_t)t * K + b) * mcol] = acc[b] - o[b];
+    }
   }
 }
 
@@ -125,7 +141,35 @@ __global__ void iota_kernel(int32_t* p, int64_t n) {
 
 // --------------------------------------------------------------- gathering
 
+// Directory over the top H = min(K, 16) bits of the codes: dir[t][h] is the
+// first position of table t's sorted codes whose top bits are >= h, for
+// h = 0..2^H (dir[t][2^H] = n), so a bucket search starts inside one slot.
+constexpr int kDirBits = 16;
+__host__ __device__ inline int dir_bits(int K) { return K < kDirBits ? K : kDirBits; }
+
+__device__ inline int64_t lower_bound_u32(const uint32_t* a, int64_t lo, int64_t hi, uint32_t v) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void lsh_directory_kernel(const uint32_t* __restrict__ sorted, int64_t ld, int64_t n, int L, int K,
+                                     int32_t* __restrict__ dir) {
+  const int H = dir_bits(K);
+  const int64_t slots = ((int64_t)1 << H) + 1;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= slots * L) return;
+  const int t = (int)(e / slots);
+  const int64_t h = e - (int64_t)t * slots;
+  const uint64_t first = (uint64_t)h << (K - H);  // smallest code with top bits h
+  dir[e] = first >= ((uint64_t)1 << K) ? (int32_t)n
+                                       : (int32_t)lower_bound_u32(sorted + (int64_t)t * ld, 0, n, (uint32_t)first);
+}
+
 struct LshGatherArgs {
+  const int32_t* dir;      // [L][2^H + 1] directory (lsh_directory_kernel)
   const uint32_t* sorted;  // [L][ld] sorted codes of each table
   const int32_t* perm;     // [L][ld] rows in bucket order
   int64_t ld;              // table stride
@@ -137,15 +181,6 @@ struct LshGatherArgs {
   unsigned long long* raw_total;
   unsigned long long* distinct_total;
 };
-
-__device__ inline int64_t lower_bound_u32(const uint32_t* a, int64_t n, uint32_t v) {
-  int64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (a[mid] < v) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
 
 // Sort key of a probe candidate bit: |margin| as an ordered bit pattern
 // (NaN after +inf), then the bit index.
@@ -186,13 +221,19 @@ __global__ __launch_bounds__(kLshBlock) void lsh_gather_kernel(LshGatherArgs a) 
   }
   __syncthreads();
 
-  // 2. bucket ranges; counts clamped to B (only the first B entries matter)
+  // 2. bucket ranges inside the code's directory slot; counts clamped to B
+  //    (only the first B entries matter)
+  const int H = dir_bits(K);
+  const int64_t slots = ((int64_t)1 << H) + 1;
   for (int e = threadIdx.x; e < E; e += blockDim.x) {
     const int t = e % L;
     const uint32_t* s = a.sorted + (int64_t)t * a.ld;
     const uint32_t c = pcode[e];
-    const int64_t lo = lower_bound_u32(s, a.n, c);
-    const int64_t hi = c == 0xffffffffu ? a.n : lower_bound_u32(s, a.n, c + 1);
+    const int32_t* dt = a.dir + (int64_t)t * slots;
+    const uint32_t h = c >> (K - H);
+    const int64_t s0 = dt[h], s1 = dt[h + 1];
+    const int64_t lo = lower_bound_u32(s, s0, s1, c);
+    const int64_t hi = lower_bound_u32(s, lo, s1, c + 1);
     elo[e] = (int32_t)lo;
     ecnt[e] = (int32_t)(hi - lo < B ? hi - lo : B);
   }
@@ -287,10 +328,12 @@ struct rplsh_lsh {
   float* x = nullptr;         // [capacity][ldx], zero padded
   float* xn = nullptr;        // [capacity]
   float* w = nullptr;         // [L*K][dim]
+  float* wj = nullptr;        // [L][dim][KB]: W j-major per table, zero padded to KB (hash kernel)
   float* off = nullptr;       // [L*K] hyperplane offsets (0, or medians set at the first add)
   bool off_set = false;
   uint32_t* codes = nullptr;  // [L][capacity] codes in row order
   uint32_t* sorted = nullptr; // [L][capacity]
+  int32_t* dir = nullptr;     // [L][2^min(K,16) + 1] bucket directory
   int32_t* perm = nullptr;    // [L][capacity]
   DevBuf iota, sort_tmp, ctr_dots, ctr_sorted, ctr_seg, qstage, qpad, qn, qcodes, qmarg, cand, idev, ddev, counters, status, relk, fp64;
   rplsh_lsh_stats last{};
@@ -324,10 +367,26 @@ void launch_hash(const rplsh_lsh* h, const float* rows, int64_t n, uint32_t* cod
                  float* margins, int64_t mrow, int64_t mcol, hipStream_t st) {
   if (n <= 0) return;
   const int L = h->cfg.tables, K = h->cfg.bits;
-  const dim3 grid((unsigned)div_up(n, kLshBlock), (unsigned)L);
-  const size_t lds = (size_t)K * h->dim * sizeof(float);
-  lsh_hash_kernel<<<grid, kLshBlock, lds, st>>>(rows, h->ldx, n, h->dim, h->w, h->off, L, K, codes, ldc,
-                                                margins, mrow, mcol);
+  const size_t lds = (size_t)kHashRows * (h->dim + 1) * sizeof(float);
+  // Few row groups (a query batch): one table per wave over a second grid
+  // dimension, so the launch still fills the device.
+  const int64_t groups = div_up(n, kHashRows);
+  const unsigned gy = groups < 2048 ? (unsigned)div_up(L, kHashWaves) : 1u;
+  const dim3 grid((unsigned)groups, gy), block(kHashRows * kHashWaves);
+  switch (hash_width(K)) {
+    case 8:
+      lsh_hash_kernel<8><<<grid, block, lds, st>>>(rows, h->ldx, n, h->dim, h->wj, h->off, L, K, codes, ldc,
+                                                   margins, mrow, mcol);
+      break;
+    case 16:
+      lsh_hash_kernel<16><<<grid, block, lds, st>>>(rows, h->ldx, n, h->dim, h->wj, h->off, L, K, codes, ldc,
+                                                    margins, mrow, mcol);
+      break;
+    default:
+      lsh_hash_kernel<24><<<grid, block, lds, st>>>(rows, h->ldx, n, h->dim, h->wj, h->off, L, K, codes, ldc,
+                                                    margins, mrow, mcol);
+      break;
+  }
   RPLSH_HIP(hipGetLastError());
 }
 
@@ -381,6 +440,10 @@ void build_tables(rplsh_lsh* h, hipStream_t st) {
     RPLSH_HIP(hipcub::DeviceRadixSort::SortPairs(tp, tb, h->codes + o, h->sorted + o, rows, h->perm + o,
                                                  (int)h->n, 0, h->cfg.bits, st));
   }
+  const int64_t entries = (((int64_t)1 << dir_bits(h->cfg.bits)) + 1) * L;
+  lsh_directory_kernel<<<(unsigned)div_up(entries, 256), 256, 0, st>>>(h->sorted, h->capacity, h->n, L,
+                                                                       h->cfg.bits, h->dir);
+  RPLSH_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -436,11 +499,22 @@ rplsh_status rplsh_lsh_create(rplsh_resources* res, int dim, const rplsh_lsh_con
     try {
       RPLSH_HIP(hipMalloc(&h->w, wh.size() * sizeof(float)));
       RPLSH_HIP(hipMemcpy(h->w, wh.data(), wh.size() * sizeof(float), hipMemcpyHostToDevice));
+      const int kb = hash_width(c.bits);
+      std::vector<float> wjh((size_t)c.tables * dim * kb, 0.0f);
+      for (int t = 0; t < c.tables; ++t)
+        for (int b = 0; b < c.bits; ++b)
+          for (int j = 0; j < dim; ++j)
+            wjh[((size_t)t * dim + j) * kb + b] = wh[((size_t)t * c.bits + b) * dim + j];
+      RPLSH_HIP(hipMalloc(&h->wj, wjh.size() * sizeof(float)));
+      RPLSH_HIP(hipMemcpy(h->wj, wjh.data(), wjh.size() * sizeof(float), hipMemcpyHostToDevice));
       RPLSH_HIP(hipMalloc(&h->off, (size_t)rows * sizeof(float)));
       clear_offsets(h);
+      RPLSH_HIP(hipMalloc(&h->dir, (((size_t)1 << dir_bits(c.bits)) + 1) * c.tables * sizeof(int32_t)));
     } catch (...) {
       if (h->w) (void)hipFree(h->w);
+      if (h->wj) (void)hipFree(h->wj);
       if (h->off) (void)hipFree(h->off);
+      if (h->dir) (void)hipFree(h->dir);
       delete h;
       throw;
     }
@@ -455,7 +529,9 @@ rplsh_status rplsh_lsh_destroy(rplsh_lsh* h) {
     (void)hipDeviceSynchronize();
     lsh_free(h);
     if (h->w) RPLSH_HIP(hipFree(h->w));
+    if (h->wj) RPLSH_HIP(hipFree(h->wj));
     if (h->off) RPLSH_HIP(hipFree(h->off));
+    if (h->dir) RPLSH_HIP(hipFree(h->dir));
     delete h;
   });
 }
@@ -566,6 +642,7 @@ rplsh_status rplsh_lsh_search(rplsh_lsh* h, int64_t nq, const float* queries, in
       // candidates
       uint64_t* cand = h->cand.as<uint64_t>((size_t)nq * B);
       LshGatherArgs ga{};
+      ga.dir = h->dir;
       ga.sorted = h->sorted;
       ga.perm = h->perm;
       ga.ld = h->capacity;
