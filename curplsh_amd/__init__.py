@@ -23,7 +23,8 @@ from ._lib import PREC_BF16, PREC_FP8, PREC_FP32, RplshError, call, device_count
 
 __all__ = [
     "DeviceResources", "DeviceArray", "IndexBF", "IndexBFConfig", "RplshError",
-    "search_l2_distance", "compute_l2_norm", "encode_bf16", "encode_e4m3",
+    "search_l2_distance", "compute_l2_norm", "l2_select", "block_select", "broadcast_rows_sum",
+    "encode_bf16", "encode_e4m3",
     "gen_uniform", "gen_gaussian", "gen_mixture", "device_count", "Event", "load",
     "PREC_FP32", "PREC_BF16", "PREC_FP8", "Server", "IndexRPLSH", "IndexRPLSHConfig",
 ]
@@ -538,6 +539,31 @@ def search_l2_distance(bases: DeviceArray, bases_norm: DeviceArray | None, queri
     call("rplsh_search_l2_dev", bases.ptr if bases is not None else None, n, d,
          bases_norm.ptr if bases_norm is not None else None, queries.ptr, nq, int(k),
          indices.ptr, distances.ptr, int(exact), stream)
+
+
+def l2_select(products: DeviceArray, base_norms: DeviceArray, k: int, distances: DeviceArray,
+              indices: DeviceArray, stream: int = 0) -> None:
+    """l2Select (include/internal/select.h:29-34): per row the k smallest
+    base_norms[i] + products[r][i], ascending, ties by the lower i."""
+    rows, n = products.shape
+    call("rplsh_l2_select_dev", products.ptr, rows, n, n, base_norms.ptr, int(k), distances.ptr,
+         indices.ptr, stream)
+
+
+def block_select(values: DeviceArray, k: int, out_k: DeviceArray, out_v: DeviceArray,
+                 labels: DeviceArray | None = None, select_max: bool = False, stream: int = 0) -> None:
+    """blockSelect (include/internal/select.h:8-24): per row the k smallest
+    (or largest) values with column or given labels, ties by the lower label."""
+    rows, n = values.shape
+    call("rplsh_block_select_dev", values.ptr, labels.ptr if labels is not None else None, rows, n,
+         int(k), int(select_max), out_k.ptr, out_v.ptr, stream)
+
+
+def broadcast_rows_sum(v: DeviceArray, m: DeviceArray, stream: int = 0) -> None:
+    """broadcastAlongRowsSum (include/internal/broadcast.h:7-8): m[r][c] += v[r]."""
+    rows, cols = m.shape
+    call("rplsh_broadcast_rows_sum_dev", v.ptr, m.ptr, rows, cols, stream)
+    call("rplsh_stream_sync", stream)
 
 
 def encode_bf16(x: DeviceArray, ld: int, out: DeviceArray, stream: int = 0) -> None:

@@ -198,6 +198,9 @@ _SIGS = {
     "rplsh_encode_bf16_dev": [_vp, _i64, _i32, _i32, _vp, _vp],
     "rplsh_encode_e4m3_dev": [_vp, _i64, _i32, _i32, _vp, _vp, _vp],
     "rplsh_search_l2_dev": [_vp, _i64, _i32, _vp, _vp, _i64, _i32, _vp, _vp, _i32, _vp],
+    "rplsh_l2_select_dev": [_vp, _i64, _i64, _i64, _vp, _i32, _vp, _vp, _vp],
+    "rplsh_block_select_dev": [_vp, _vp, _i64, _i64, _i32, _i32, _vp, _vp, _vp],
+    "rplsh_broadcast_rows_sum_dev": [_vp, _vp, _i64, _i64, _vp],
     "rplsh_bf_config_default": [C.POINTER(BfConfig)],
     "rplsh_bf_create": [_vp, _i32, C.POINTER(BfConfig), C.POINTER(_vp)],
     "rplsh_bf_destroy": [_vp],

@@ -8,7 +8,9 @@
 #include "curplsh/index_bf.h"
 #include "curplsh/index_rplsh.h"
 #include "curplsh/internal/assertions.h"
+#include "curplsh/internal/broadcast.h"
 #include "curplsh/internal/norm.h"
+#include "curplsh/internal/select.h"
 #include "curplsh/internal/search.h"
 
 namespace curplsh {
@@ -207,6 +209,44 @@ void computeL2Norm(const Tensor<float, 2>& vectors, Tensor<float, 1>& norms, boo
   checkStatus(rplsh_l2norm_dev(vectors.data(), vectors.getSize(0), (int)vectors.getSize(1),
                                norms.data(), squared ? 1 : 0, stream),
               "computeL2Norm");
+}
+
+// Shape preconditions of select.cu (row counts and k agree).
+void blockSelect(Tensor<float, 2>& in, Tensor<float, 2>& outK, Tensor<int, 2>& outV, int k,
+                 bool selectMax, Stream stream) {
+  host_assert(outK.getSize(0) == in.getSize(0) && outV.getSize(0) == in.getSize(0));
+  host_assert(outK.getSize(1) == k && outV.getSize(1) == k);
+  checkStatus(rplsh_block_select_dev(in.data(), nullptr, in.getSize(0), in.getSize(1), k, selectMax ? 1 : 0,
+                                     outK.data(), outV.data(), stream),
+              "blockSelect");
+}
+
+void blockSelect(Tensor<float, 2>& inK, Tensor<int, 2>& inV, Tensor<float, 2>& outK,
+                 Tensor<int, 2>& outV, int k, bool selectMax, Stream stream) {
+  host_assert(inV.getSize(0) == inK.getSize(0) && inV.getSize(1) == inK.getSize(1));
+  host_assert(outK.getSize(0) == inK.getSize(0) && outV.getSize(0) == inK.getSize(0));
+  host_assert(outK.getSize(1) == k && outV.getSize(1) == k);
+  checkStatus(rplsh_block_select_dev(inK.data(), inV.data(), inK.getSize(0), inK.getSize(1), k,
+                                     selectMax ? 1 : 0, outK.data(), outV.data(), stream),
+              "blockSelect");
+}
+
+void l2Select(Tensor<float, 2>& productDistances, Tensor<float, 1>& baseNorms, Tensor<float, 2>& distances,
+              Tensor<int, 2>& indices, int k, Stream stream) {
+  host_assert(baseNorms.getSize(0) == productDistances.getSize(1));
+  host_assert(distances.getSize(0) == productDistances.getSize(0) && indices.getSize(0) == distances.getSize(0));
+  host_assert(distances.getSize(1) == k && indices.getSize(1) == k);
+  checkStatus(rplsh_l2_select_dev(productDistances.data(), productDistances.getSize(0), productDistances.getSize(1),
+                                  productDistances.getStride(0), baseNorms.data(), k, distances.data(),
+                                  indices.data(), stream),
+              "l2Select");
+}
+
+void broadcastAlongRowsSum(Tensor<float, 1>& input, Tensor<float, 2>& output, Stream stream) {
+  host_assert(input.getSize(0) == output.getSize(0));  // broadcast.cu:43
+  checkStatus(rplsh_broadcast_rows_sum_dev(input.data(), output.data(), output.getSize(0), output.getSize(1),
+                                           stream),
+              "broadcastAlongRowsSum");
 }
 
 // Shape preconditions of search.cu:86-89.

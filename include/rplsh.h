@@ -132,6 +132,27 @@ rplsh_status rplsh_search_l2_dev(const float* x, int64_t n, int d, const float* 
                                  const float* q, int64_t nq, int k, int32_t* indices,
                                  float* distances, int exact, void* stream);
 
+/* l2Select (include/internal/select.h:29-34, src/internal/select.cu:189-224):
+ * per row r of products [rows][n] (row stride ldp), the k smallest
+ * v = base_norms[i] + products[r][i] (one FP32 add), ascending, ties by the
+ * lower i; outputs [rows][k]; k > n pads FLT_MAX / -1. k <= 4096. Device
+ * pointers; returns when the results are in place. */
+rplsh_status rplsh_l2_select_dev(const float* products, int64_t rows, int64_t n, int64_t ldp,
+                                 const float* base_norms, int k, float* out_dist, int32_t* out_idx,
+                                 void* stream);
+/* blockSelect, both overloads (include/internal/select.h:8-24): per row of
+ * in [rows][n] the k smallest values (select_max == 0, ascending) or the k
+ * largest (descending), labelled by the column (labels == NULL) or by
+ * labels[r][i]; ties by the lower label; k > n pads FLT_MAX (-FLT_MAX for the
+ * largest) / -1. k <= 4096. Device pointers; returns when done. */
+rplsh_status rplsh_block_select_dev(const float* in, const int32_t* labels, int64_t rows, int64_t n,
+                                    int k, int select_max, float* out_k, int32_t* out_v, void* stream);
+/* broadcastAlongRowsSum (include/internal/broadcast.h:7-8,
+ * src/internal/broadcast.cu:15-74): m[r][c] += v[r]. Device pointers,
+ * enqueued on `stream`. */
+rplsh_status rplsh_broadcast_rows_sum_dev(const float* v, float* m, int64_t rows, int64_t cols,
+                                          void* stream);
+
 /* ------------------------------------------------------------ IndexBF -- */
 /* IndexBFConfig + IndexConfig (include/index_bf.h:8-19, include/index.h:8-16)
  * plus the precision-form knobs of this library. */

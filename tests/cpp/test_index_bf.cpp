@@ -16,6 +16,9 @@
 //     arrays left as they were.
 //  6. IndexRPLSH (LSH front end) vs the oracle's LSH restatement with the
 //     index's projections and offsets: bit-exact I and D; reset empties it.
+//  7. The internal selection functions: blockSelect (smallest / largest,
+//     implicit and given labels), l2Select + broadcastAlongRowsSum on a small
+//     fixture with exact ties (lower label first).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -27,7 +30,9 @@
 #include "curplsh/dataset.h"
 #include "curplsh/index_bf.h"
 #include "curplsh/index_rplsh.h"
+#include "curplsh/internal/broadcast.h"
 #include "curplsh/internal/norm.h"
+#include "curplsh/internal/select.h"
 #include "curplsh/internal/search.h"
 #include "../../oracle/oracle.h"
 
@@ -191,6 +196,50 @@ static int test_dataset(DeviceResources& res) {
   return 0;
 }
 
+static int test_internal_select(DeviceResources& res) {
+  // rows: {5, 1, 3, 1, 9} and {2, 2, 8, 0.5, 2}; labels 10..14 / 20..24
+  const float vals[10] = {5, 1, 3, 1, 9, 2, 2, 8, 0.5f, 2};
+  const int labs[10] = {10, 11, 12, 13, 14, 24, 23, 22, 21, 20};
+  Stream s = res.getDefaultStream(0);
+  DeviceTensor<float, 2> in({2, 5});
+  DeviceTensor<int, 2> lab({2, 5});
+  in.copyFrom(vals, s);
+  lab.copyFrom(labs, s);
+  DeviceTensor<float, 2> ok({2, 3});
+  DeviceTensor<int, 2> ov({2, 3});
+  float k3[6];
+  int v3[6];
+  blockSelect(in, ok, ov, 3, false, s);  // column labels
+  ok.copyTo(k3, s);
+  ov.copyTo(v3, s);
+  streamSynchronize(s);
+  CHECK(k3[0] == 1 && v3[0] == 1 && k3[1] == 1 && v3[1] == 3 && k3[2] == 3 && v3[2] == 2);
+  CHECK(k3[3] == 0.5f && v3[3] == 3 && k3[4] == 2 && v3[4] == 0 && k3[5] == 2 && v3[5] == 1);
+  blockSelect(in, lab, ok, ov, 3, true, s);  // given labels, largest first
+  ok.copyTo(k3, s);
+  ov.copyTo(v3, s);
+  streamSynchronize(s);
+  CHECK(k3[0] == 9 && v3[0] == 14 && k3[1] == 5 && v3[1] == 10 && k3[2] == 3 && v3[2] == 12);
+  CHECK(k3[3] == 8 && v3[3] == 22 && k3[4] == 2 && v3[4] == 20 && k3[5] == 2 && v3[5] == 23);
+  // l2Select: norms + products, then + 10 per row
+  const float norms[5] = {0, 1, 0, 2, 0};
+  DeviceTensor<float, 1> dn({5});
+  dn.copyFrom(norms, s);
+  DeviceTensor<float, 1> add({2});
+  const float adds[2] = {10, 20};
+  add.copyFrom(adds, s);
+  l2Select(in, dn, ok, ov, 3, s);
+  broadcastAlongRowsSum(add, ok, s);
+  ok.copyTo(k3, s);
+  ov.copyTo(v3, s);
+  streamSynchronize(s);
+  // row 0 keys {5, 2, 3, 3, 9} -> (2,1) (3,2) (3,3); row 1 {2, 3, 8, 2.5, 2} -> (2,0) (2,4) (2.5,3)
+  CHECK(k3[0] == 12 && v3[0] == 1 && k3[1] == 13 && v3[1] == 2 && k3[2] == 13 && v3[2] == 3);
+  CHECK(k3[3] == 22 && v3[3] == 0 && k3[4] == 22 && v3[4] == 4 && k3[5] == 22.5f && v3[5] == 3);
+  std::printf("internal select ok\n");
+  return 0;
+}
+
 static int test_rplsh(DeviceResources& res) {
   const int n = 20000, dim = 48, nq = 64, k = 10;
   std::vector<float> x((size_t)n * dim), q((size_t)nq * dim);
@@ -237,6 +286,7 @@ int main() {
   if (test_index(res)) return 1;
   if (test_dataset(res)) return 1;
   if (test_rplsh(res)) return 1;
+  if (test_internal_select(res)) return 1;
   std::printf("all C++ tests passed\n");
   return 0;
 }

@@ -42,6 +42,10 @@ def test_cpp_library_exports_index_api():
                  "curplsh::Index::Index(curplsh::DeviceResources*, int, curplsh::IndexConfig)",
                  "curplsh::searchL2Distance(",
                  "curplsh::computeL2Norm(",
+                 "curplsh::l2Select(",
+                 "curplsh::blockSelect(",
+                 "curplsh::broadcastAlongRowsSum(",
+                 "curplsh::IndexRPLSH::search(int, float const*, int, int*, float*)",
                  "curplsh::DeviceResources::getDefaultStream(int)"]:
         assert name in out, name
 
