@@ -271,6 +271,67 @@ __global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) 
 // during stage s+1, so every thread reads the same tail and takes the same
 // branch, and it is not reused before the barrier that ends stage s+1.
 constexpr int kQueue = 512;  // entries per survivor queue (power of two)
+constexpr int kHitSlots = 8;  // lanes a wave stages at a time in emit_hits
+
+// Slow path of the filters: emits every value >= its query's threshold in the
+// wave's current tile (row0 = its first row), query tile by query tile. The
+// lanes that hold one stage their 16 accumulator values in the wave's LDS
+// slots hb[kHitSlots][16], kHitSlots lanes at a time; the wave then serves
+// four staged lanes per step, 16 lanes each: lane j of group g compares value
+// j of staged lane L_g, a ballot gives the hits, one LDS atomic on `tail`
+// reserves their queue positions and put() stores them. No per-value branch:
+// a sparse tile (the full-database level) costs a few instructions per
+// hitting lane instead of a compare-and-branch per accumulator value. Value i
+// of lane L is row row0 + 8 (i >> 2) + 4 (L >> 5) + (i & 3) of the query in
+// qrow of lane L (the 32x32 MFMA accumulator layout).
+template <int TN, typename Put>
+__device__ __forceinline__ void emit_hits(const f32x16 (&acc)[TN], const float (&mx)[TN],
+                                          const float (&thr)[TN], const int (&qrow)[TN], int64_t row0,
+                                          int lane, float* hb, unsigned* tail, Put&& put) {
+  const int g = lane >> 4, j = lane & 15;
+#pragma unroll
+  for (int nt = 0; nt < TN; ++nt) {
+    uint64_t B = __ballot(mx[nt] >= thr[nt]);
+    while (B != 0) {
+      const unsigned rank =
+          __builtin_amdgcn_mbcnt_hi((uint32_t)(B >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)B, 0u));
+      if ((B >> lane & 1ull) && rank < (unsigned)kHitSlots) {
+        f32x4* d = reinterpret_cast<f32x4*>(hb + rank * 16);
+        d[0] = f32x4{acc[nt][0], acc[nt][1], acc[nt][2], acc[nt][3]};
+        d[1] = f32x4{acc[nt][4], acc[nt][5], acc[nt][6], acc[nt][7]};
+        d[2] = f32x4{acc[nt][8], acc[nt][9], acc[nt][10], acc[nt][11]};
+        d[3] = f32x4{acc[nt][12], acc[nt][13], acc[nt][14], acc[nt][15]};
+      }
+      __builtin_amdgcn_wave_barrier();
+      const int staged = __popcll(B) < kHitSlots ? __popcll(B) : kHitSlots;
+      for (int s0 = 0; s0 < staged; s0 += 4) {
+        // staged lanes s0 .. s0 + 3: the lowest set bits of B, one per group
+        int lsel = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int L = B != 0 ? __ffsll((unsigned long long)B) - 1 : 0;  // wave-uniform
+          if (u == g) lsel = L;
+          if (s0 + u < staged) B &= B - 1;
+        }
+        const bool live = s0 + g < staged;
+        const float thr_l = __shfl(thr[nt], lsel);
+        const int qr_l = __shfl(qrow[nt], lsel);
+        const float v = live ? hb[(s0 + g) * 16 + j] : -INFINITY;
+        const bool hit = v >= thr_l;
+        const uint64_t H = __ballot(hit);
+        unsigned base = 0u;
+        if (lane == 0) base = atomicAdd(tail, (unsigned)__popcll(H));
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (hit) {
+          const unsigned t =
+              base + __builtin_amdgcn_mbcnt_hi((uint32_t)(H >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)H, 0u));
+          put(t, qr_l, v, row0 + 8 * (j >> 2) + 4 * (lsel >> 5) + (j & 3));
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
 
 template <Elem E, int DPAD, bool WIDE, int NW>
 struct StageShape {
@@ -284,7 +345,7 @@ struct StageShape {
   static constexpr int SM = kFit > kMaxSM ? kMaxSM : (kFit < 1 ? 1 : kFit);
   static constexpr int SR = 32 * SM;  // rows per stage
   static constexpr int STAGE = SM * G::TILE;
-  static constexpr int LDS = 2 * STAGE + 2 * SR * 8 + 2 * kQueue * 12;
+  static constexpr int LDS = 2 * STAGE + 2 * SR * 8 + 2 * kQueue * 12 + NW * kHitSlots * 64;
   static constexpr int WAVES_PER_SIMD = (NW == 4 && !WIDE && 3 * LDS <= 160 * 1024) ? 3 : 2;
 };
 
@@ -313,6 +374,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
   __shared__ uint64_t qword[2][kQueue];
   __shared__ int qquery[2][kQueue];
   __shared__ unsigned qtail[2];
+  __shared__ __attribute__((aligned(16))) float hitbuf[NW][kHitSlots * 16];
 
   const int per_xcd = (int)(gridDim.x >> 3);
   const int logical = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
@@ -420,9 +482,10 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
   };
 
   unsigned head[2] = {0u, 0u};  // flushed up to here (same in every thread)
-  auto emit = [&](int par, int qr, float v, int64_t row) {
+  // Stores a survivor at queue position t (reserved on qtail[par]); past the
+  // queue's capacity it goes straight to the query's list.
+  auto put = [&](int par, unsigned t, int qr, float v, int64_t row) {
     const uint64_t w = pack_cand(-2.0f * v, (int32_t)(row * stride));
-    const unsigned t = atomicAdd(&qtail[par], 1u);
     if (t - head[par] < (unsigned)kQueue) {
       qword[par][t & (kQueue - 1)] = w;
       qquery[par][t & (kQueue - 1)] = qr;
@@ -503,18 +566,9 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
           mx[nt] = v;
           hit |= v >= thr[nt];
         }
-        if (__any(hit)) {
-#pragma unroll
-          for (int nt = 0; nt < TN; ++nt) {
-            if (mx[nt] >= thr[nt]) {
-#pragma unroll
-              for (int i = 0; i < 16; ++i) {
-                const float v = acc[nt][i];
-                if (v >= thr[nt]) emit(par, qrow[nt], v, j0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3));
-              }
-            }
-          }
-        }
+        if (__any(hit))
+          emit_hits<TN>(acc, mx, thr, qrow, j0 + mt * 32, lane, hitbuf[wave], &qtail[par],
+                        [&](unsigned t, int qr, float v, int64_t row) { put(par, t, qr, v, row); });
       }
       if (has_next) sstore(buf ^ 1);
       __syncthreads();
@@ -555,7 +609,7 @@ struct RingShape {
   static constexpr int DATA_OPS = STAGE / 1024;  // one op: 64 lanes x 16 B
   static constexpr int ROW_OPS = (SR + 63) / 64;  // ops per per-row array
   static constexpr int OPS = DATA_OPS + ROW_OPS * (E == Elem::FP8 ? 2 : 1);
-  static constexpr int LDS = kRing * STAGE + kRing * SR * 8 + 2 * kQueue * 12;
+  static constexpr int LDS = kRing * STAGE + kRing * SR * 8 + 2 * kQueue * 12 + 4 * kHitSlots * 64;
   static constexpr int WAVES_PER_SIMD = (!WIDE && 3 * LDS <= 160 * 1024) ? 3 : 2;
   static_assert(STAGE % 1024 == 0, "stage is whole 1 KB copies");
 };
@@ -592,6 +646,7 @@ __global__ __launch_bounds__(kBlock, (RingShape<E, DPAD, WIDE>::WAVES_PER_SIMD))
   __shared__ uint64_t qword[2][kQueue];
   __shared__ int qquery[2][kQueue];
   __shared__ unsigned qtail[2];
+  __shared__ __attribute__((aligned(16))) float hitbuf[kBlock / 64][kHitSlots * 16];
 
   const int per_xcd = (int)(gridDim.x >> 3);
   const int logical = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
@@ -677,9 +732,8 @@ __global__ __launch_bounds__(kBlock, (RingShape<E, DPAD, WIDE>::WAVES_PER_SIMD))
   };
 
   unsigned head[2] = {0u, 0u};
-  auto emit = [&](int par, int qr, float v, int64_t row) {
+  auto put = [&](int par, unsigned t, int qr, float v, int64_t row) {
     const uint64_t w = pack_cand(-2.0f * v, (int32_t)(row * stride));
-    const unsigned t = atomicAdd(&qtail[par], 1u);
     if (t - head[par] < (unsigned)kQueue) {
       qword[par][t & (kQueue - 1)] = w;
       qquery[par][t & (kQueue - 1)] = qr;
@@ -766,18 +820,9 @@ __global__ __launch_bounds__(kBlock, (RingShape<E, DPAD, WIDE>::WAVES_PER_SIMD))
           mx[nt] = v;
           hit |= v >= thr[nt];
         }
-        if (__any(hit)) {
-#pragma unroll
-          for (int nt = 0; nt < TN; ++nt) {
-            if (mx[nt] >= thr[nt]) {
-#pragma unroll
-              for (int i = 0; i < 16; ++i) {
-                const float v = acc[nt][i];
-                if (v >= thr[nt]) emit(par, qrow[nt], v, j0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3));
-              }
-            }
-          }
-        }
+        if (__any(hit))
+          emit_hits<TN>(acc, mx, thr, qrow, j0 + mt * 32, lane, hitbuf[wave], &qtail[par],
+                        [&](unsigned t, int qr, float v, int64_t row) { put(par, t, qr, v, row); });
       }
       slot = slot == kRing - 1 ? 0 : slot + 1;
     }
