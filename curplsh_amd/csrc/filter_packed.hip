@@ -333,6 +333,85 @@ __device__ __forceinline__ void emit_hits(const f32x16 (&acc)[TN], const float (
   }
 }
 
+// Deferred form of emit_hits for the LDS kernels. In a tile's epilogue the
+// hitting lanes only stage their 16 values plus their threshold, query and
+// row offset (mt * 32 + 4 (L >> 5)) in the wave's kPend pending slots (LDS
+// stores, nothing waited on); drain_hits() emits them -- four staged lanes
+// per step as in emit_hits -- right after the next tile's MFMAs are issued,
+// so its LDS round trips overlap MFMA execution, and at the end of the
+// stage. A query tile with more hitting lanes than free slots drains first.
+constexpr int kPend = 12;
+
+struct PendSlots {
+  float* v;    // [kPend][16]
+  float* thr;  // [kPend]
+  int* qr;     // [kPend]
+  int* off;    // [kPend] row offset within the stage
+};
+
+template <typename Put>
+__device__ __forceinline__ void drain_hits(int pend, const PendSlots& ps, int64_t j0, int lane, unsigned* tail,
+                                           Put&& put) {
+  const int g = lane >> 4, j = lane & 15;
+  for (int s0 = 0; s0 < pend; s0 += 4) {
+    const int e = s0 + g;
+    const bool live = e < pend;
+    const int ec = live ? e : 0;
+    const float thr_l = ps.thr[ec];
+    const int qr_l = ps.qr[ec];
+    const int off = ps.off[ec];
+    const float v = live ? ps.v[ec * 16 + j] : -INFINITY;
+    const bool hit = v >= thr_l;
+    const uint64_t H = __ballot(hit);
+    unsigned base = 0u;
+    if (lane == 0) base = atomicAdd(tail, (unsigned)__popcll(H));
+    base = __builtin_amdgcn_readfirstlane(base);
+    if (hit) {
+      const unsigned t =
+          base + __builtin_amdgcn_mbcnt_hi((uint32_t)(H >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)H, 0u));
+      put(t, qr_l, v, j0 + off + 8 * (j >> 2) + (j & 3));
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Stages the hitting lanes of the tile (first row offset mt * 32 within the
+// stage); `pend` is the wave's pending count (uniform), drained when full.
+template <int TN, typename Put>
+__device__ __forceinline__ void stage_hits(const f32x16 (&acc)[TN], const float (&mx)[TN], const float (&thr)[TN],
+                                           const int (&qrow)[TN], int mt, int64_t j0, int lane, int& pend,
+                                           const PendSlots& ps, unsigned* tail, Put&& put) {
+#pragma unroll
+  for (int nt = 0; nt < TN; ++nt) {
+    uint64_t B = __ballot(mx[nt] >= thr[nt]);
+    while (B != 0) {
+      if (pend == kPend) {
+        __builtin_amdgcn_wave_barrier();
+        drain_hits(pend, ps, j0, lane, tail, put);
+        pend = 0;
+      }
+      const int room = kPend - pend;
+      const unsigned rank =
+          __builtin_amdgcn_mbcnt_hi((uint32_t)(B >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)B, 0u));
+      const bool take = (B >> lane & 1ull) && rank < (unsigned)room;
+      if (take) {
+        const int slot = pend + (int)rank;
+        f32x4* d = reinterpret_cast<f32x4*>(ps.v + slot * 16);
+        d[0] = f32x4{acc[nt][0], acc[nt][1], acc[nt][2], acc[nt][3]};
+        d[1] = f32x4{acc[nt][4], acc[nt][5], acc[nt][6], acc[nt][7]};
+        d[2] = f32x4{acc[nt][8], acc[nt][9], acc[nt][10], acc[nt][11]};
+        d[3] = f32x4{acc[nt][12], acc[nt][13], acc[nt][14], acc[nt][15]};
+        ps.thr[slot] = thr[nt];
+        ps.qr[slot] = qrow[nt];
+        ps.off[slot] = mt * 32 + 4 * (lane >> 5);
+      }
+      const uint64_t T = __ballot(take);
+      pend += __popcll(T);
+      B &= ~T;
+    }
+  }
+}
+
 template <Elem E, int DPAD, bool WIDE, int NW>
 struct StageShape {
   using G = Geometry<E, DPAD>;
@@ -345,7 +424,7 @@ struct StageShape {
   static constexpr int SM = kFit > kMaxSM ? kMaxSM : (kFit < 1 ? 1 : kFit);
   static constexpr int SR = 32 * SM;  // rows per stage
   static constexpr int STAGE = SM * G::TILE;
-  static constexpr int LDS = 2 * STAGE + 2 * SR * 8 + 2 * kQueue * 12 + NW * kHitSlots * 64;
+  static constexpr int LDS = 2 * STAGE + 2 * SR * 8 + 2 * kQueue * 12 + NW * kPend * 76;
   static constexpr int WAVES_PER_SIMD = (NW == 4 && !WIDE && 3 * LDS <= 160 * 1024) ? 3 : 2;
 };
 
@@ -374,7 +453,10 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
   __shared__ uint64_t qword[2][kQueue];
   __shared__ int qquery[2][kQueue];
   __shared__ unsigned qtail[2];
-  __shared__ __attribute__((aligned(16))) float hitbuf[NW][kHitSlots * 16];
+  __shared__ __attribute__((aligned(16))) float pend_v[NW][kPend * 16];
+  __shared__ float pend_thr[NW][kPend];
+  __shared__ int pend_qr[NW][kPend];
+  __shared__ int pend_off[NW][kPend];
 
   const int per_xcd = (int)(gridDim.x >> 3);
   const int logical = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
@@ -482,6 +564,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
   };
 
   unsigned head[2] = {0u, 0u};  // flushed up to here (same in every thread)
+  const PendSlots ps{pend_v[wave], pend_thr[wave], pend_qr[wave], pend_off[wave]};
   // Stores a survivor at queue position t (reserved on qtail[par]); past the
   // queue's capacity it goes straight to the query's list.
   auto put = [&](int par, unsigned t, int qr, float v, int64_t row) {
@@ -522,6 +605,8 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
       const bool has_next = j0 + SR < r_end;
       if (has_next) gload(j0 + SR);
       const int par = s & 1;
+      auto put_par = [&](unsigned t, int qr, float v, int64_t row) { put(par, t, qr, v, row); };
+      int pend = 0;  // staged hits of this stage not yet emitted (wave-uniform)
       // One 32-row tile per iteration; not unrolled (code size).
 #pragma unroll 1
       for (int mt = 0; mt < SM; ++mt) {
@@ -543,6 +628,12 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
 #pragma unroll
           for (int nt = 0; nt < TN; ++nt)
             acc[nt] = mfma_chunk<E>(af, qf[nt][c], c == 0 ? ci : acc[nt], xsc, qsc[nt]);
+        }
+        if (!DENSE && pend != 0) {
+          // The previous tiles' hits, while this tile's MFMAs run.
+          __builtin_amdgcn_sched_barrier(0);
+          drain_hits(pend, ps, j0, lane, &qtail[par], put_par);
+          pend = 0;
         }
         if (DENSE) {
 #pragma unroll
@@ -566,9 +657,11 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
           mx[nt] = v;
           hit |= v >= thr[nt];
         }
-        if (__any(hit))
-          emit_hits<TN>(acc, mx, thr, qrow, j0 + mt * 32, lane, hitbuf[wave], &qtail[par],
-                        [&](unsigned t, int qr, float v, int64_t row) { put(par, t, qr, v, row); });
+        if (__any(hit)) stage_hits<TN>(acc, mx, thr, qrow, mt, j0, lane, pend, ps, &qtail[par], put_par);
+      }
+      if (!DENSE && pend != 0) {
+        drain_hits(pend, ps, j0, lane, &qtail[par], put_par);
+        pend = 0;
       }
       if (has_next) sstore(buf ^ 1);
       __syncthreads();
