@@ -74,7 +74,8 @@ def main() -> int:
     ap.add_argument("--copy-gib", type=float, default=1.0)
     ap.add_argument("--lsh-data", default="c2,uniform16", help="rplsh: workloads (c2, uniform16)")
     ap.add_argument("--lsh-n", type=int, default=1_000_000, help="rplsh: rows of the uniform16 workload")
-    ap.add_argument("--lsh-configs", default="", help="rplsh: tables:bits:probes:budget,... (default: built-in list)")
+    ap.add_argument("--lsh-configs", default="",
+                    help="rplsh: tables:bits:probes:budget[:prerank],... (default: built-in list)")
     ap.add_argument("--rate", type=float, default=400.0, help="serving: Poisson arrival rate (req/s)")
     ap.add_argument("--requests", type=int, default=4096, help="serving: number of requests")
     ap.add_argument("--indexes", type=int, default=64, help="serving: resident indexes")
@@ -520,9 +521,11 @@ def serving(args, ca) -> int:
     return 0
 
 
-# (tables, bits, probes, max_candidates) of the IndexRPLSH configurations timed
-RPLSH_CONFIGS = [(8, 12, 0, 512), (16, 14, 2, 2048), (32, 12, 1, 4096), (16, 20, 2, 1024),
-                 (32, 20, 4, 2048), (32, 24, 8, 2048), (64, 22, 4, 4096)]
+# (tables, bits, probes, max_candidates, prerank) of the IndexRPLSH configurations
+# timed; prerank f > 0 scores the candidates in BF16 and re-ranks only the f*k best
+RPLSH_CONFIGS = [(8, 12, 0, 512, 0), (16, 14, 2, 2048, 0), (32, 12, 1, 4096, 0), (16, 20, 2, 1024, 0),
+                 (32, 20, 4, 2048, 0), (32, 24, 8, 2048, 0), (64, 22, 4, 4096, 0),
+                 (32, 20, 4, 2048, 8), (32, 24, 8, 2048, 8), (64, 22, 4, 4096, 8), (64, 22, 4, 4096, 16)]
 
 
 def rplsh_data(ca, name: str, n: int, nq: int):
@@ -582,15 +585,17 @@ def rplsh_front_end(args, ca) -> int:
         rows = []
         configs = ([tuple(int(v) for v in c.split(":")) for c in args.lsh_configs.split(",")]
                    if args.lsh_configs else RPLSH_CONFIGS)
-        for L, K, P, B in configs:
+        for cfg in configs:
+            L, K, P, B = cfg[:4]
+            f = cfg[4] if len(cfg) > 4 else 0
             ix = ca.IndexRPLSH(res, d, ca.IndexRPLSHConfig(tables=L, bits=K, probes=P, max_candidates=B,
-                                                           seed=1))
+                                                           seed=1, prerank=f))
             t0 = time.perf_counter()
             ix.add(x)
             build_s = time.perf_counter() - t0
             ms, I = timed(ix, "ms_total")
             st = ix.last_stats()
-            r = {"tables": L, "bits": K, "probes": P, "max_candidates": B, "ms": round(ms, 3),
+            r = {"tables": L, "bits": K, "probes": P, "max_candidates": B, "prerank": f, "ms": round(ms, 3),
                  "qps": round(nq / ms * 1e3, 1), "recall_at_10": round(float(ca_recall(I, truth)), 5),
                  "raw_per_query": round(st["raw_candidates"] / nq, 1),
                  "distinct_per_query": round(st["distinct_candidates"] / nq, 1),

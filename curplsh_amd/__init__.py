@@ -203,7 +203,8 @@ class IndexRPLSHConfig:
     entries taken per query before de-duplication (<= 4096); seed: projection
     seed (standard normal, the library's generator); center: hyperplane
     offsets at the projections' medians over the first add's first 65536 rows
-    (L2 proximity) instead of through the origin (angle)."""
+    (L2 proximity) instead of through the origin (angle); prerank: f > 0 keeps
+    the f*k candidates with the smallest BF16 distance for the exact re-rank."""
 
     device: int = 0
     tables: int = 16
@@ -212,6 +213,7 @@ class IndexRPLSHConfig:
     max_candidates: int = 1024
     seed: int = 1234
     center: bool = True
+    prerank: int = 0
 
     def to_c(self) -> _lib.LshConfig:
         c = _lib.LshConfig()

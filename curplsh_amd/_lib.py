@@ -148,6 +148,7 @@ class LshConfig(C.Structure):
         ("max_candidates", C.c_int),
         ("seed", C.c_uint64),
         ("center", C.c_int),
+        ("prerank", C.c_int),
     ]
 
 

@@ -315,6 +315,113 @@ __global__ __launch_bounds__(kLshBlock) void lsh_gather_kernel(LshGatherArgs a) 
   }
 }
 
+// BF16 pre-rank of the distinct candidates (first stage when `prerank` is
+// set): per query (one workgroup), rel_b = fmaf(-2, dot_b, ||x||^2), written
+// as (rel_b, index) words in candidate order (empty slots stay empty).
+// dot_b is the product of bf16(q) and bf16(x) (both RNE, zero padded to ldb =
+// d rounded up to 8) in piece/tree order: the P = ldb / 8 pieces of 8 elements
+// are spread over G = P rounded up to a power of two lanes (P <= 32: the
+// dimension is at most 256), lane p chaining fmaf over the 8 elements of piece p
+// from +0 in element order (lanes past P keep +0); the G partial sums are added pairwise,
+// s[p] += s[p + m] for m = G/2, ..., 1. On the GPU that sum is a butterfly
+// (xor m) that leaves every lane of the group with the same bits, because
+// addition commutes. G lanes read one row, 16 B each, so one load instruction
+// covers 64 / G rows (1 KB, 8 cache lines, when G = 16): the gather runs at
+// the rate of the exact re-rank's in half the bytes, and every lane works.
+// Each wave owns a quarter of the candidate list and walks it in chunks of 64
+// (one candidate word per lane), issuing the chunk's row loads in batches of
+// up to 16 before using them.
+__device__ __forceinline__ float bf16_piece_dot(const uint4 xv, const float (&qf)[8], float acc) {
+  const uint32_t h[4] = {xv.x, xv.y, xv.z, xv.w};
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const float e = __uint_as_float(t & 1 ? h[t >> 1] & 0xffff0000u : h[t >> 1] << 16);
+    acc = fmaf(qf[t], e, acc);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ void bf16_piece_unpack(const uint4 v, float (&f)[8]) {
+  const uint32_t h[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int t = 0; t < 8; ++t) f[t] = __uint_as_float(t & 1 ? h[t >> 1] & 0xffff0000u : h[t >> 1] << 16);
+}
+
+template <int G>
+__global__ __launch_bounds__(kLshBlock) void lsh_prerank_kernel(const uint16_t* __restrict__ xb, int64_t ldb,
+                                                                const float* __restrict__ xn, int pieces,
+                                                                const uint16_t* __restrict__ qb,
+                                                                const uint64_t* __restrict__ cand, int B, int nq,
+                                                                uint64_t* __restrict__ out) {
+  constexpr int R = 64 / G;             // rows per load instruction
+  constexpr int NL = G < 16 ? G : 16;   // loads issued before use
+  __shared__ float sdot[kLshBlock];
+  const int q = blockIdx.x;
+  if (q >= nq) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int p = lane % G, grp = lane / G;
+  const bool live = p < pieces;
+  const int pc = live ? p : 0;  // lanes past the row re-read piece 0 and keep +0
+  float qf[8];
+  bf16_piece_unpack(*reinterpret_cast<const uint4*>(qb + (int64_t)q * ldb + 8 * pc), qf);
+  const int per = (B + 3) / 4;
+  const int c0 = min(B, w * per), c1 = min(B, c0 + per);
+  const uint64_t* src = cand + (int64_t)q * B;
+  uint64_t* dst = out + (int64_t)q * B;
+  float* const wdot = sdot + w * 64;
+  for (int cb = c0; cb < c1; cb += 64) {
+    const int nc = min(64, c1 - cb);
+    const uint64_t word = lane < nc ? src[cb + lane] : kEmptyCand;
+    const int32_t myidx = word != kEmptyCand ? cand_idx(word) : -1;
+    const uint64_t present = __ballot(myidx >= 0);
+    if (present == 0) {
+      if (lane < nc) dst[cb + lane] = kEmptyCand;
+      continue;
+    }
+    const int32_t any_row = __shfl(myidx, __ffsll((unsigned long long)present) - 1, 64);
+    const int32_t rowi = myidx >= 0 ? myidx : any_row;
+    const float xnr = xn[rowi];
+#pragma unroll
+    for (int sb = 0; sb < G / NL; ++sb) {
+      uint4 v[NL];
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        const int64_t ri = __shfl(rowi, (sb * NL + l) * R + grp, 64);
+        v[l] = *reinterpret_cast<const uint4*>(xb + ri * ldb + 8 * pc);
+      }
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        float acc = bf16_piece_dot(v[l], qf, 0.0f);
+        acc = live ? acc : 0.0f;
+#pragma unroll
+        for (int m = G / 2; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 64);
+        if (p == 0) wdot[(sb * NL + l) * R + grp] = acc;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < nc) dst[cb + lane] = myidx >= 0 ? pack_cand(fmaf(-2.0f, wdot[lane], xnr), myidx) : kEmptyCand;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+void launch_prerank(const uint16_t* xb, int64_t ldb, const float* xn, const uint16_t* qb, const uint64_t* cand,
+                    int B, int nq, uint64_t* out, hipStream_t st) {
+  const int pieces = (int)(ldb / 8);  // <= 32: the index's dimension is at most 256
+  if (pieces < 1 || pieces > 32) fail(RPLSH_ERR_UNSUPPORTED, "prerank: row of %d pieces (at most 32)", pieces);
+  int G = 1;
+  while (G < pieces) G <<= 1;
+  const dim3 grid((unsigned)nq), block(kLshBlock);
+  switch (G) {
+    case 1: lsh_prerank_kernel<1><<<grid, block, 0, st>>>(xb, ldb, xn, pieces, qb, cand, B, nq, out); break;
+    case 2: lsh_prerank_kernel<2><<<grid, block, 0, st>>>(xb, ldb, xn, pieces, qb, cand, B, nq, out); break;
+    case 4: lsh_prerank_kernel<4><<<grid, block, 0, st>>>(xb, ldb, xn, pieces, qb, cand, B, nq, out); break;
+    case 8: lsh_prerank_kernel<8><<<grid, block, 0, st>>>(xb, ldb, xn, pieces, qb, cand, B, nq, out); break;
+    case 16: lsh_prerank_kernel<16><<<grid, block, 0, st>>>(xb, ldb, xn, pieces, qb, cand, B, nq, out); break;
+    default: lsh_prerank_kernel<32><<<grid, block, 0, st>>>(xb, ldb, xn, pieces, qb, cand, B, nq, out); break;
+  }
+  RPLSH_HIP(hipGetLastError());
+}
+
 }  // namespace
 }  // namespace rplsh
 
@@ -327,6 +434,8 @@ struct rplsh_lsh {
   int64_t n = 0, capacity = 0, ldx = 0;
   float* x = nullptr;         // [capacity][ldx], zero padded
   float* xn = nullptr;        // [capacity]
+  uint16_t* xb = nullptr;     // [capacity][ldb] BF16 rows (pre-rank only)
+  int64_t ldb = 0;
   float* w = nullptr;         // [L*K][dim]
   float* wj = nullptr;        // [L][dim][KB]: W j-major per table, zero padded to KB (hash kernel)
   float* off = nullptr;       // [L*K] hyperplane offsets (0, or medians set at the first add)
@@ -335,7 +444,8 @@ struct rplsh_lsh {
   uint32_t* sorted = nullptr; // [L][capacity]
   int32_t* dir = nullptr;     // [L][2^min(K,16) + 1] bucket directory
   int32_t* perm = nullptr;    // [L][capacity]
-  DevBuf iota, sort_tmp, ctr_dots, ctr_sorted, ctr_seg, qstage, qpad, qn, qcodes, qmarg, cand, idev, ddev, counters, status, relk, fp64;
+  DevBuf iota, sort_tmp, ctr_dots, ctr_sorted, ctr_seg, qstage, qpad, qn, qbf, scored, topk, ktau, kbound,
+      kcount, qcodes, qmarg, cand, idev, ddev, counters, status, relk, fp64;
   rplsh_lsh_stats last{};
 };
 
@@ -349,9 +459,10 @@ hipStream_t lsh_stream(rplsh_lsh* h) {
 }
 
 void lsh_free(rplsh_lsh* h) {
-  for (void* p : {(void*)h->x, (void*)h->xn, (void*)h->codes, (void*)h->sorted, (void*)h->perm})
+  for (void* p : {(void*)h->x, (void*)h->xn, (void*)h->xb, (void*)h->codes, (void*)h->sorted, (void*)h->perm})
     if (p) RPLSH_HIP(hipFree(p));
   h->x = h->xn = nullptr;
+  h->xb = nullptr;
   h->codes = h->sorted = nullptr;
   h->perm = nullptr;
   h->n = h->capacity = 0;
@@ -460,6 +571,7 @@ void rplsh_lsh_config_default(rplsh_lsh_config* cfg) {
   cfg->max_candidates = 1024;
   cfg->seed = 1234;
   cfg->center = 1;
+  cfg->prerank = 0;
 }
 
 rplsh_status rplsh_lsh_create(rplsh_resources* res, int dim, const rplsh_lsh_config* cfg, rplsh_lsh** out) {
@@ -480,6 +592,7 @@ rplsh_status rplsh_lsh_create(rplsh_resources* res, int dim, const rplsh_lsh_con
                   "(probes + 1) * tables must be <= %d", kLshMaxEntries);
     RPLSH_REQUIRE(c.max_candidates >= 1 && c.max_candidates <= kLshMaxBudget,
                   "max_candidates must be in [1, %d]", kLshMaxBudget);
+    RPLSH_REQUIRE(c.prerank >= 0 && c.prerank <= 64, "prerank must be in [0, 64]");
     if (rplsh_resources_init_device(res, c.device) != RPLSH_OK)
       fail(RPLSH_ERR_STATE, "%s", rplsh_last_error());
     DeviceScope scope(c.device);
@@ -488,6 +601,7 @@ rplsh_status rplsh_lsh_create(rplsh_resources* res, int dim, const rplsh_lsh_con
     h->dim = dim;
     h->cfg = c;
     h->ldx = round_up(dim, 4);
+    h->ldb = round_up(dim, 8);
     // matrixRandomInitialize (matrix_ops.cu:71-76) with mean 0, stddev 1: the
     // signs are what matter, so the scale is left at 1.
     const int64_t rows = (int64_t)c.tables * c.bits;
@@ -561,6 +675,8 @@ rplsh_status rplsh_lsh_add(rplsh_lsh* h, int64_t num, const float* data) {
       float *nx = nullptr, *nn = nullptr;
       uint32_t *nc = nullptr, *ns = nullptr;
       int32_t* np = nullptr;
+      uint16_t* nb = nullptr;
+      if (h->cfg.prerank > 0) RPLSH_HIP(hipMalloc(&nb, (size_t)cap * h->ldb * sizeof(uint16_t)));
       RPLSH_HIP(hipMalloc(&nx, (size_t)cap * h->ldx * sizeof(float)));
       RPLSH_HIP(hipMalloc(&nn, (size_t)cap * sizeof(float)));
       RPLSH_HIP(hipMalloc(&nc, (size_t)cap * L * sizeof(uint32_t)));
@@ -569,6 +685,7 @@ rplsh_status rplsh_lsh_add(rplsh_lsh* h, int64_t num, const float* data) {
       if (h->n > 0) {
         RPLSH_HIP(hipMemcpyAsync(nx, h->x, (size_t)h->n * h->ldx * sizeof(float), hipMemcpyDeviceToDevice, st));
         RPLSH_HIP(hipMemcpyAsync(nn, h->xn, (size_t)h->n * sizeof(float), hipMemcpyDeviceToDevice, st));
+        if (nb) RPLSH_HIP(hipMemcpyAsync(nb, h->xb, (size_t)h->n * h->ldb * sizeof(uint16_t), hipMemcpyDeviceToDevice, st));
         RPLSH_HIP(hipMemcpy2DAsync(nc, cap * sizeof(uint32_t), h->codes, h->capacity * sizeof(uint32_t),
                                    h->n * sizeof(uint32_t), L, hipMemcpyDeviceToDevice, st));
       }
@@ -577,6 +694,7 @@ rplsh_status rplsh_lsh_add(rplsh_lsh* h, int64_t num, const float* data) {
       lsh_free(h);
       h->x = nx;
       h->xn = nn;
+      h->xb = nb;
       h->codes = nc;
       h->sorted = ns;
       h->perm = np;
@@ -592,6 +710,7 @@ rplsh_status rplsh_lsh_add(rplsh_lsh* h, int64_t num, const float* data) {
                                  h->dim * sizeof(float), num, hipMemcpyDefault, st));
     }
     launch_l2norm(dst, num, h->dim, h->ldx, h->xn + h->n, 1, st);
+    if (h->xb) launch_encode_bf16(dst, num, h->dim, h->ldx, (int)h->ldb, h->xb + h->n * h->ldb, st);
     if (h->cfg.center && !h->off_set) set_offsets(h, dst, num, st);
     h->off_set = true;
     launch_hash(h, dst, num, h->codes + h->n, h->capacity, nullptr, 0, 0, st);
@@ -674,8 +793,34 @@ rplsh_status rplsh_lsh_search(rplsh_lsh* h, int64_t nq, const float* queries, in
       ra.q = qp;
       ra.ldq = h->ldx;
       ra.qnorm = qn;
+      const int kpre = h->cfg.prerank > 0 ? (int)std::min<int64_t>((int64_t)h->cfg.prerank * k, B) : B;
+      if (kpre < B) {
+        // BF16 pre-rank of the B candidates, then the kpre smallest words
+        uint16_t* qbf = h->qbf.as<uint16_t>((size_t)nq * h->ldb);
+        launch_encode_bf16(qp, nq, h->dim, h->ldx, (int)h->ldb, qbf, st);
+        uint64_t* scored = h->scored.as<uint64_t>((size_t)nq * B);
+        launch_prerank(h->xb, h->ldb, h->xn, qbf, cand, B, (int)nq, scored, st);
+        int* kcount = h->kcount.as<int>((size_t)nq);
+        float* ktau = h->ktau.as<float>((size_t)nq);
+        float* kbound = h->kbound.as<float>((size_t)nq);
+        uint64_t* top = h->topk.as<uint64_t>((size_t)nq * kpre);
+        launch_fill_i32(kcount, nq, B, st);
+        launch_fill_f32(ktau, nq, INFINITY, st);
+        SelectArgs sa{};
+        sa.cand = scored;
+        sa.counts = kcount;
+        sa.cap = B;
+        sa.nq = (int)nq;
+        sa.K = kpre;
+        sa.topk_mode = 1;
+        sa.tau_in = ktau;
+        sa.topk_out = top;
+        sa.bound_out = kbound;
+        launch_select(sa, st);
+        cand = top;
+      }
       ra.topk = cand;
-      ra.kp = B;
+      ra.kp = kpre;
       ra.k = k;
       ra.nq = (int)nq;
       ra.exact = 1;

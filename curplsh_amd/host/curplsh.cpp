@@ -165,6 +165,7 @@ IndexRPLSH::IndexRPLSH(DeviceResources* resources, int dim, IndexRPLSHConfig con
   c.max_candidates = config.maxCandidates;
   c.seed = config.seed;
   c.center = config.center ? 1 : 0;
+  c.prerank = config.prerankFactor;
   host_assert(config.memorySpace == MemorySpace::Device);
   checkStatus(rplsh_lsh_create(resources->handle(), dim, &c, &handle_), "IndexRPLSH");
   isTrained_ = true;

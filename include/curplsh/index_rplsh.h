@@ -25,7 +25,13 @@ namespace curplsh {
 
 struct IndexRPLSHConfig : public IndexConfig {
   IndexRPLSHConfig()
-      : numTables(16), numBits(16), numProbes(0), maxCandidates(1024), seed(1234), center(true) {}
+      : numTables(16),
+        numBits(16),
+        numProbes(0),
+        maxCandidates(1024),
+        seed(1234),
+        center(true),
+        prerankFactor(0) {}
   /// Hash tables.
   int numTables;
   /// Code bits per table (1..24).
@@ -39,6 +45,9 @@ struct IndexRPLSHConfig : public IndexConfig {
   /// Hyperplanes through the projections' medians over the first add (L2
   /// proximity) rather than through the origin (angle).
   bool center;
+  /// 0: exact re-rank of every distinct candidate; f > 0: a BF16 pre-rank keeps
+  /// the f * k nearest of them for the exact re-rank.
+  int prerankFactor;
 };
 
 class IndexRPLSH : public Index {

@@ -292,7 +292,8 @@ rplsh_status rplsh_bf_clear_tuning(rplsh_bf* index);
  *    the bucket with the bit of its p-th smallest |dot - offset| flipped
  *    (ties: lower bit); buckets are concatenated in (probe, table) order, cut after
  *    max_candidates entries and de-duplicated;
- *  - the distinct candidates are re-ranked exactly (canonical FP32, as the
+ *  - the distinct candidates (or, with `prerank`, the prerank*k of them with
+ *    the smallest BF16 distance) are re-ranked exactly (canonical FP32, as the
  *    plain path): k smallest by (distance, index), padded with -1 / FLT_MAX. */
 typedef struct rplsh_lsh_config {
   int device;
@@ -303,10 +304,14 @@ typedef struct rplsh_lsh_config {
   int max_candidates;  /* bucket entries taken per query before de-duplication, 1..4096 */
   uint64_t seed;       /* projection seed */
   int center;          /* 1: median offsets (L2 proximity), 0: through the origin (angle) */
+  int prerank;         /* 0: exact re-rank of every distinct candidate; f > 0: a BF16
+                          pre-rank (RNE-encoded rows and query, canonical FP32 fmaf chain,
+                          rel = fmaf(-2, dot, ||x||^2)) keeps the f*k smallest by
+                          (rel, index) for the exact re-rank; 0..64 */
 } rplsh_lsh_config;
 
 /* Fills the defaults: device 0, 16 tables x 16 bits, no extra probes,
- * 1024 candidates, seed 1234, median offsets. */
+ * 1024 candidates, seed 1234, median offsets, no pre-rank. */
 void rplsh_lsh_config_default(rplsh_lsh_config* cfg);
 
 typedef struct rplsh_lsh rplsh_lsh;

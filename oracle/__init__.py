@@ -18,6 +18,7 @@ reference repository):
                     recall and distance error for every mode)
   recall            Dataset::evaluate, include/dataset.h:135-152
   bf16 / e4m3       encoders from IEEE-754 and the OCP OFP8 spec
+  bf16_tree_dot     the LSH pre-rank's BF16 dot product (piece/tree order)
   lsh_hash /        the library's random-projection LSH front end (IndexRPLSH);
   lsh_search        the reference has no LSH step (only the Gaussian
                     initialiser of src/internal/matrix_ops.cu:71-76), so this
@@ -32,6 +33,6 @@ candidate sets rebuilt by set membership and by an explicit bucket
 concatenation in numpy.
 """
 from .oracle import (  # noqa: F401
-    LIB_PATH, bf16_rne, bf16_to_float, e4m3_rne_sat, e4m3_to_float, load, lsh_hash, lsh_search,
+    LIB_PATH, bf16_rne, bf16_to_float, bf16_tree_dot, e4m3_rne_sat, e4m3_to_float, load, lsh_hash, lsh_search,
     norms_canonical, pair_dist_fp64, recall, search_canonical, search_fp64,
 )

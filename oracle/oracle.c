@@ -443,8 +443,37 @@ static uint64_t probe_key(float m, int b) {
   return ((uint64_t)f2u(fabsf(m)) << 5) | (uint64_t)b;
 }
 
+/* BF16 dot product in piece/tree order (the LSH pre-rank's score; the order
+ * is this library's own, chosen so that G lanes can share a row):
+ * both vectors are rounded to BF16 (RNE) and zero padded to P = ceil(d / 8)
+ * pieces of 8 elements; G = min(64, P rounded up to a power of two) partial
+ * sums, sum p chaining fmaf over the elements of pieces p, p + G, ... in
+ * element order from +0 (sums past P stay +0); then s[p] += s[p + m] for
+ * m = G/2, ..., 1 (p < m), and the result is s[0]. */
+float oracle_bf16_tree_dot(const float* a, const float* b, int d) {
+  const int P = (d + 7) / 8;
+  int G = 1;
+  while (G < P && G < 64) G <<= 1;
+  float s[64];
+  for (int p = 0; p < G; ++p) {
+    float acc = 0.0f;
+    for (int c = p; c < P; c += G) {
+      for (int t = 0; t < 8; ++t) {
+        const int j = 8 * c + t;
+        const float u = j < d ? oracle_bf16_to_float(oracle_bf16_rne(a[j])) : 0.0f;
+        const float v = j < d ? oracle_bf16_to_float(oracle_bf16_rne(b[j])) : 0.0f;
+        acc = fmaf(u, v, acc);
+      }
+    }
+    s[p] = acc;
+  }
+  for (int m = G / 2; m >= 1; m >>= 1)
+    for (int p = 0; p < m; ++p) s[p] = s[p] + s[p + m];
+  return s[0];
+}
+
 void oracle_lsh_search(const float* x, int64_t n, int d, const float* w, const float* off, int L,
-                       int K, int probes, int budget, const float* q, int64_t nq, int k,
+                       int K, int probes, int budget, int prerank, const float* q, int64_t nq, int k,
                        int32_t* out_idx, float* out_dist, int64_t* raw_total,
                        int64_t* distinct_total) {
   if (raw_total) *raw_total = 0;
@@ -510,21 +539,36 @@ void oracle_lsh_search(const float* x, int64_t n, int d, const float* w, const f
     }
     raw += R;
     qsort(buf, (size_t)R, sizeof(int32_t), i32_cmp);
+    int nd = 0;  /* distinct rows, compacted to the front of buf */
+    for (int j = 0; j < R; ++j)
+      if (j == 0 || buf[j] != buf[j - 1]) buf[nd++] = buf[j];
+    distinct += nd;
+    const float* qv = q + qi * (int64_t)d;
+    int nc = nd;
+    const int kpre = prerank > 0 ? (prerank * k < budget ? prerank * k : budget) : budget;
+    if (prerank > 0 && kpre < budget && nd > kpre) {
+      /* BF16 pre-rank: keep the kpre smallest (rel_b, index) */
+      kv32* pre = (kv32*)malloc(sizeof(kv32) * (size_t)nd);
+      for (int j = 0; j < nd; ++j) {
+        pre[j].key = fmaf(-2.0f, oracle_bf16_tree_dot(qv, x + (int64_t)buf[j] * d, d), xn[buf[j]]);
+        pre[j].idx = buf[j];
+      }
+      qsort(pre, (size_t)nd, sizeof(kv32), kv32_cmp);
+      for (int j = 0; j < kpre; ++j) buf[j] = pre[j].idx;
+      nc = kpre;
+      free(pre);
+    }
     kv32* h = (kv32*)malloc(sizeof(kv32) * (size_t)k);
     for (int j = 0; j < k; ++j) {
       h[j].key = INFINITY;
       h[j].idx = INT32_MAX;
     }
-    int nd = 0;
-    const float* qv = q + qi * (int64_t)d;
-    for (int j = 0; j < R; ++j) {
-      if (j > 0 && buf[j] == buf[j - 1]) continue;
-      ++nd;
+    for (int j = 0; j < nc; ++j) {
       const int32_t r = buf[j];
       const float rel = fmaf(-2.0f, oracle_dot_canonical(qv, x + (int64_t)r * d, d), xn[r]);
       heap32_push(h, k, rel, r);
     }
-    distinct += nd;
+    nd = nc;
     qsort(h, (size_t)k, sizeof(kv32), kv32_cmp);
     for (int j = 0; j < k; ++j) {
       const int64_t o = qi * (int64_t)k + j;

@@ -80,15 +80,24 @@ float oracle_e4m3_to_float(uint8_t b);
 void oracle_lsh_hash(const float* x, int64_t n, int d, const float* w, const float* off, int L,
                      int K, uint32_t* codes, float* margins);
 
+/* BF16 dot product of a and b (length d) in the LSH pre-rank's piece/tree
+ * order: P = ceil(d/8) zero-padded pieces of 8, G = min(64, pow2 >= P)
+ * partial fmaf chains (chain p: pieces p, p+G, ...), then s[p] += s[p+m] for
+ * m = G/2 .. 1; returns s[0]. */
+float oracle_bf16_tree_dot(const float* a, const float* b, int d);
+
 /* LSH search: tables = rows stably sorted by code; per query, buckets of
  * (probe p = 0..probes, table t) in that order -- p = 0 the query's code,
  * p >= 1 the code with the bit of the p-th smallest (|dot - off| bits, bit)
  * key flipped -- concatenated and cut after `budget` entries, de-duplicated,
  * then rel = fmaf(-2, dot, ||x||^2) over the distinct rows, k smallest by
- * (rel, index), D = rel + ||q||^2, padded with -1 / FLT_MAX. raw_total /
+ * (rel, index), D = rel + ||q||^2, padded with -1 / FLT_MAX. prerank f > 0
+ * (when f*k < budget): first keep the f*k distinct rows with the smallest
+ * (rel_b, index), rel_b = fmaf(-2, dot_b, ||x||^2) with dot_b =
+ * oracle_bf16_tree_dot(q, x, d). raw_total /
  * distinct_total (may be NULL): entries taken and distinct rows, summed. */
 void oracle_lsh_search(const float* x, int64_t n, int d, const float* w, const float* off, int L,
-                       int K, int probes, int budget, const float* q, int64_t nq, int k,
+                       int K, int probes, int budget, int prerank, const float* q, int64_t nq, int k,
                        int32_t* out_idx, float* out_dist, int64_t* raw_total,
                        int64_t* distinct_total);
 

@@ -45,10 +45,12 @@ def load() -> C.CDLL:
     lib.oracle_e4m3_rne_sat.restype = C.c_uint8
     lib.oracle_e4m3_to_float.argtypes = [C.c_uint8]
     lib.oracle_e4m3_to_float.restype = C.c_float
+    lib.oracle_bf16_tree_dot.argtypes = [_vp, _vp, _i32]
+    lib.oracle_bf16_tree_dot.restype = C.c_float
     lib.oracle_lsh_hash.argtypes = [_vp, _i64, _i32, _vp, _vp, _i32, _i32, _vp, _vp]
     lib.oracle_lsh_hash.restype = None
-    lib.oracle_lsh_search.argtypes = [_vp, _i64, _i32, _vp, _vp, _i32, _i32, _i32, _i32, _vp,
-                                      _i64, _i32, _vp, _vp, _vp, _vp]
+    lib.oracle_lsh_search.argtypes = [_vp, _i64, _i32, _vp, _vp, _i32, _i32, _i32, _i32, _i32,
+                                      _vp, _i64, _i32, _vp, _vp, _vp, _vp]
     lib.oracle_lsh_search.restype = None
     _lib = lib
     return lib
@@ -117,6 +119,13 @@ def bf16_to_float(h: int) -> float:
     return float(load().oracle_bf16_to_float(int(h)))
 
 
+def bf16_tree_dot(a: np.ndarray, b: np.ndarray) -> np.float32:
+    """The LSH pre-rank's BF16 dot product (piece/tree order) of two vectors."""
+    a, b = _f32(a), _f32(b)
+    assert a.shape == b.shape and a.ndim == 1
+    return np.float32(load().oracle_bf16_tree_dot(a.ctypes.data, b.ctypes.data, a.shape[0]))
+
+
 def e4m3_rne_sat(v: float) -> int:
     return int(load().oracle_e4m3_rne_sat(float(v)))
 
@@ -141,7 +150,7 @@ def lsh_hash(x: np.ndarray, w: np.ndarray, L: int, K: int, off: np.ndarray | Non
 
 
 def lsh_search(x: np.ndarray, w: np.ndarray, L: int, K: int, probes: int, budget: int,
-               q: np.ndarray, k: int, off: np.ndarray | None = None):
+               q: np.ndarray, k: int, off: np.ndarray | None = None, prerank: int = 0):
     """LSH search restatement: (indices, distances, raw entries, distinct rows)."""
     q = _f32(q)
     d = q.shape[1]
@@ -152,7 +161,7 @@ def lsh_search(x: np.ndarray, w: np.ndarray, L: int, K: int, probes: int, budget
     D = np.empty((nq, k), np.float32)
     raw, dist = C.c_int64(0), C.c_int64(0)
     load().oracle_lsh_search(x.ctypes.data, x.shape[0], d, w.ctypes.data,
-                             o.ctypes.data if o is not None else None, L, K, probes, budget,
+                             o.ctypes.data if o is not None else None, L, K, probes, budget, prerank,
                              q.ctypes.data, nq, k, I.ctypes.data, D.ctypes.data, C.byref(raw),
                              C.byref(dist))
     return I, D, raw.value, dist.value

@@ -261,7 +261,7 @@ static int test_rplsh(DeviceResources& res) {
   index.search(nq, q.data(), k, I.data(), D.data());
   int64_t raw = 0, distinct = 0;
   oracle_lsh_search(x.data(), n, dim, w.data(), off.data(), cfg.numTables, cfg.numBits, cfg.numProbes,
-                    cfg.maxCandidates, q.data(), nq, k, Io.data(), Do.data(), &raw, &distinct);
+                    cfg.maxCandidates, 0, q.data(), nq, k, Io.data(), Do.data(), &raw, &distinct);
   CHECK(I == Io);
   CHECK(std::memcmp(D.data(), Do.data(), D.size() * sizeof(float)) == 0);
   const rplsh_lsh_stats st = index.lastStats();

@@ -83,6 +83,51 @@ def test_search_bitexact_vs_oracle(ca, orc, res, case):
     assert st["queries"] == nq
 
 
+PRE_CASES = [  # n, d, nq, k, tables, bits, probes, budget, prerank
+    (20000, 64, 200, 10, 8, 12, 3, 1024, 4),
+    (30000, 100, 97, 100, 16, 10, 2, 4096, 2),
+    (3000, 200, 20, 20, 6, 8, 4, 512, 3),     # d > 128, ragged BF16 rows
+    (4000, 33, 7, 10, 8, 10, 2, 300, 1),      # batch under 16 queries, ld padding
+    (2000, 17, 30, 50, 2, 14, 0, 1000, 4),    # fewer distinct candidates than f*k: padding
+    (3000, 5, 20, 5, 4, 6, 1, 200, 2),        # one piece per row: 64 rows per load
+    (3000, 60, 20, 10, 4, 8, 1, 400, 2),      # 8 pieces
+    (1500, 256, 12, 10, 4, 8, 1, 256, 2),     # 32 pieces: two rows per load
+]
+
+
+@pytest.mark.parametrize("case", PRE_CASES, ids=[f"n{c[0]}d{c[1]}q{c[2]}k{c[3]}P{c[6]}B{c[7]}f{c[8]}"
+                                                  for c in PRE_CASES])
+def test_prerank_bitexact_vs_oracle(ca, orc, res, case):
+    n, d, nq, k, L, K, P, B, f = case
+    x = ca.gen_mixture(n, d, 32, 0.2, 3, 14)
+    q = ca.gen_mixture(nq, d, 32, 0.2, 3, 15)
+    idx = make(ca, res, d, tables=L, bits=K, probes=P, max_candidates=B, seed=12, prerank=f)
+    idx.add(x[: n // 3])
+    idx.add(x[n // 3:])  # the BF16 copy grows with the index
+    off = idx.offsets()
+    I, D = idx.search(q, k)
+    Io, Do, raw, distinct = orc.lsh_search(x, idx.projections(), L, K, P, B, q, k, off, prerank=f)
+    bad = np.nonzero((I != Io).any(1) | (bits(D) != bits(Do)).any(1))[0]
+    assert bad.size == 0, f"{bad.size} rows differ, first {bad[:5]}"
+    st = idx.last_stats()
+    assert st["raw_candidates"] == raw and st["distinct_candidates"] == distinct
+
+
+def test_prerank_integer_data_equals_full_rerank(ca, res):
+    """Integer data is exact in BF16: the pre-rank keeps the true f*k nearest."""
+    rng = np.random.default_rng(5)
+    x = rng.integers(-8, 9, (20000, 48)).astype(np.float32)
+    q = rng.integers(-8, 9, (64, 48)).astype(np.float32)
+    cfg = dict(tables=8, bits=10, probes=2, max_candidates=2048, seed=4)
+    full = make(ca, res, 48, **cfg)
+    full.add(x)
+    I0, D0 = full.search(q, 10)
+    pre = make(ca, res, 48, prerank=2, **cfg)
+    pre.add(x)
+    I1, D1 = pre.search(q, 10)
+    assert np.array_equal(I1, I0) and np.array_equal(bits(D1), bits(D0))
+
+
 def test_all_buckets_equals_plain_brute_force(ca, res):
     x = ca.gen_uniform(3000, 48, 7, -1, 1)
     q = ca.gen_uniform(64, 48, 8, -1, 1)
@@ -149,6 +194,7 @@ def test_recall_on_clustered_data(ca, orc, res):
 
 def test_invalid_configs_rejected(ca, res):
     for kw in (dict(bits=0), dict(bits=25), dict(probes=17, bits=16), dict(max_candidates=5000),
-               dict(tables=100, probes=10, bits=12), dict(tables=0)):
+               dict(tables=100, probes=10, bits=12), dict(tables=0), dict(prerank=-1),
+               dict(prerank=65)):
         with pytest.raises(ca.RplshError):
             make(ca, res, 16, **kw)
