@@ -46,7 +46,8 @@ struct IndexRPLSHConfig : public IndexConfig {
   /// proximity) rather than through the origin (angle).
   bool center;
   /// 0: exact re-rank of every distinct candidate; f > 0: a BF16 pre-rank keeps
-  /// the f * k nearest of them for the exact re-rank.
+  /// the f * k nearest of them for the exact re-rank (BF16 distance as defined at
+  /// rplsh_lsh_config::prerank; 0..64).
   int prerankFactor;
 };
 

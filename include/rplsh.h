@@ -304,10 +304,15 @@ typedef struct rplsh_lsh_config {
   int max_candidates;  /* bucket entries taken per query before de-duplication, 1..4096 */
   uint64_t seed;       /* projection seed */
   int center;          /* 1: median offsets (L2 proximity), 0: through the origin (angle) */
-  int prerank;         /* 0: exact re-rank of every distinct candidate; f > 0: a BF16
-                          pre-rank (RNE-encoded rows and query, canonical FP32 fmaf chain,
-                          rel = fmaf(-2, dot, ||x||^2)) keeps the f*k smallest by
-                          (rel, index) for the exact re-rank; 0..64 */
+  int prerank;         /* 0: exact re-rank of every distinct candidate; f > 0 (when
+                          f*k < max_candidates): a BF16 pre-rank keeps the f*k smallest
+                          (rel_b, index) for the exact re-rank, rel_b = fmaf(-2, dot_b,
+                          ||x||^2) with dot_b the product of the RNE-encoded row and
+                          query in piece/tree order: P = ceil(dim/8) zero-padded pieces
+                          of 8, G = P rounded up to a power of two partial sums (sum p:
+                          fmaf over piece p's elements in order, from +0; sums past P
+                          are +0), then s[p] += s[p+m] for m = G/2 .. 1; result s[0].
+                          Keeps a BF16 copy of the rows (2*dim bytes per row); 0..64 */
 } rplsh_lsh_config;
 
 /* Fills the defaults: device 0, 16 tables x 16 bits, no extra probes,
