@@ -66,6 +66,33 @@ struct FilterArgs {
                           // slot j, counts are preset to m by the caller
   int* counts;            // [nq] survivors (may exceed cap: overflow)
   uint64_t* cand;         // [nq][cap]
+  // LDS kernels, full level after a carried sample level: tiles u (32 rows)
+  // with u < rank_tiles and tile_rank[u] < skip_tiles were scored by that
+  // level and are skipped (nullptr: none).
+  const int32_t* tile_rank;
+  int64_t rank_tiles;
+  int64_t skip_tiles;
+};
+
+// Sample copy of a packed index: whole 32-row tiles, sample tile s holding
+// tile (s * step) mod tiles of the index (step coprime with tiles, the number
+// of whole tiles), so every prefix is a spread-out set of distinct tiles.
+// Index row of sample row r:
+__host__ __device__ inline int64_t sample_source_row(int64_t r, int64_t step, int64_t tiles) {
+  return (int64_t)(((uint64_t)(r >> 5) * (uint64_t)step) % (uint64_t)tiles) * 32 + (r & 31);
+}
+
+// Carry of the last sample level's survivors into the full level's lists.
+struct CarryArgs {
+  uint64_t* cand;        // [nq][cap]: the sample level's lists, compacted in place
+  int* counts;           // [nq]: in the sample level's counts, out the carried counts
+  int cap;
+  int nq;
+  const int* nq_dev;     // if set: live queries (<= nq), read on device
+  const float* tau;      // [nq] the full level's threshold
+  int64_t step, tiles;   // sample copy mapping (sample_source_row)
+  int64_t skip_tiles;    // sample tiles the full level skips
+  int* status;           // [nq] kStatusOverflow when a list overflowed
 };
 
 struct SelectArgs {

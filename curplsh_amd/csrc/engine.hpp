@@ -96,12 +96,18 @@ struct DbView {
   int64_t ldenc = 0;
   int dvalid_bytes = 0;
   const float* xscale = nullptr;  // FP8 per-row scales
-  // Packed sample copy for the sample levels: row r is row (r * sstep) mod n,
-  // r < srows (0: none); same layout, with its own norm and scale arrays.
+  // Packed sample copy for the sample levels: whole tiles, row r is row
+  // sample_source_row(r, sstep, stiles) (device.hpp), r < srows (0: none);
+  // same layout, with its own norm and scale arrays. trank[u] (u <
+  // rank_tiles, the packed copy's tiles): the sample tile holding tile u, or
+  // INT32_MAX.
   const void* senc = nullptr;
   const float* sscale = nullptr;
   const float* snorm = nullptr;
   int64_t srows = 0;
+  int64_t sstep = 0, stiles = 0;
+  const int32_t* trank = nullptr;
+  int64_t rank_tiles = 0;
   // -||x||^2 / 2 per row, with one -inf entry (index *_inf) for rows past
   // the end: the C operand the ring kernels copy into LDS as is.
   const float* xnh = nullptr;
@@ -169,8 +175,11 @@ struct rplsh_bf {
   float xn_max = 0.0f;
   void* xenc = nullptr;
   float* xscale = nullptr;
-  // Packed sample copy (sample levels): rows (r * sstep) mod n for r < srows.
+  // Packed sample copy (sample levels): whole tiles, row r < srows is row
+  // sample_source_row(r, sstep, n / 32); trank: its inverse per packed tile.
   void* senc = nullptr;
+  int32_t* trank = nullptr;
+  int64_t trank_cap = 0;
   float* sscale = nullptr;
   float* snorm = nullptr;
   float* snh = nullptr;   // -snorm / 2, then one -inf

@@ -502,8 +502,18 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
   i32x4 pre[UPT];
   float pn = 0.0f;
   int pe = 127;
+  // Tile skipping: lane t < SM of every wave loads the rank of stage tile t
+  // with the stage (a vector load, retired in order with the data loads);
+  // sstore turns the ranks into a wave-uniform mask for that stage.
+  int32_t grk = INT32_MAX;
+  unsigned gskip = 0u;  // bit t: tile t of the stored stage is skipped
+  static_assert(SM <= 32, "one skip bit per stage tile");
 
   auto gload = [&](int64_t j0) {
+    if (a.tile_rank) {  // uniform; tiles past rank_tiles hold no rows (clamped)
+      const int64_t u = (j0 >> 5) + (lane < SM ? lane : 0);
+      grk = a.tile_rank[u < a.rank_tiles ? u : a.rank_tiles - 1];
+    }
     if (stride == 1 && j0 + SR <= r_end) {
       // Whole stage at stride 1: j0 is a multiple of 32, so the stage is SM
       // consecutive whole tiles, one contiguous run of STAGE bytes.
@@ -561,6 +571,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
       sn[buf][tid] = pn;
       se[buf][tid] = pe;
     }
+    gskip = (unsigned)__ballot(lane < SM && grk < a.skip_tiles);
   };
 
   unsigned head[2] = {0u, 0u};  // flushed up to here (same in every thread)
@@ -603,6 +614,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
     int buf = 0;
     for (int s = 0; j0 < r_end; j0 += SR, ++s) {
       const bool has_next = j0 + SR < r_end;
+      const unsigned skip = DENSE ? 0u : gskip;  // this stage's, set when it was stored
       if (has_next) gload(j0 + SR);
       const int par = s & 1;
       auto put_par = [&](unsigned t, int qr, float v, int64_t row) { put(par, t, qr, v, row); };
@@ -610,6 +622,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
       // One 32-row tile per iteration; not unrolled (code size).
 #pragma unroll 1
       for (int mt = 0; mt < SM; ++mt) {
+        if (skip & (1u << mt)) continue;  // scored by the carried sample level
         f32x16 ci;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
@@ -978,8 +991,9 @@ __global__ void encode_packed_kernel(const float* __restrict__ x, int64_t n, int
                    sub * (16 / PackTraits<E>::kBytes);
     i32x4 piece = {0, 0, 0, 0};
     if (row < n) {
-      // Source row: row itself, or (row * map_step) mod map_n for a sample copy.
-      const int64_t src = map_step > 0 ? (int64_t)(((uint64_t)row * (uint64_t)map_step) % (uint64_t)map_n) : row;
+      // Source row: row itself, or its index row for a sample copy (map_n =
+      // the index's whole tiles).
+      const int64_t src = map_step > 0 ? sample_source_row(row, map_step, map_n) : row;
       if (E == Elem::BF16) {
         uint16_t v[8];
 #pragma unroll
@@ -1098,16 +1112,16 @@ void encode_one(const float* x, int64_t n, int d, int64_t ldx, int64_t row_from,
   RPLSH_HIP(hipGetLastError());
 }
 
-// Sample copy: row r holds source row (r * step) mod n, r < sm.
+// Sample copy: row r < sm holds index row sample_source_row(r, step, tiles).
 template <Elem E, int DPAD>
-void encode_sample_one(const float* x, int64_t n, int d, int64_t ldx, int64_t sm, int64_t step,
+void encode_sample_one(const float* x, int64_t tiles, int d, int64_t ldx, int64_t sm, int64_t step,
                        char* out, const float* scales, hipStream_t st) {
   using G = Geometry<E, DPAD>;
   const int64_t t1 = div_up(sm, 32);
   const int64_t work = t1 * (G::TILE / 16);
   int64_t g = div_up(work, 256);
   if (g > 65536) g = 65536;
-  encode_packed_kernel<E, DPAD><<<(unsigned)g, 256, 0, st>>>(x, sm, d, ldx, 0, t1, out, scales, step, n);
+  encode_packed_kernel<E, DPAD><<<(unsigned)g, 256, 0, st>>>(x, sm, d, ldx, 0, t1, out, scales, step, tiles);
   RPLSH_HIP(hipGetLastError());
 }
 
@@ -1137,17 +1151,66 @@ auto dispatch(Elem e, int dpad, F&& f) {
 }
 
 __global__ void sample_rows_kernel(const float* __restrict__ xn, const float* __restrict__ xscale,
-                                   int64_t n, int64_t sm, int64_t step, float* __restrict__ snorm,
+                                   int64_t tiles, int64_t sm, int64_t step, float* __restrict__ snorm,
                                    float* __restrict__ snh, int64_t nh_inf,
                                    float* __restrict__ sscale) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r < sm) {
-    const int64_t src = (int64_t)(((uint64_t)r * (uint64_t)step) % (uint64_t)n);
+    const int64_t src = sample_source_row(r, step, tiles);
     snorm[r] = xn[src];
     snh[r] = -0.5f * xn[src];
     if (sscale) sscale[r] = xscale[src];
   }
   if (r == 0) snh[nh_inf] = -INFINITY;
+}
+
+// rank[u] = (u * step_inv) mod tiles -- the sample tile holding index tile u --
+// when below stiles, else INT32_MAX (also for u >= tiles: the partial tile).
+__global__ void tile_ranks_kernel(int64_t tiles, int64_t stiles, int64_t step_inv, int32_t* __restrict__ rank,
+                                  int64_t rank_tiles) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= rank_tiles) return;
+  int32_t v = INT32_MAX;
+  if (u < tiles) {
+    const int64_t s = (int64_t)(((uint64_t)u * (uint64_t)step_inv) % (uint64_t)tiles);
+    if (s < stiles) v = (int32_t)s;
+  }
+  rank[u] = v;
+}
+
+// One wave per query: keeps, in list order, the words whose row lies in the
+// first skip_tiles sample tiles and that pass the full level's own test on
+// acc = -key/2 (the filter wrote key = -2 acc, so both halvings are exact),
+// with the sample row replaced by its index row. Reads of a chunk precede its
+// writes, and writes never pass the chunk being read.
+__global__ __launch_bounds__(256) void carry_kernel(CarryArgs a) {
+  const int q = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (q >= a.nq || (a.nq_dev && q >= *a.nq_dev)) return;
+  const int count = a.counts[q];
+  const int n = count < a.cap ? count : a.cap;
+  const float thr = fmaxf(-0.5f * a.tau[q], -FLT_MAX);
+  const int64_t rlim = a.skip_tiles * 32;
+  uint64_t* list = a.cand + (int64_t)q * a.cap;
+  int kept = 0;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i = i0 + lane;
+    uint64_t w = i < n ? list[i] : kEmptyCand;
+    bool keep = false;
+    if (w != kEmptyCand) {
+      const int32_t r = cand_idx(w);
+      keep = -0.5f * cand_key(w) >= thr && r < rlim;
+      if (keep) w = pack_cand(cand_key(w), (int32_t)sample_source_row(r, a.step, a.tiles));
+    }
+    const uint64_t m = __ballot(keep);
+    const int pos = kept + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    if (keep) list[pos] = w;
+    kept += __popcll(m);
+  }
+  if (lane == 0) {
+    a.counts[q] = kept;
+    if (count > a.cap && a.status) atomicOr(a.status + q, (int)kStatusOverflow);
+  }
 }
 
 __global__ void neg_half_norms_kernel(const float* __restrict__ xn, int64_t from, int64_t n,
@@ -1160,11 +1223,41 @@ __global__ void neg_half_norms_kernel(const float* __restrict__ xn, int64_t from
 
 }  // namespace
 
-void launch_sample_rows(const float* xn, const float* xscale, int64_t n, int64_t sm, int64_t step,
+void launch_sample_rows(const float* xn, const float* xscale, int64_t tiles, int64_t sm, int64_t step,
                         float* snorm, float* snh, int64_t nh_inf, float* sscale, hipStream_t st) {
-  RPLSH_REQUIRE(sm > 0 && n > 0 && step > 0 && sm <= nh_inf, "sample rows: bad arguments");
-  sample_rows_kernel<<<(unsigned)div_up(sm, (int64_t)256), 256, 0, st>>>(xn, xscale, n, sm, step,
+  RPLSH_REQUIRE(sm > 0 && sm % 32 == 0 && tiles > 0 && sm <= 32 * tiles && step > 0 && sm <= nh_inf,
+                "sample rows: bad arguments");
+  sample_rows_kernel<<<(unsigned)div_up(sm, (int64_t)256), 256, 0, st>>>(xn, xscale, tiles, sm, step,
                                                                         snorm, snh, nh_inf, sscale);
+  RPLSH_HIP(hipGetLastError());
+}
+
+void launch_tile_ranks(int64_t tiles, int64_t stiles, int64_t step, int32_t* rank, int64_t rank_tiles,
+                       hipStream_t st) {
+  RPLSH_REQUIRE(tiles > 0 && stiles >= 0 && stiles <= tiles && rank_tiles >= tiles && tiles < INT32_MAX,
+                "tile ranks: bad arguments");
+  // step^-1 mod tiles (extended Euclid; step is coprime with tiles)
+  int64_t r0 = tiles, r1 = step % tiles, t0 = 0, t1 = 1;
+  while (r1 != 0) {
+    const int64_t qq = r0 / r1;
+    int64_t tmp = r0 - qq * r1;
+    r0 = r1;
+    r1 = tmp;
+    tmp = t0 - qq * t1;
+    t0 = t1;
+    t1 = tmp;
+  }
+  RPLSH_REQUIRE(r0 == 1 || tiles == 1, "tile ranks: step not coprime with the tile count");
+  const int64_t inv = ((t0 % tiles) + tiles) % tiles;
+  tile_ranks_kernel<<<(unsigned)div_up(rank_tiles, (int64_t)256), 256, 0, st>>>(tiles, stiles, inv, rank,
+                                                                                rank_tiles);
+  RPLSH_HIP(hipGetLastError());
+}
+
+void launch_carry(const CarryArgs& a, hipStream_t st) {
+  if (a.nq <= 0) return;
+  RPLSH_REQUIRE(a.cand && a.counts && a.tau && a.tiles > 0 && a.step > 0, "carry: bad arguments");
+  carry_kernel<<<(unsigned)div_up(a.nq, 4), 256, 0, st>>>(a);
   RPLSH_HIP(hipGetLastError());
 }
 
@@ -1204,7 +1297,7 @@ struct LaunchFilter {
 
 struct EncodeSample {
   const float* x;
-  int64_t n;
+  int64_t tiles;
   int d;
   int64_t ldx, sm, step;
   char* out;
@@ -1212,7 +1305,7 @@ struct EncodeSample {
   hipStream_t st;
   template <Elem E, int DPAD>
   int operator()() const {
-    encode_sample_one<E, DPAD>(x, n, d, ldx, sm, step, out, scales, st);
+    encode_sample_one<E, DPAD>(x, tiles, d, ldx, sm, step, out, scales, st);
     return 0;
   }
 };
@@ -1247,9 +1340,10 @@ void launch_encode_packed(Elem e, int dpad, const float* x, int64_t n, int d, in
   dispatch(e, dpad, EncodePacked{x, n, d, ldx, row_from, static_cast<char*>(out), scales, st});
 }
 
-void launch_encode_sample(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx, int64_t sm,
+void launch_encode_sample(Elem e, int dpad, const float* x, int64_t tiles, int d, int64_t ldx, int64_t sm,
                           int64_t step, void* out, const float* scales, hipStream_t st) {
-  dispatch(e, dpad, EncodeSample{x, n, d, ldx, sm, step, static_cast<char*>(out), scales, st});
+  RPLSH_REQUIRE(sm % 32 == 0 && sm <= 32 * tiles, "sample copy: bad size");
+  dispatch(e, dpad, EncodeSample{x, tiles, d, ldx, sm, step, static_cast<char*>(out), scales, st});
 }
 
 }  // namespace rplsh

@@ -48,18 +48,27 @@ enum FilterVariant : int {
 };
 int filter_packed_block_queries(Elem e, int dpad, int variant);
 void launch_filter_packed(Elem e, int dpad, int variant, const FilterArgs& a, hipStream_t st);
-// Encodes rows [row_from, n) (re-encoding the partial tile before row_from);
-// FP8 also writes the per-row power-of-two scales of rows [row_from, n).
-// Sample copy of n rows: row r < sm holds source row (r * step) mod n.
-// snorm[r] = xn[src], snh[r] = -xn[src] / 2, sscale[r] = xscale[src] (if
-// sscale), snh[nh_inf] = -inf.
-void launch_sample_rows(const float* xn, const float* xscale, int64_t n, int64_t sm, int64_t step,
+// Sample copy (sm rows, a multiple of 32, at most 32 * tiles) of an index
+// with `tiles` whole 32-row tiles: sample row r holds index row
+// sample_source_row(r, step, tiles). snorm[r] = xn[src], snh[r] = -xn[src] / 2,
+// sscale[r] = xscale[src] (if sscale), snh[nh_inf] = -inf.
+void launch_sample_rows(const float* xn, const float* xscale, int64_t tiles, int64_t sm, int64_t step,
                         float* snorm, float* snh, int64_t nh_inf, float* sscale, hipStream_t st);
+// rank[u] = s for the index tile u = (s * step) mod tiles held by sample tile
+// s < stiles; INT32_MAX for every other u < rank_tiles.
+void launch_tile_ranks(int64_t tiles, int64_t stiles, int64_t step, int32_t* rank, int64_t rank_tiles,
+                       hipStream_t st);
+// Compacts each query's list to the survivors of the full level's test
+// (acc = -key/2 >= max(-tau/2, -FLT_MAX)) from the first skip_tiles sample
+// tiles, their ids mapped to index rows; counts become the carried counts.
+void launch_carry(const CarryArgs& a, hipStream_t st);
 // nh[i] = -xn[i] / 2 for i in [from, n); nh[inf_at] = -inf.
 void launch_neg_half_norms(const float* xn, int64_t from, int64_t n, float* nh, int64_t inf_at,
                            hipStream_t st);
-void launch_encode_sample(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx, int64_t sm,
+void launch_encode_sample(Elem e, int dpad, const float* x, int64_t tiles, int d, int64_t ldx, int64_t sm,
                           int64_t step, void* out, const float* scales, hipStream_t st);
+// Encodes rows [row_from, n) (re-encoding the partial tile before row_from);
+// FP8 also writes the per-row power-of-two scales of rows [row_from, n).
 void launch_encode_packed(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx,
                           int64_t row_from, void* out, float* scales, hipStream_t st);
 

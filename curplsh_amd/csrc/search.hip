@@ -12,10 +12,14 @@
 //      database with growing density and then over all rows; after each
 //      sample level a radix select turns the rho-th smallest survivor into the
 //      next level's per-query threshold tau. The last level keeps every row
-//      with approximate relative distance <= tau (about 8 k' rows per query)
+//      with approximate relative distance <= tau (about c k' rows per query)
 //      and selects the k' best by (approximate key, index). This is exact
 //      whenever the last level keeps between k' and the capacity rows: a row
-//      the filter drops has key > tau >= every kept key;
+//      the filter drops has key > tau >= every kept key. Packed copies read
+//      the sample levels from a sample copy of whole tiles; the last sample
+//      level's survivors that pass the final threshold are carried into the
+//      full level's lists and the full level skips those tiles, so it keeps
+//      the same rows as a scan of every tile (LDS filter variants);
 //   3. re-rank: canonical FP32 distances of the k' candidates, top k;
 //      precision forms re-order the boundary band in FP64; the plain path
 //      certifies that the MFMA error margin cannot hide a better row;
@@ -130,10 +134,12 @@ static int64_t level_cap(const std::vector<Level>& L, int kp, int c_factor) {
 // instead of every stride-th row (its candidates only set a threshold, their
 // row numbers are never used). `dense`: tau is +inf for every query (first
 // pass), so every row survives and is written to its own slot.
+// `skip_tiles` > 0 (full level after a carried sample level): the tiles that
+// level scored are skipped and the lists already hold its carried survivors.
 static void run_filter(const DbView& db, const Level& lv, bool dense, const void* qenc,
                        const float* qscale, const float* tau, int nq, const int* nq_dev, int cap,
                        int* counts, uint64_t* cand, int64_t target_wgs, int variant,
-                       hipStream_t st) {
+                       int64_t skip_tiles, hipStream_t st) {
   FilterArgs fa{};
   fa.x = db.xenc;
   fa.ldx = db.ldenc;
@@ -169,12 +175,33 @@ static void run_filter(const DbView& db, const Level& lv, bool dense, const void
   fa.cap = cap;
   fa.counts = counts;
   fa.cand = cand;
-  launch_fill_i32(counts, nq, fa.dense ? (int)lv.m : 0, st);
+  if (skip_tiles > 0) {
+    RPLSH_REQUIRE(db.trank && db.rank_tiles > 0 && lv.stride == 1 && !fa.dense, "tile skip: bad level");
+    fa.tile_rank = db.trank;
+    fa.rank_tiles = db.rank_tiles;
+    fa.skip_tiles = skip_tiles;
+  } else {
+    launch_fill_i32(counts, nq, fa.dense ? (int)lv.m : 0, st);
+  }
   if (db.packed) {
     launch_filter_packed(db.elem, db.dpad, variant, fa, st);
   } else {
     launch_filter(db.elem, db.dpad, fa, st);
   }
+}
+
+// Carry: when the level before the full one read whole tiles of the sample
+// copy, its survivors that pass the full level's threshold start the full
+// level's lists and the full level skips those tiles (LDS variants); the full
+// level then keeps exactly the rows it would have kept scanning all. Returns
+// the number of sample tiles carried (0: no carry).
+static int64_t carried_tiles(const DbView& db, const std::vector<Level>& L, int variant) {
+  if (L.size() < 2) return 0;
+  const Level& lp = L[L.size() - 2];
+  const bool lds = variant == kFilterLdsWide || variant == kFilterLdsNarrow || variant == kFilterLdsWide8;
+  if (lds && db.packed && db.trank && lp.stride > 1 && db.srows >= lp.m && L.back().stride == 1)
+    return lp.m / 32;
+  return 0;
 }
 
 // Capturable part of a batch: query norms and encoding, filter levels,
@@ -221,13 +248,14 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
   float* relk = ws.relk.as<float>(nq);
   launch_fill_f32(tau, nq, INFINITY, st);
   launch_fill_i32(status, nq, 0, st);
+  const int64_t carry_tiles = carried_tiles(db, L, p.variant);
   rec(0);
   for (size_t li = 0; li < L.size(); ++li) {
     const Level& lv = L[li];
     const bool last = li + 1 == L.size();
     if (last) rec(1);
     run_filter(db, lv, li == 0, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs,
-               p.variant, st);
+               p.variant, last ? carry_tiles : 0, st);
     if (last) rec(2);
     SelectArgs sa{};
     sa.cand = cand;
@@ -243,6 +271,20 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
       sa.status = nullptr;  // a loose intermediate threshold only costs time
       launch_select(sa, st);
       std::swap(tau, tau2);
+      if (li + 2 == L.size() && carry_tiles > 0) {
+        CarryArgs ca{};
+        ca.cand = cand;
+        ca.counts = counts;
+        ca.cap = cap;
+        ca.nq = nq;
+        ca.nq_dev = nq_dev;
+        ca.tau = tau;
+        ca.step = db.sstep;
+        ca.tiles = db.stiles;
+        ca.skip_tiles = carry_tiles;
+        ca.status = status;
+        launch_carry(ca, st);
+      }
     } else {
       sa.K = kp;
       sa.topk_mode = 1;
@@ -313,7 +355,7 @@ void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& 
   if (depth == 0) {
     const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
     stats.levels = (int32_t)L.size();
-    stats.rows_filter_full = L.back().m;
+    stats.rows_filter_full = L.back().m - 32 * carried_tiles(db, L, p.variant);
     stats.filter_variant = p.variant;
     stats.target_wgs = (int32_t)p.target_wgs;
     if (pe && pe->on) {
@@ -400,6 +442,9 @@ static int elem_bytes(Elem e) { return e == Elem::F32 ? 4 : (e == Elem::BF16 ? 2
 // (Re-)encode rows [from, n) of the candidate copy for the configured precision.
 static void bf_free_sample(rplsh_bf* b) {
   if (b->senc) (void)hipFree(b->senc);
+  if (b->trank) (void)hipFree(b->trank);
+  b->trank = nullptr;
+  b->trank_cap = 0;
   if (b->sscale) (void)hipFree(b->sscale);
   if (b->snorm) (void)hipFree(b->snorm);
   if (b->snh) (void)hipFree(b->snh);
@@ -411,32 +456,46 @@ static void bf_free_sample(rplsh_bf* b) {
   b->sprec = -1;
 }
 
-// Sample copy for the sample levels: srows = max(n / 8, 32768) rows (at most
-// n), row r = (r * P) mod n with P ~ 0.618 n coprime to n, so every prefix is
-// a spread-out set of distinct rows. Rebuilt whenever the rows change.
+// Sample copy for the sample levels: srows = max(n / 8, 32768) rows rounded
+// down to whole tiles (at most the n / 32 whole tiles of the index), sample
+// tile s = index tile (s * P) mod (n / 32) with P ~ 0.618 n / 32 coprime to
+// it, so every prefix is a spread-out set of distinct tiles; trank maps each
+// index tile back to its sample tile (the full level skips the tiles a
+// carried sample level scored). Rebuilt whenever the rows change.
 static void bf_encode_sample(rplsh_bf* b, Elem e, int dpad, hipStream_t st) {
   constexpr int64_t kMinRows = 32768;
   if (b->n < 2 * kMinRows) {
     bf_free_sample(b);
     return;
   }
-  const int64_t sm = std::min<int64_t>(b->n, std::max<int64_t>(b->n / 8, kMinRows));
+  const int64_t tiles = b->n / 32;
+  const int64_t sm = std::min<int64_t>(tiles, std::max<int64_t>(b->n / 8, kMinRows) / 32) * 32;
   if (sm > b->scap || b->sprec != b->cfg.precision) {
     bf_free_sample(b);
-    const int64_t tiles = div_up(sm + sm / 4, 32);  // room to grow with the index
-    RPLSH_HIP(hipMalloc(&b->senc, (size_t)(tiles * packed_tile_bytes(e, dpad))));
-    if (e == Elem::FP8) RPLSH_HIP(hipMalloc((void**)&b->sscale, (size_t)tiles * 32 * sizeof(float)));
-    RPLSH_HIP(hipMalloc((void**)&b->snorm, (size_t)tiles * 32 * sizeof(float)));
-    RPLSH_HIP(hipMalloc((void**)&b->snh, (size_t)(tiles * 32 + 1) * sizeof(float)));
-    b->scap = tiles * 32;
+    const int64_t alloc_tiles = div_up(sm + sm / 4, 32);  // room to grow with the index
+    RPLSH_HIP(hipMalloc(&b->senc, (size_t)(alloc_tiles * packed_tile_bytes(e, dpad))));
+    if (e == Elem::FP8) RPLSH_HIP(hipMalloc((void**)&b->sscale, (size_t)alloc_tiles * 32 * sizeof(float)));
+    RPLSH_HIP(hipMalloc((void**)&b->snorm, (size_t)alloc_tiles * 32 * sizeof(float)));
+    RPLSH_HIP(hipMalloc((void**)&b->snh, (size_t)(alloc_tiles * 32 + 1) * sizeof(float)));
+    b->scap = alloc_tiles * 32;
     b->sprec = b->cfg.precision;
   }
-  int64_t p = (int64_t)std::llround(0.6180339887498949 * (double)b->n);
-  while (std::gcd(p, b->n) != 1) ++p;
+  const int64_t rank_tiles = div_up(b->n, 32);
+  if (rank_tiles > b->trank_cap) {
+    if (b->trank) (void)hipFree(b->trank);
+    b->trank = nullptr;
+    b->trank_cap = 0;
+    const int64_t cap = rank_tiles + rank_tiles / 4;
+    RPLSH_HIP(hipMalloc((void**)&b->trank, (size_t)cap * sizeof(int32_t)));
+    b->trank_cap = cap;
+  }
+  int64_t p = std::max<int64_t>(1, (int64_t)std::llround(0.6180339887498949 * (double)tiles));
+  while (std::gcd(p, tiles) != 1) ++p;
   b->srows = sm;
   b->sstep = p;
-  launch_sample_rows(b->xn, b->xscale, b->n, sm, p, b->snorm, b->snh, b->scap, b->sscale, st);
-  launch_encode_sample(e, dpad, b->x, b->n, b->dim, b->ldx, sm, p, b->senc, b->sscale, st);
+  launch_sample_rows(b->xn, b->xscale, tiles, sm, p, b->snorm, b->snh, b->scap, b->sscale, st);
+  launch_encode_sample(e, dpad, b->x, tiles, b->dim, b->ldx, sm, p, b->senc, b->sscale, st);
+  launch_tile_ranks(tiles, sm / 32, p, b->trank, rank_tiles, st);
 }
 
 static void bf_encode(rplsh_bf* b, int64_t from, hipStream_t st) {
@@ -499,6 +558,10 @@ static DbView bf_view(const rplsh_bf* b) {
     v.xnh = b->xnh;
     v.xnh_inf = div_up(b->capacity, 32) * 32;
     v.srows = b->senc ? b->srows : 0;
+    v.sstep = b->sstep;
+    v.stiles = b->n / 32;
+    v.trank = b->senc ? b->trank : nullptr;
+    v.rank_tiles = div_up(b->n, 32);
     v.packed = true;
   }
   return v;
@@ -1017,7 +1080,9 @@ rplsh_status rplsh_bf_autotune(rplsh_bf* b, const float* sample, int64_t nsample
       return ms;
     };
     const int wgs[] = {4096, 8192, 16384};
-    const int cr[][2] = {{3, 0}, {4, 16}, {2, 32}, {3, 32}, {2, 64}, {8, 8}};
+    // With a carried last sample level (LDS variants) a denser last sample
+    // costs little (the full level skips its tiles), so tight c pays off.
+    const int cr[][2] = {{3, 0}, {4, 16}, {2, 32}, {3, 32}, {2, 48}, {2, 64}, {8, 8}};
     // Packed copies: LDS wide / narrow, ring wide (the direct and ring-narrow
     // variants are left to explicit tuning entries).
     const int vars_packed[] = {kFilterLdsWide, kFilterLdsNarrow, kFilterRingWide};

@@ -224,7 +224,9 @@ typedef struct rplsh_search_stats {
   float ms_filter_samples;  /* device time of the sample-level filter launches */
   float ms_select;          /* device time of the last-level top-k' select */
   float ms_rerank;          /* device time of the re-rank */
-  int64_t rows_filter_full; /* database rows of the full filter launch */
+  int64_t rows_filter_full; /* database rows the full filter launch scores: all
+                               rows, less the tiles of a carried last sample
+                               level (packed copies, LDS filter variants) */
   int32_t filter_variant;   /* packed copies: filter variant used (rplsh_tune_result) */
   int32_t target_wgs;       /* workgroups the filter grid aimed for */
 } rplsh_search_stats;

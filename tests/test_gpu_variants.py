@@ -2,7 +2,9 @@
 direct, ring wide / narrow) and
 workgroup target computes the same MFMA chain and the same threshold tests,
 so a search returns bit-identical indices and distances under each of them,
-on the sample levels (sample copy, dense first pass) and the full level."""
+on the sample levels (sample copy, dense first pass) and the full level, and
+the LDS variants' carry of the last sample level (its tiles skipped by the
+full level) keeps the same full-level survivor count as a scan of every row."""
 import numpy as np
 import pytest
 
@@ -32,10 +34,13 @@ def test_variants_bit_identical(ca, orc, res, precision, n, d):
             assert st["filter_variant"] == v
             assert st["kprime"] == 4 * k
             if ref is None:
-                ref = (I, D)
+                ref = (I, D, st["survivors"])
                 I64, _ = orc.search_fp64(x, q, k)
-                print(precision, "recall", orc.recall(I, I64), "levels", st["levels"])
+                print(precision, "recall", orc.recall(I, I64), "levels", st["levels"], "survivors", st["survivors"])
                 continue
             assert np.array_equal(I, ref[0]), f"variant {v} tw {tw}"
             assert np.array_equal(D.view(np.uint32), ref[1].view(np.uint32)), f"variant {v} tw {tw}"
+            # LDS variants carry the last sample level's survivors and skip its
+            # tiles; the others scan every row: the full level keeps the same rows
+            assert st["survivors"] == ref[2], f"variant {v} tw {tw}"
     idx.clear_tuning()
