@@ -54,6 +54,14 @@ struct DevBuf {
   }
 };
 
+// Per-batch counters, cleared at the start of a batch and read back with one
+// copy after it.
+struct BatchCounters {
+  long long survivors;  // final-level survivors kept (sum over queries)
+  int failed;           // queries to re-run
+  int fp64;             // boundary candidates re-ranked in FP64
+};
+
 // Events around the phases of a batch (depth 0 only), read after the sync.
 struct PhaseEvents {
   hipEvent_t ev[8] = {};
@@ -78,7 +86,7 @@ struct PhaseEvents {
 
 struct Workspace {
   DevBuf qnorm, qenc, qscale, qpad, tau, tau2, counts, cand, topk, bound, status, relk;
-  DevBuf flist, fcount, fp64, surv;
+  DevBuf flist, ctr;  // failed-query list; BatchCounters
   DevBuf qsub, isub, dsub;  // a retried subset of queries
 };
 

@@ -246,6 +246,9 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
   float* bound = ws.bound.as<float>(nq);
   int* status = ws.status.as<int>(nq);
   float* relk = ws.relk.as<float>(nq);
+  BatchCounters* ctr = ws.ctr.as<BatchCounters>(1);
+  static_assert(sizeof(BatchCounters) == 4 * sizeof(int), "counters cleared as four words");
+  launch_fill_i32(reinterpret_cast<int*>(ctr), 4, 0, st);
   launch_fill_f32(tau, nq, INFINITY, st);
   launch_fill_i32(status, nq, 0, st);
   const int64_t carry_tiles = carried_tiles(db, L, p.variant);
@@ -295,13 +298,9 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
       rec(3);
     }
   }
-  long long* sv = ws.surv.as<long long>(1);
-  launch_fill_i32(reinterpret_cast<int*>(sv), 2, 0, st);
-  launch_sum_counts(counts, nq, cap, sv, st);
+  launch_sum_counts(counts, nq, cap, &ctr->survivors, st);
 
   // 3. re-rank
-  int* fp64c = ws.fp64.as<int>(1);
-  launch_fill_i32(fp64c, 1, 0, st);
   RerankArgs ra{};
   ra.x = db.x;
   ra.ldx = db.ldx;
@@ -325,16 +324,14 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
   ra.out_dist = out_d;
   ra.status = status;
   ra.relk_out = relk;
-  ra.fp64_count = fp64c;
+  ra.fp64_count = &ctr->fp64;
   rec(4);
   launch_rerank(ra, st);
   rec(5);
 
   // 4. failed queries
   int* flist = ws.flist.as<int>(nq);
-  int* fcount = ws.fcount.as<int>(1);
-  launch_fill_i32(fcount, 1, 0, st);
-  launch_collect_failed(status, nq, nq_dev, flist, fcount, st);
+  launch_collect_failed(status, nq, nq_dev, flist, &ctr->failed, st);
 }
 
 // Host side after enqueue_batch: waits for st, reads the counters and re-runs
@@ -345,12 +342,14 @@ void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& 
   Workspace& ws = wss.at(depth);
   const int d = db.d;
   const int kp = (int)std::min<int64_t>(p.kp, db.n);
-  int nfail = 0, nfp64 = 0;
-  long long nsurv = 0;
-  RPLSH_HIP(hipMemcpyAsync(&nfail, ws.fcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
-  RPLSH_HIP(hipMemcpyAsync(&nfp64, ws.fp64.p, sizeof(int), hipMemcpyDeviceToHost, st));
-  RPLSH_HIP(hipMemcpyAsync(&nsurv, ws.surv.p, sizeof(long long), hipMemcpyDeviceToHost, st));
+  // One read-back of the batch's counters (a pageable target: workspaces are
+  // created per cached graph, and a pinned allocation per workspace would cost
+  // more than it saves).
+  BatchCounters hc{};
+  RPLSH_HIP(hipMemcpyAsync(&hc, ws.ctr.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
   RPLSH_HIP(hipStreamSynchronize(st));
+  const int nfail = hc.failed, nfp64 = hc.fp64;
+  const long long nsurv = hc.survivors;
   stats.boundary_fp64 += nfp64;
   if (depth == 0) {
     const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);

@@ -143,8 +143,8 @@ size_t workspace_bytes(const std::vector<Workspace>& wss) {
   size_t b = 0;
   for (const auto& w : wss) {
     for (const DevBuf* d : {&w.qnorm, &w.qenc, &w.qscale, &w.qpad, &w.tau, &w.tau2, &w.counts,
-                            &w.cand, &w.topk, &w.bound, &w.status, &w.relk, &w.flist, &w.fcount,
-                            &w.fp64, &w.surv, &w.qsub, &w.isub, &w.dsub})
+                            &w.cand, &w.topk, &w.bound, &w.status, &w.relk, &w.flist, &w.ctr,
+                            &w.qsub, &w.isub, &w.dsub})
       b += d->bytes;
   }
   return b;

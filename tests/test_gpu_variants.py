@@ -44,3 +44,42 @@ def test_variants_bit_identical(ca, orc, res, precision, n, d):
             # tiles; the others scan every row: the full level keeps the same rows
             assert st["survivors"] == ref[2], f"variant {v} tw {tw}"
     idx.clear_tuning()
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp8"])
+def test_carry_after_incremental_add_and_partial_tile(ca, res, precision):
+    """The sample copy and its tile rank map are rebuilt on every add: an index
+    grown in two adds to n = 250,001 rows (not a multiple of 32: a partial last
+    tile, never skipped) answers bit-identically to one add, with the LDS wide
+    variant (carry) and the ring wide variant (every row scanned), and both
+    keep the same full-level survivors; the full launch scores n less the
+    carried tiles."""
+    n, d, k = 250_001, 64, 100  # k' = 400, c = 2, rho = 64: three sample levels, the last carried
+    x = ca.gen_gaussian(n, d, 51, True)
+    q = ca.gen_gaussian(500, d, 52, True)
+    one = ca.IndexBF(res, d, ca.IndexBFConfig(precision=precision, kprime_factor=4))
+    one.add(x)
+    two = ca.IndexBF(res, d, ca.IndexBFConfig(precision=precision, kprime_factor=4))
+    two.add(x[:90_017])
+    two.add(x[90_017:])
+    ref = None
+    for name, idx in (("one add", one), ("two adds", two)):
+        for v in (0, 3):  # LDS wide (carry), ring wide (no carry)
+            idx.set_tuning(dict(k=k, batch_nq=q.shape[0], kprime_factor=4, target_wgs=16384, c_factor=2,
+                                variant=v, rho=64, recall=0.0, ms_default=0.0, ms_tuned=0.0))
+            I, D = idx.search(q, k)
+            st = idx.last_stats()
+            print(name, "variant", v, "survivors", st["survivors"], "retried", st["retried_queries"],
+                  "rows", st["rows_filter_full"])
+            if v == 0:
+                assert 0 < n - st["rows_filter_full"] < n // 2 and (n - st["rows_filter_full"]) % 32 == 0
+            else:
+                assert st["rows_filter_full"] == n
+            if ref is None:
+                ref = (I, D, st["survivors"], st["retried_queries"])
+                continue
+            assert np.array_equal(I, ref[0]) and np.array_equal(D.view(np.uint32), ref[1].view(np.uint32)), \
+                (name, v)
+            # same rows kept at the full level, the same queries re-run
+            assert (st["survivors"], st["retried_queries"]) == ref[2:], (name, v)
+        idx.close()
