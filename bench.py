@@ -535,7 +535,8 @@ def serving(args, ca) -> int:
 # timed; prerank f > 0 scores the candidates in BF16 and re-ranks only the f*k best
 RPLSH_CONFIGS = [(8, 12, 0, 512, 0), (16, 14, 2, 2048, 0), (32, 12, 1, 4096, 0), (16, 20, 2, 1024, 0),
                  (32, 20, 4, 2048, 0), (32, 24, 8, 2048, 0), (64, 22, 4, 4096, 0),
-                 (32, 20, 4, 2048, 8), (32, 24, 8, 2048, 8), (64, 22, 4, 4096, 8), (64, 22, 4, 4096, 16)]
+                 (32, 20, 4, 2048, 8), (32, 24, 8, 2048, 8), (64, 22, 4, 4096, 8), (64, 22, 4, 4096, 16),
+                 (64, 24, 4, 4096, 4), (64, 24, 4, 8192, 4), (64, 24, 8, 8192, 4), (128, 24, 4, 8192, 4)]
 
 
 def rplsh_data(ca, name: str, n: int, nq: int):

@@ -200,7 +200,8 @@ class IndexRPLSHConfig:
 
     tables x bits: hash tables and code bits per table; probes: extra
     single-bit-flip buckets per table (query-directed); max_candidates: bucket
-    entries taken per query before de-duplication (<= 4096); seed: projection
+    entries taken per query before de-duplication (<= 8192; above 4096 only
+    with prerank > 0 and prerank * k <= 4096); seed: projection
     seed (standard normal, the library's generator); center: hyperplane
     offsets at the projections' medians over the first add's first 65536 rows
     (L2 proximity) instead of through the origin (angle); prerank: f > 0 keeps

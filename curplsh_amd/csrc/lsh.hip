@@ -55,7 +55,8 @@ namespace {
 
 constexpr int kLshMaxBits = 24;
 constexpr int kLshMaxEntries = 1024;  // (probes + 1) * tables
-constexpr int kLshMaxBudget = 4096;   // the re-rank's longest list
+constexpr int kLshMaxBudget = 8192;   // candidates gathered per query (LDS sort: 32 KB)
+constexpr int kLshMaxExact = 4096;    // the exact re-rank's longest list
 constexpr int kLshBlock = 256;
 constexpr int64_t kLshCenterSample = 65536;
 
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(kLshBlock) void lsh_gather_kernel(LshGatherArgs a) 
   __shared__ int32_t ecnt[kLshMaxEntries];
   __shared__ int32_t eoff[kLshMaxEntries + 1];
   __shared__ int32_t part[kLshBlock];
-  __shared__ int32_t buf[kLshMaxBudget];
+  extern __shared__ int32_t buf[];  // [pow2 >= budget] (dynamic: sized by the launch)
   __shared__ int32_t sh_distinct;
   const int q = blockIdx.x;
   if (q >= a.nq) return;
@@ -593,6 +594,9 @@ rplsh_status rplsh_lsh_create(rplsh_resources* res, int dim, const rplsh_lsh_con
     RPLSH_REQUIRE(c.max_candidates >= 1 && c.max_candidates <= kLshMaxBudget,
                   "max_candidates must be in [1, %d]", kLshMaxBudget);
     RPLSH_REQUIRE(c.prerank >= 0 && c.prerank <= 64, "prerank must be in [0, 64]");
+    RPLSH_REQUIRE(c.max_candidates <= kLshMaxExact || c.prerank > 0,
+                  "max_candidates above %d needs a pre-rank (the exact re-rank takes at most %d)",
+                  kLshMaxExact, kLshMaxExact);
     if (rplsh_resources_init_device(res, c.device) != RPLSH_OK)
       fail(RPLSH_ERR_STATE, "%s", rplsh_last_error());
     DeviceScope scope(c.device);
@@ -727,6 +731,13 @@ rplsh_status rplsh_lsh_search(rplsh_lsh* h, int64_t nq, const float* queries, in
     if (nq <= 0) return;
     RPLSH_REQUIRE(queries && indices && distances, "NULL pointer");
     RPLSH_REQUIRE(k > 0, "k must be > 0 (got %d)", k);
+    {
+      const int64_t B0 = h->cfg.max_candidates;
+      const int64_t kpre0 = h->cfg.prerank > 0 ? std::min<int64_t>((int64_t)h->cfg.prerank * k, B0) : B0;
+      if (kpre0 > kLshMaxExact)
+        fail(RPLSH_ERR_UNSUPPORTED, "exact re-rank of %lld candidates per query (at most %d: lower prerank * k)",
+             (long long)kpre0, kLshMaxExact);
+    }
     RPLSH_REQUIRE(nq < (1ll << 31) && (int64_t)nq * k < (1ll << 40), "batch too large");
     DeviceScope scope(h->cfg.device);
     hipStream_t st = lsh_stream(h);
@@ -776,7 +787,9 @@ rplsh_status rplsh_lsh_search(rplsh_lsh* h, int64_t nq, const float* queries, in
       ga.cand = cand;
       ga.raw_total = ctr;
       ga.distinct_total = ctr + 1;
-      lsh_gather_kernel<<<(unsigned)nq, kLshBlock, 0, st>>>(ga);
+      int p2 = 1;
+      while (p2 < B) p2 <<= 1;
+      lsh_gather_kernel<<<(unsigned)nq, kLshBlock, (size_t)p2 * sizeof(int32_t), st>>>(ga);
       RPLSH_HIP(hipGetLastError());
       RPLSH_HIP(hipEventRecord(ev[1], st));
       // exact re-rank of the distinct candidates (no certificate, no FP64 band)

@@ -38,7 +38,8 @@ struct IndexRPLSHConfig : public IndexConfig {
   int numBits;
   /// Extra buckets probed per table (0..numBits).
   int numProbes;
-  /// Bucket entries taken per query before de-duplication (<= 4096).
+  /// Bucket entries taken per query before de-duplication (<= 8192; above
+  /// 4096 only with prerankFactor > 0 and prerankFactor * k <= 4096).
   int maxCandidates;
   /// Seed of the projection matrix.
   unsigned long long seed;

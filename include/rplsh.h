@@ -303,7 +303,10 @@ typedef struct rplsh_lsh_config {
   int bits;            /* bits per table code, 1..24 */
   int probes;          /* extra single-bit-flip buckets per table, 0..bits;
                           (probes + 1) * tables <= 1024 */
-  int max_candidates;  /* bucket entries taken per query before de-duplication, 1..4096 */
+  int max_candidates;  /* bucket entries taken per query before de-duplication, 1..8192;
+                          above 4096 only with a pre-rank (the exact re-rank takes at
+                          most 4096 per query: a search whose min(prerank*k,
+                          max_candidates) exceeds that fails with UNSUPPORTED) */
   uint64_t seed;       /* projection seed */
   int center;          /* 1: median offsets (L2 proximity), 0: through the origin (angle) */
   int prerank;         /* 0: exact re-rank of every distinct candidate; f > 0 (when

@@ -92,6 +92,8 @@ PRE_CASES = [  # n, d, nq, k, tables, bits, probes, budget, prerank
     (3000, 5, 20, 5, 4, 6, 1, 200, 2),        # one piece per row: 64 rows per load
     (3000, 60, 20, 10, 4, 8, 1, 400, 2),      # 8 pieces
     (1500, 256, 12, 10, 4, 8, 1, 256, 2),     # 32 pieces: two rows per load
+    (30000, 64, 40, 10, 64, 10, 4, 6000, 4),  # budget above the exact re-rank's 4096
+    (40000, 32, 30, 20, 64, 9, 6, 8192, 8),   # the largest budget
 ]
 
 
@@ -192,8 +194,22 @@ def test_recall_on_clustered_data(ca, orc, res):
     assert rec[3] >= rec[0] and rec[3] >= 0.8
 
 
+def test_budget_above_exact_limit_needs_prerank(ca, res):
+    with pytest.raises(ca.RplshError):
+        make(ca, res, 16, max_candidates=5000)               # no pre-rank: 5000 exact
+    with pytest.raises(ca.RplshError):
+        make(ca, res, 16, max_candidates=8193, prerank=4)    # past the gather's limit
+    idx = make(ca, res, 16, tables=8, bits=6, probes=2, max_candidates=8192, prerank=64, seed=2)
+    x = ca.gen_uniform(5000, 16, 3, -1, 1)
+    idx.add(x)
+    I, _ = idx.search(x[:4], 64)                             # 64 * 64 = 4096 exact: fine
+    assert I.shape == (4, 64)
+    with pytest.raises(ca.RplshError):
+        idx.search(x[:4], 65)                                # 65 * 64 > 4096
+
+
 def test_invalid_configs_rejected(ca, res):
-    for kw in (dict(bits=0), dict(bits=25), dict(probes=17, bits=16), dict(max_candidates=5000),
+    for kw in (dict(bits=0), dict(bits=25), dict(probes=17, bits=16), dict(max_candidates=9000),
                dict(tables=100, probes=10, bits=12), dict(tables=0), dict(prerank=-1),
                dict(prerank=65)):
         with pytest.raises(ca.RplshError):
