@@ -31,8 +31,8 @@
 // wave hashes a quarter of the tables for them, a lane's K chains of a table
 // interleaved, projections through the scalar cache), lsh_gather_kernel (one
 // workgroup per query: probe codes, bucket ranges by binary search, a scan,
-// the gather, a bitonic sort and de-duplication in LDS; bucket searches start
-// from a directory over the codes' top 16 bits). The table build is a
+// the gather into an LDS hash set that keeps each row once; bucket searches
+// start from a directory over the codes' top 16 bits). The table build is a
 // stable LSD radix sort per table (hipCUB) at add time, off the search path.
 #include <hip/hip_runtime.h>
 
@@ -55,8 +55,27 @@ namespace {
 
 constexpr int kLshMaxBits = 24;
 constexpr int kLshMaxEntries = 1024;  // (probes + 1) * tables
-constexpr int kLshMaxBudget = 8192;   // candidates gathered per query (LDS sort: 32 KB)
+constexpr int kLshMaxBudget = 8192;   // candidates gathered per query
 constexpr int kLshMaxExact = 4096;    // the exact re-rank's longest list
+constexpr int kLshMaxTable = 11261;   // hash-set slots, prime (44 KB: the block stays under 64 KB of LDS)
+
+// Slots of the gather's de-duplication hash set for a budget of B entries:
+// the smallest prime >= twice B rounded up to a power of two, at most
+// kLshMaxTable (load <= 0.73); prime, so a double-hashing probe sequence
+// visits every slot.
+inline int lsh_table_slots(int B) {
+  int p = 1;
+  while (p < B) p <<= 1;
+  int t = std::min(2 * p, kLshMaxTable);
+  auto prime = [](int v) {
+    if (v < 2) return false;
+    for (int i = 2; i * i <= v; ++i)
+      if (v % i == 0) return false;
+    return true;
+  };
+  while (!prime(t)) ++t;
+  return t;
+}
 constexpr int kLshBlock = 256;
 constexpr int64_t kLshCenterSample = 65536;
 
@@ -178,7 +197,9 @@ struct LshGatherArgs {
   const uint32_t* qcodes;  // [L][nq]
   const float* qmarg;      // [nq][L*K]
   int nq, L, K, probes, budget;
-  uint64_t* cand;          // [nq][budget] words (index only; key ignored by the re-rank)
+  int table;               // de-duplication hash-set slots (> budget)
+  uint64_t* cand;          // [nq][budget] words (index only; key ignored by the re-rank):
+                           // the distinct rows first, in no particular order, then empty
   unsigned long long* raw_total;
   unsigned long long* distinct_total;
 };
@@ -195,7 +216,7 @@ __global__ __launch_bounds__(kLshBlock) void lsh_gather_kernel(LshGatherArgs a) 
   __shared__ int32_t ecnt[kLshMaxEntries];
   __shared__ int32_t eoff[kLshMaxEntries + 1];
   __shared__ int32_t part[kLshBlock];
-  extern __shared__ int32_t buf[];  // [pow2 >= budget] (dynamic: sized by the launch)
+  extern __shared__ int32_t tab[];  // [table] hash set of rows (dynamic: sized by the launch)
   __shared__ int32_t sh_distinct;
   const int q = blockIdx.x;
   if (q >= a.nq) return;
@@ -264,55 +285,62 @@ __global__ __launch_bounds__(kLshBlock) void lsh_gather_kernel(LshGatherArgs a) 
   __syncthreads();
   const int R = min(eoff[E], B);
 
-  // 4. gather the first R entries of the concatenation
-  for (int j = threadIdx.x; j < R; j += blockDim.x) {
-    int lo = 0, hi = E;  // last e with eoff[e] <= j
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (eoff[mid] <= j) lo = mid; else hi = mid;
-    }
-    const int t = lo % L;
-    buf[j] = a.perm[(int64_t)t * a.ld + elo[lo] + (j - eoff[lo])];
-  }
-  int P2 = 1;
-  while (P2 < R) P2 <<= 1;
-  for (int j = R + threadIdx.x; j < P2; j += blockDim.x) buf[j] = INT_MAX;
+  // 4. gather the first R entries of the concatenation into an LDS hash set
+  //    (open addressing, double hashing over a prime T > R slots, so a free one
+  //    is always reached): a row's first insertion appends it to the query's list, so the
+  //    list holds each distinct row once, in no particular order (every
+  //    consumer orders candidates by (distance, index)); appends are counted
+  //    per wave (ballot, one atomic).
+  const int T = a.table;
+  for (int i = threadIdx.x; i < T; i += blockDim.x) tab[i] = -1;
   if (threadIdx.x == 0) sh_distinct = 0;
   __syncthreads();
-
-  // 5. bitonic sort of buf[0, P2)
-  for (int k = 2; k <= P2; k <<= 1) {
-    for (int s = k >> 1; s > 0; s >>= 1) {
-      for (int i = threadIdx.x; i < (P2 >> 1); i += blockDim.x) {
-        const int lo = ((i & ~(s - 1)) << 1) | (i & (s - 1));
-        const int hi = lo + s;
-        const bool up = (lo & k) == 0;
-        const int x = buf[lo], y = buf[hi];
-        if ((x > y) == up) {
-          buf[lo] = y;
-          buf[hi] = x;
-        }
-      }
-      __syncthreads();
-    }
-  }
-
-  // 6. distinct rows as candidate words, the rest empty
-  int mine = 0;
   uint64_t* out = a.cand + (int64_t)q * B;
-  for (int j = threadIdx.x; j < B; j += blockDim.x) {
-    uint64_t wv = kEmptyCand;
-    if (j < R && (j == 0 || buf[j - 1] != buf[j])) {
-      wv = pack_cand(0.0f, buf[j]);
-      ++mine;
+  const int lane = threadIdx.x & 63;
+  for (int j0 = 0; j0 < R; j0 += blockDim.x) {  // uniform trip count
+    const int j = j0 + threadIdx.x;
+    bool fresh = false;
+    int32_t row = -1;
+    if (j < R) {
+      int lo = 0, hi = E;  // last e with eoff[e] <= j
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (eoff[mid] <= j) lo = mid; else hi = mid;
+      }
+      const int t = lo % L;
+      row = a.perm[(int64_t)t * a.ld + elo[lo] + (j - eoff[lo])];
+      uint32_t slot = (uint32_t)(((uint64_t)((uint32_t)row * 0x9E3779B1u) * (uint32_t)T) >> 32);
+      const uint32_t step =
+          1u + (uint32_t)(((uint64_t)((uint32_t)row * 0x85EBCA77u) * (uint32_t)(T > 1 ? T - 1 : 1)) >> 32);
+      while (true) {
+        const int old = atomicCAS(&tab[slot], -1, row);
+        if (old == -1) {
+          fresh = true;
+          break;
+        }
+        if (old == row) break;
+        slot += step;  // step in [1, T - 1]
+        if (slot >= (uint32_t)T) slot -= (uint32_t)T;
+      }
     }
-    out[j] = wv;
+    const uint64_t m = __ballot(fresh);
+    if (m != 0) {
+      const int leader = __ffsll((unsigned long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&sh_distinct, __popcll(m));
+      base = __shfl(base, leader, 64);
+      const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      if (fresh) out[base + rank] = pack_cand(0.0f, row);
+    }
   }
-  if (mine) atomicAdd(&sh_distinct, mine);
   __syncthreads();
+
+  // 5. the rest of the list empty; totals
+  const int nd = sh_distinct;
+  for (int j = nd + threadIdx.x; j < B; j += blockDim.x) out[j] = kEmptyCand;
   if (threadIdx.x == 0) {
     atomicAdd(a.raw_total, (unsigned long long)R);
-    atomicAdd(a.distinct_total, (unsigned long long)sh_distinct);
+    atomicAdd(a.distinct_total, (unsigned long long)nd);
   }
 }
 
@@ -787,9 +815,8 @@ rplsh_status rplsh_lsh_search(rplsh_lsh* h, int64_t nq, const float* queries, in
       ga.cand = cand;
       ga.raw_total = ctr;
       ga.distinct_total = ctr + 1;
-      int p2 = 1;
-      while (p2 < B) p2 <<= 1;
-      lsh_gather_kernel<<<(unsigned)nq, kLshBlock, (size_t)p2 * sizeof(int32_t), st>>>(ga);
+      ga.table = lsh_table_slots(B);
+      lsh_gather_kernel<<<(unsigned)nq, kLshBlock, (size_t)ga.table * sizeof(int32_t), st>>>(ga);
       RPLSH_HIP(hipGetLastError());
       RPLSH_HIP(hipEventRecord(ev[1], st));
       // exact re-rank of the distinct candidates (no certificate, no FP64 band)
