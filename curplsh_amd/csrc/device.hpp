@@ -72,6 +72,9 @@ struct FilterArgs {
   const int32_t* tile_rank;
   int64_t rank_tiles;
   int64_t skip_tiles;
+  // First logical row (a multiple of 32): a sample level after a carried one
+  // scores rows [row_begin, m) only; the slices cover m - row_begin rows.
+  int64_t row_begin;
 };
 
 // Sample copy of a packed index: whole 32-row tiles, sample tile s holding
@@ -82,16 +85,16 @@ __host__ __device__ inline int64_t sample_source_row(int64_t r, int64_t step, in
   return (int64_t)(((uint64_t)(r >> 5) * (uint64_t)step) % (uint64_t)tiles) * 32 + (r & 31);
 }
 
-// Carry of the last sample level's survivors into the full level's lists.
+// Carry of a sample level's survivors into the next level's lists.
 struct CarryArgs {
   uint64_t* cand;        // [nq][cap]: the sample level's lists, compacted in place
   int* counts;           // [nq]: in the sample level's counts, out the carried counts
   int cap;
   int nq;
   const int* nq_dev;     // if set: live queries (<= nq), read on device
-  const float* tau;      // [nq] the full level's threshold
-  int64_t step, tiles;   // sample copy mapping (sample_source_row)
-  int64_t skip_tiles;    // sample tiles the full level skips
+  const float* tau;      // [nq] the next level's threshold
+  int64_t row_limit;     // rows carried: sample rows below it (the next level skips them)
+  int64_t step, tiles;   // step > 0: map sample rows to index rows (sample_source_row)
   int* status;           // [nq] kStatusOverflow when a list overflowed
 };
 

@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) 
   const int slice = logical / a.num_qtiles;
   const int nq = a.nq_dev ? min(*a.nq_dev, a.nq) : a.nq;
   if (qtile * BQ >= nq) return;
-  const int64_t r_begin = (int64_t)slice * a.rows_per_slice;
+  const int64_t r_begin = a.row_begin + (int64_t)slice * a.rows_per_slice;
   int64_t r_end = r_begin + a.rows_per_slice;
   if (r_end > a.m) r_end = a.m;
   if (r_begin >= r_end) return;
@@ -465,7 +465,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
   const int slice = logical / a.num_qtiles;
   const int nq = a.nq_dev ? min(*a.nq_dev, a.nq) : a.nq;
   if (qtile * BQ >= nq) return;
-  const int64_t r_begin = (int64_t)slice * a.rows_per_slice;
+  const int64_t r_begin = a.row_begin + (int64_t)slice * a.rows_per_slice;
   int64_t r_end = r_begin + a.rows_per_slice;
   if (r_end > a.m) r_end = a.m;
   if (r_begin >= r_end) return;
@@ -761,7 +761,7 @@ __global__ __launch_bounds__(kBlock, (RingShape<E, DPAD, WIDE>::WAVES_PER_SIMD))
   const int slice = logical / a.num_qtiles;
   const int nq = a.nq_dev ? min(*a.nq_dev, a.nq) : a.nq;
   if (qtile * BQ >= nq) return;
-  const int64_t r_begin = (int64_t)slice * a.rows_per_slice;
+  const int64_t r_begin = a.row_begin + (int64_t)slice * a.rows_per_slice;
   int64_t r_end = r_begin + a.rows_per_slice;
   if (r_end > a.m) r_end = a.m;
   if (r_begin >= r_end) return;
@@ -1178,11 +1178,11 @@ __global__ void tile_ranks_kernel(int64_t tiles, int64_t stiles, int64_t step_in
   rank[u] = v;
 }
 
-// One wave per query: keeps, in list order, the words whose row lies in the
-// first skip_tiles sample tiles and that pass the full level's own test on
-// acc = -key/2 (the filter wrote key = -2 acc, so both halvings are exact),
-// with the sample row replaced by its index row. Reads of a chunk precede its
-// writes, and writes never pass the chunk being read.
+// One wave per query: keeps, in list order, the words whose row lies below
+// row_limit and that pass the next level's own test on acc = -key/2 (the
+// filter wrote key = -2 acc, so both halvings are exact), with the sample row
+// replaced by its index row when step > 0 (into the full level). Reads of a
+// chunk precede its writes, and writes never pass the chunk being read.
 __global__ __launch_bounds__(256) void carry_kernel(CarryArgs a) {
   const int q = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -1190,7 +1190,7 @@ __global__ __launch_bounds__(256) void carry_kernel(CarryArgs a) {
   const int count = a.counts[q];
   const int n = count < a.cap ? count : a.cap;
   const float thr = fmaxf(-0.5f * a.tau[q], -FLT_MAX);
-  const int64_t rlim = a.skip_tiles * 32;
+  const int64_t rlim = a.row_limit;
   uint64_t* list = a.cand + (int64_t)q * a.cap;
   int kept = 0;
   for (int i0 = 0; i0 < n; i0 += 64) {
@@ -1200,7 +1200,7 @@ __global__ __launch_bounds__(256) void carry_kernel(CarryArgs a) {
     if (w != kEmptyCand) {
       const int32_t r = cand_idx(w);
       keep = -0.5f * cand_key(w) >= thr && r < rlim;
-      if (keep) w = pack_cand(cand_key(w), (int32_t)sample_source_row(r, a.step, a.tiles));
+      if (keep && a.step > 0) w = pack_cand(cand_key(w), (int32_t)sample_source_row(r, a.step, a.tiles));
     }
     const uint64_t m = __ballot(keep);
     const int pos = kept + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -1256,7 +1256,8 @@ void launch_tile_ranks(int64_t tiles, int64_t stiles, int64_t step, int32_t* ran
 
 void launch_carry(const CarryArgs& a, hipStream_t st) {
   if (a.nq <= 0) return;
-  RPLSH_REQUIRE(a.cand && a.counts && a.tau && a.tiles > 0 && a.step > 0, "carry: bad arguments");
+  RPLSH_REQUIRE(a.cand && a.counts && a.tau && a.row_limit > 0 && (a.step == 0 || a.tiles > 0),
+                "carry: bad arguments");
   carry_kernel<<<(unsigned)div_up(a.nq, 4), 256, 0, st>>>(a);
   RPLSH_HIP(hipGetLastError());
 }

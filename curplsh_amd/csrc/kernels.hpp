@@ -58,9 +58,10 @@ void launch_sample_rows(const float* xn, const float* xscale, int64_t tiles, int
 // s < stiles; INT32_MAX for every other u < rank_tiles.
 void launch_tile_ranks(int64_t tiles, int64_t stiles, int64_t step, int32_t* rank, int64_t rank_tiles,
                        hipStream_t st);
-// Compacts each query's list to the survivors of the full level's test
-// (acc = -key/2 >= max(-tau/2, -FLT_MAX)) from the first skip_tiles sample
-// tiles, their ids mapped to index rows; counts become the carried counts.
+// Compacts each query's list to the survivors of the next level's test
+// (acc = -key/2 >= max(-tau/2, -FLT_MAX)) from sample rows below row_limit,
+// their ids mapped to index rows when step > 0; counts become the carried
+// counts.
 void launch_carry(const CarryArgs& a, hipStream_t st);
 // nh[i] = -xn[i] / 2 for i in [from, n); nh[inf_at] = -inf.
 void launch_neg_half_norms(const float* xn, int64_t from, int64_t n, float* nh, int64_t inf_at,

@@ -135,11 +135,13 @@ static int64_t level_cap(const std::vector<Level>& L, int kp, int c_factor) {
 // row numbers are never used). `dense`: tau is +inf for every query (first
 // pass), so every row survives and is written to its own slot.
 // `skip_tiles` > 0 (full level after a carried sample level): the tiles that
-// level scored are skipped and the lists already hold its carried survivors.
+// level scored are skipped; `row_begin` > 0 (sample level after a carried
+// one): rows below it are not scored. Either way the lists already hold the
+// carried survivors.
 static void run_filter(const DbView& db, const Level& lv, bool dense, const void* qenc,
                        const float* qscale, const float* tau, int nq, const int* nq_dev, int cap,
                        int* counts, uint64_t* cand, int64_t target_wgs, int variant,
-                       int64_t skip_tiles, hipStream_t st) {
+                       int64_t skip_tiles, int64_t row_begin, hipStream_t st) {
   FilterArgs fa{};
   fa.x = db.xenc;
   fa.ldx = db.ldenc;
@@ -167,11 +169,15 @@ static void run_filter(const DbView& db, const Level& lv, bool dense, const void
   const int bq = db.packed ? filter_packed_block_queries(db.elem, db.dpad, variant)
                            : filter_block_queries(db.elem, db.dpad);
   fa.num_qtiles = (int)div_up(nq, bq);
+  RPLSH_REQUIRE(row_begin >= 0 && row_begin % 32 == 0 && row_begin < lv.m && (row_begin == 0 || !dense),
+                "filter: bad first row");
+  fa.row_begin = row_begin;
+  const int64_t rows = lv.m - row_begin;
   int64_t slices = std::max<int64_t>(1, div_up(target_wgs, fa.num_qtiles));
-  slices = std::min<int64_t>(slices, div_up(lv.m, 256));
+  slices = std::min<int64_t>(slices, div_up(rows, 256));
   slices = std::max<int64_t>(slices, 1);
-  fa.rows_per_slice = round_up(div_up(lv.m, slices), 64);
-  fa.num_slices = (int)div_up(lv.m, fa.rows_per_slice);
+  fa.rows_per_slice = round_up(div_up(rows, slices), 64);
+  fa.num_slices = (int)div_up(rows, fa.rows_per_slice);
   fa.cap = cap;
   fa.counts = counts;
   fa.cand = cand;
@@ -180,9 +186,8 @@ static void run_filter(const DbView& db, const Level& lv, bool dense, const void
     fa.tile_rank = db.trank;
     fa.rank_tiles = db.rank_tiles;
     fa.skip_tiles = skip_tiles;
-  } else {
-    launch_fill_i32(counts, nq, fa.dense ? (int)lv.m : 0, st);
   }
+  if (skip_tiles == 0 && row_begin == 0) launch_fill_i32(counts, nq, fa.dense ? (int)lv.m : 0, st);
   if (db.packed) {
     launch_filter_packed(db.elem, db.dpad, variant, fa, st);
   } else {
@@ -190,17 +195,20 @@ static void run_filter(const DbView& db, const Level& lv, bool dense, const void
   }
 }
 
-// Carry: when the level before the full one read whole tiles of the sample
-// copy, its survivors that pass the full level's threshold start the full
-// level's lists and the full level skips those tiles (LDS variants); the full
-// level then keeps exactly the rows it would have kept scanning all. Returns
-// the number of sample tiles carried (0: no carry).
-static int64_t carried_tiles(const DbView& db, const std::vector<Level>& L, int variant) {
-  if (L.size() < 2) return 0;
-  const Level& lp = L[L.size() - 2];
+// Carry: a level whose predecessor read whole tiles of the sample copy starts
+// from the predecessor's survivors that pass its own threshold and does not
+// score those rows again (LDS variants): a sample level starts at the
+// predecessor's last whole tile, the full level skips the predecessor's tiles.
+// Every level then keeps exactly the rows it would have kept scanning all.
+// Returns the number of sample tiles carried into level li (0: no carry).
+static int64_t carried_tiles(const DbView& db, const std::vector<Level>& L, size_t li, int variant) {
+  if (li == 0 || li >= L.size()) return 0;
+  const Level& lp = L[li - 1];
+  const Level& lv = L[li];
   const bool lds = variant == kFilterLdsWide || variant == kFilterLdsNarrow || variant == kFilterLdsWide8;
-  if (lds && db.packed && db.trank && lp.stride > 1 && db.srows >= lp.m && L.back().stride == 1)
-    return lp.m / 32;
+  if (!lds || !db.packed || !db.trank || lp.stride <= 1 || db.srows < lp.m) return 0;
+  if (lv.stride == 1) return lp.m / 32;                       // into the full level
+  if (db.srows >= lv.m && lp.m / 32 * 32 < lv.m) return lp.m / 32;  // into a sample level
   return 0;
 }
 
@@ -251,14 +259,14 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
   launch_fill_i32(reinterpret_cast<int*>(ctr), 4, 0, st);
   launch_fill_f32(tau, nq, INFINITY, st);
   launch_fill_i32(status, nq, 0, st);
-  const int64_t carry_tiles = carried_tiles(db, L, p.variant);
   rec(0);
   for (size_t li = 0; li < L.size(); ++li) {
     const Level& lv = L[li];
     const bool last = li + 1 == L.size();
     if (last) rec(1);
+    const int64_t carried = carried_tiles(db, L, li, p.variant);
     run_filter(db, lv, li == 0, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs,
-               p.variant, last ? carry_tiles : 0, st);
+               p.variant, last ? carried : 0, last ? 0 : 32 * carried, st);
     if (last) rec(2);
     SelectArgs sa{};
     sa.cand = cand;
@@ -274,7 +282,8 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
       sa.status = nullptr;  // a loose intermediate threshold only costs time
       launch_select(sa, st);
       std::swap(tau, tau2);
-      if (li + 2 == L.size() && carry_tiles > 0) {
+      const int64_t carry_next = carried_tiles(db, L, li + 1, p.variant);
+      if (carry_next > 0) {
         CarryArgs ca{};
         ca.cand = cand;
         ca.counts = counts;
@@ -282,9 +291,10 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
         ca.nq = nq;
         ca.nq_dev = nq_dev;
         ca.tau = tau;
-        ca.step = db.sstep;
+        ca.row_limit = 32 * carry_next;
+        const bool into_full = li + 2 == L.size();
+        ca.step = into_full ? db.sstep : 0;
         ca.tiles = db.stiles;
-        ca.skip_tiles = carry_tiles;
         ca.status = status;
         launch_carry(ca, st);
       }
@@ -354,7 +364,7 @@ void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& 
   if (depth == 0) {
     const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
     stats.levels = (int32_t)L.size();
-    stats.rows_filter_full = L.back().m - 32 * carried_tiles(db, L, p.variant);
+    stats.rows_filter_full = L.back().m - 32 * carried_tiles(db, L, L.size() - 1, p.variant);
     stats.filter_variant = p.variant;
     stats.target_wgs = (int32_t)p.target_wgs;
     if (pe && pe->on) {
