@@ -59,6 +59,7 @@ CASES = [  # n, d, nq, k, tables, bits, probes, budget, center
     (4000, 96, 7, 10, 8, 10, 2, 300, 1),     # batch under 16 queries
     (2000, 17, 30, 50, 2, 14, 0, 100, 1),    # many queries with < k candidates: padding
     (70000, 32, 64, 10, 4, 20, 2, 512, 1),   # median over a 65536-row sample
+    (500, 16, 10, 10, 64, 1, 1, 4096, 1),    # each row ~64 times in the cut: de-duplication
 ]
 
 
@@ -94,6 +95,7 @@ PRE_CASES = [  # n, d, nq, k, tables, bits, probes, budget, prerank
     (1500, 256, 12, 10, 4, 8, 1, 256, 2),     # 32 pieces: two rows per load
     (30000, 64, 40, 10, 64, 10, 4, 6000, 4),  # budget above the exact re-rank's 4096
     (40000, 32, 30, 20, 64, 9, 6, 8192, 8),   # the largest budget
+    (500, 16, 10, 10, 64, 1, 1, 8192, 4),     # 8192 entries of 500 rows: heavy de-duplication
 ]
 
 
