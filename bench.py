@@ -56,6 +56,9 @@ def main() -> int:
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--d", type=int, default=96)
     ap.add_argument("--k", type=int, default=100)
+    ap.add_argument("--data", default="gaussian", choices=["gaussian", "uniform01", "mixture"],
+                    help="headline data: unit Gaussian (seeds 4/5, the metric's), U[0,1) (seeds 0/1), "
+                         "or 256 centres U[-1,1) + N(0, 0.1^2) (seed 2, queries seed 30)")
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp8", "fp32"])
     ap.add_argument("--forms", default="bf16,fp32,fp8",
                     help="forms to time (the --precision form is the headline value)")
@@ -106,8 +109,7 @@ def main() -> int:
     n, nq, d, k = args.n, args.nq, args.d, args.k
     log(f"[bench] generating {n}x{d} database and {nq} queries")
     t0 = time.time()
-    x = ca.gen_gaussian(n, d, 4, True)
-    q = ca.gen_gaussian(nq, d, 5, True)
+    x, q = headline_data(ca, args.data, n, nq, d)
     log(f"[bench] generated in {time.time() - t0:.1f}s")
 
     res = ca.DeviceResources(0)
@@ -267,9 +269,9 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": form,
-        "data": "synthetic: seeded unit-normalised Gaussian vectors (counter-based generator)",
+        "data": DATA_DESC[args.data],
         "config": {
-            "workload": f"{n}x{d} database, {nq} queries, k={k}, {form} candidates k'=4k + FP32 re-rank"
+            "workload": f"{n}x{d} database ({args.data}), {nq} queries, k={k}, {form} candidates k'=4k + FP32 re-rank"
                         + (" + FP64 boundary" if form != "fp32" else ""),
             "n": n, "d": d, "nq": nq, "k": k, "precision": form, "kprime": 4 * k if form != "fp32" else None,
         },
@@ -322,8 +324,7 @@ def pmc_pass(args, ca) -> int:
     """One search of the headline workload with the given tuning: the program
     the traffic passes run under rocprofv3 --pmc."""
     n, nq, d, k = args.n, args.nq, args.d, args.k
-    x = ca.gen_gaussian(n, d, 4, True)
-    q = ca.gen_gaussian(nq, d, 5, True)
+    x, q = headline_data(ca, args.data, n, nq, d)
     res = ca.DeviceResources(0)
     idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=args.precision, kprime_factor=4))
     idx.add(x)
@@ -372,7 +373,8 @@ def pmc_traffic(args, tuning) -> dict:
     tune = json.dumps({key: tuning[key] for key in tuning}) if tuning else ""
     progs = {
         "filter": [sys.executable, me, "--mode", "pmc-pass", "--precision", args.precision,
-                   "--n", str(args.n), "--nq", str(args.nq), "--d", str(args.d), "--k", str(args.k)]
+                   "--n", str(args.n), "--nq", str(args.nq), "--d", str(args.d), "--k", str(args.k),
+                   "--data", args.data]
                   + (["--tuning", tune] if tune else []),
         "copy": [sys.executable, me, "--mode", "copy", "--copy-gib", "1", "--steps", "2"],
     }
@@ -635,6 +637,23 @@ def rplsh_front_end(args, ca) -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+DATA_DESC = {
+    "gaussian": "synthetic: seeded unit-normalised Gaussian vectors (counter-based generator)",
+    "uniform01": "synthetic: seeded U[0,1) vectors (counter-based generator)",
+    "mixture": "synthetic: seeded mixture of 256 centres U[-1,1) with N(0, 0.1^2) noise",
+}
+
+
+def headline_data(ca, name: str, n: int, nq: int, d: int):
+    """(x, q) of the headline workload: the metric's unit Gaussian vectors, or
+    the uniform / clustered sets of the smaller configurations."""
+    if name == "uniform01":
+        return ca.gen_uniform(n, d, 0, 0.0, 1.0), ca.gen_uniform(nq, d, 1, 0.0, 1.0)
+    if name == "mixture":
+        return ca.gen_mixture(n, d, 256, 0.1, 2, 3), ca.gen_mixture(nq, d, 256, 0.1, 2, 30)
+    return ca.gen_gaussian(n, d, 4, True), ca.gen_gaussian(nq, d, 5, True)
 
 
 def ca_recall(I, truth) -> float:
