@@ -56,9 +56,12 @@ def main() -> int:
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--d", type=int, default=96)
     ap.add_argument("--k", type=int, default=100)
-    ap.add_argument("--data", default="gaussian", choices=["gaussian", "uniform01", "mixture"],
+    ap.add_argument("--kprime-factor", type=int, default=4,
+                    help="k'/k of the BF16/FP8 forms (the metric's setting: 4)")
+    ap.add_argument("--data", default="gaussian", choices=["gaussian", "uniform01", "mixture", "mixture1024"],
                     help="headline data: unit Gaussian (seeds 4/5, the metric's), U[0,1) (seeds 0/1), "
-                         "or 256 centres U[-1,1) + N(0, 0.1^2) (seed 2, queries seed 30)")
+                         "256 centres U[-1,1) + N(0, 0.1^2) (seed 2, queries seed 30), or 1024 centres "
+                         "+ N(0, 0.05^2) with 5%% exact duplicate rows (seed 6, queries seed 7)")
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp8", "fp32"])
     ap.add_argument("--forms", default="bf16,fp32,fp8",
                     help="forms to time (the --precision form is the headline value)")
@@ -118,7 +121,7 @@ def main() -> int:
         forms.insert(0, args.precision)
     forms.sort(key=lambda f: f != args.precision)
 
-    idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=args.precision, kprime_factor=4))
+    idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=args.precision, kprime_factor=args.kprime_factor))
     t0 = time.time()
     idx.add(x)
     log(f"[bench] index built (add incl. H2D, norms, encode) in {time.time() - t0:.1f}s")
@@ -129,10 +132,10 @@ def main() -> int:
     results = {}
     tunings = {}
     for form in forms:
-        idx.set_precision(form, 4)
+        idx.set_precision(form, args.kprime_factor)
         if not args.no_tune:
             # Untimed: tunes grid size, survivor target and filter variant for
-            # this batch; target_recall 0 keeps k' = 4k (the metric's setting).
+            # this batch; target_recall 0 keeps the set k'/k (the metric's: 4).
             t0 = time.time()
             tunings[form] = idx.autotune(q[:256], k, nq, 0.0)
             log(f"[bench] {form} autotune in {time.time() - t0:.1f}s: {tunings[form]}")
@@ -271,9 +274,9 @@ def main() -> int:
         "dtype": form,
         "data": DATA_DESC[args.data],
         "config": {
-            "workload": f"{n}x{d} database ({args.data}), {nq} queries, k={k}, {form} candidates k'=4k + FP32 re-rank"
+            "workload": f"{n}x{d} database ({args.data}), {nq} queries, k={k}, {form} candidates k'={args.kprime_factor}k + FP32 re-rank"
                         + (" + FP64 boundary" if form != "fp32" else ""),
-            "n": n, "d": d, "nq": nq, "k": k, "precision": form, "kprime": 4 * k if form != "fp32" else None,
+            "n": n, "d": d, "nq": nq, "k": k, "precision": form, "kprime": args.kprime_factor * k if form != "fp32" else None,
         },
         "roofline": {
             "bound": "mfma",
@@ -326,7 +329,7 @@ def pmc_pass(args, ca) -> int:
     n, nq, d, k = args.n, args.nq, args.d, args.k
     x, q = headline_data(ca, args.data, n, nq, d)
     res = ca.DeviceResources(0)
-    idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=args.precision, kprime_factor=4))
+    idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=args.precision, kprime_factor=args.kprime_factor))
     idx.add(x)
     if args.tuning:
         idx.set_tuning(json.loads(args.tuning))
@@ -374,7 +377,7 @@ def pmc_traffic(args, tuning) -> dict:
     progs = {
         "filter": [sys.executable, me, "--mode", "pmc-pass", "--precision", args.precision,
                    "--n", str(args.n), "--nq", str(args.nq), "--d", str(args.d), "--k", str(args.k),
-                   "--data", args.data]
+                   "--data", args.data, "--kprime-factor", str(args.kprime_factor)]
                   + (["--tuning", tune] if tune else []),
         "copy": [sys.executable, me, "--mode", "copy", "--copy-gib", "1", "--steps", "2"],
     }
@@ -643,6 +646,8 @@ DATA_DESC = {
     "gaussian": "synthetic: seeded unit-normalised Gaussian vectors (counter-based generator)",
     "uniform01": "synthetic: seeded U[0,1) vectors (counter-based generator)",
     "mixture": "synthetic: seeded mixture of 256 centres U[-1,1) with N(0, 0.1^2) noise",
+    "mixture1024": "synthetic: seeded mixture of 1024 centres U[-1,1) with N(0, 0.05^2) noise, "
+                   "5% of rows exact copies of other rows",
 }
 
 
@@ -653,6 +658,8 @@ def headline_data(ca, name: str, n: int, nq: int, d: int):
         return ca.gen_uniform(n, d, 0, 0.0, 1.0), ca.gen_uniform(nq, d, 1, 0.0, 1.0)
     if name == "mixture":
         return ca.gen_mixture(n, d, 256, 0.1, 2, 3), ca.gen_mixture(nq, d, 256, 0.1, 2, 30)
+    if name == "mixture1024":
+        return ca.gen_mixture(n, d, 1024, 0.05, 6, 60, 0.05), ca.gen_mixture(nq, d, 1024, 0.05, 6, 7)
     return ca.gen_gaussian(n, d, 4, True), ca.gen_gaussian(nq, d, 5, True)
 
 
