@@ -66,7 +66,7 @@ def main() -> int:
     ap.add_argument("--forms", default="bf16,fp32,fp8",
                     help="forms to time (the --precision form is the headline value)")
     ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--recall-queries", type=int, default=200)
+    ap.add_argument("--recall-queries", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true",
@@ -527,6 +527,11 @@ def serving(args, ca) -> int:
         "mean_ms": round(st["mean_ms"], 3),
         "max_ms": round(st["max_ms"], 3),
         "batches": st["batches"],
+        "device_ms_p50": round(float(np.percentile([inf["device_ms"] for inf in infos], 50)), 3),
+        "device_ms_p99": round(float(np.percentile([inf["device_ms"] for inf in infos], 99)), 3),
+        "queue_ms_p50": round(float(np.percentile([inf["queue_ms"] for inf in infos], 50)), 3),
+        "graph_launches": st["graph_launches"],
+        "graph_captures": st["graph_captures"],
         "co_batched_requests": sum(1 for inf in infos if inf["batch_requests"] > 1),
         "peak_inflight": st["peak_inflight"],
         "peak_reserved_gb": round(st["peak_reserved_bytes"] / 2**30, 3),

@@ -106,6 +106,7 @@ class RequestInfo(C.Structure):
         ("queue_ms", C.c_double),
         ("batch_requests", C.c_int),
         ("batch_queries", C.c_int64),
+        ("device_ms", C.c_float),
     ]
 
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <vector>
 
@@ -54,6 +55,47 @@ struct DevBuf {
   }
 };
 
+// Growable pinned host buffer (hipHostMalloc): staging for request I/O and
+// counter read-backs, so every asynchronous copy has a page-locked end.
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  HostBuf(HostBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  HostBuf& operator=(HostBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipHostFree(p);
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void* ensure(size_t b) {
+    if (b <= bytes) return p;
+    if (p) RPLSH_HIP(hipHostFree(p));
+    p = nullptr;
+    bytes = 0;
+    size_t nb = std::max(b, (size_t)4096);
+    RPLSH_HIP(hipHostMalloc(&p, nb, hipHostMallocDefault));
+    bytes = nb;
+    return p;
+  }
+  template <typename T>
+  T* as(size_t count) {
+    return static_cast<T*>(ensure(count * sizeof(T)));
+  }
+};
+
 // Per-batch counters, cleared at the start of a batch and read back with one
 // copy after it.
 struct BatchCounters {
@@ -88,6 +130,7 @@ struct Workspace {
   DevBuf qnorm, qenc, qscale, qpad, tau, tau2, counts, cand, topk, bound, status, relk;
   DevBuf flist, ctr;  // failed-query list; BatchCounters
   DevBuf qsub, isub, dsub;  // a retried subset of queries
+  HostBuf hctr;             // pinned read-back target of ctr
 };
 
 // The database as the search kernels see it.
@@ -200,4 +243,17 @@ struct rplsh_bf {
   rplsh::DevBuf qdev, idev, ddev;
   rplsh_search_stats last{};
   std::vector<rplsh_tune_result> tunes;  // autotuned parameters per (k, batch size)
+  // Servers holding the index (rplsh_server_add_index until the server is
+  // destroyed): while non-zero every call that changes the rows, their
+  // encoding or the tuning fails with RPLSH_ERR_STATE.
+  std::atomic<int> serving{0};
+  // Bumped by every such change; cached serving graphs record the value they
+  // were captured at and are dropped when it moved.
+  std::atomic<uint64_t> generation{0};
 };
+
+namespace rplsh {
+// Fails with RPLSH_ERR_STATE while a server holds the index; otherwise bumps
+// its generation (the caller is about to change it).
+void bf_begin_change(rplsh_bf* b);
+}  // namespace rplsh

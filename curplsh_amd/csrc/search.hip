@@ -352,14 +352,12 @@ void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& 
   Workspace& ws = wss.at(depth);
   const int d = db.d;
   const int kp = (int)std::min<int64_t>(p.kp, db.n);
-  // One read-back of the batch's counters (a pageable target: workspaces are
-  // created per cached graph, and a pinned allocation per workspace would cost
-  // more than it saves).
-  BatchCounters hc{};
-  RPLSH_HIP(hipMemcpyAsync(&hc, ws.ctr.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
+  // One read-back of the batch's counters into the workspace's pinned buffer.
+  BatchCounters* hc = ws.hctr.as<BatchCounters>(1);
+  RPLSH_HIP(hipMemcpyAsync(hc, ws.ctr.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
   RPLSH_HIP(hipStreamSynchronize(st));
-  const int nfail = hc.failed, nfp64 = hc.fp64;
-  const long long nsurv = hc.survivors;
+  const int nfail = hc->failed, nfp64 = hc->fp64;
+  const long long nsurv = hc->survivors;
   stats.boundary_fp64 += nfp64;
   if (depth == 0) {
     const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
@@ -634,6 +632,12 @@ SearchParams bf_search_params(const rplsh_bf* b, int k, int64_t nq) {
 
 int bf_device_of(const rplsh_bf* b) { return b->cfg.device; }
 
+void bf_begin_change(rplsh_bf* b) {
+  if (b->serving.load() > 0)
+    fail(RPLSH_ERR_STATE, "index is registered with a running server and cannot be changed");
+  b->generation.fetch_add(1);
+}
+
 void bf_prepare(rplsh_bf* b) {
   if (b->enc_precision == b->cfg.precision || b->n == 0) return;
   DeviceScope scope(b->cfg.device);
@@ -688,6 +692,8 @@ rplsh_status rplsh_bf_create(rplsh_resources* res, int dim, const rplsh_bf_confi
 rplsh_status rplsh_bf_destroy(rplsh_bf* b) {
   return guard([&] {
     if (!b) return;
+    if (b->serving.load() > 0)
+      fail(RPLSH_ERR_STATE, "index is registered with a running server and cannot be destroyed");
     DeviceScope scope(b->cfg.device);
     (void)hipDeviceSynchronize();
     bf_free_data(b);
@@ -702,6 +708,7 @@ rplsh_status rplsh_bf_train(rplsh_bf* b, int64_t, const float*) {
 rplsh_status rplsh_bf_reset(rplsh_bf* b) {
   return guard([&] {
     RPLSH_REQUIRE(b, "index is NULL");
+    bf_begin_change(b);
     DeviceScope scope(b->cfg.device);
     RPLSH_HIP(hipStreamSynchronize(bf_stream(b)));
     bf_free_data(b);
@@ -713,6 +720,7 @@ rplsh_status rplsh_bf_add(rplsh_bf* b, int64_t num, const float* data) {
     RPLSH_REQUIRE(b, "index is NULL");
     if (num <= 0) return;
     RPLSH_REQUIRE(data, "data is NULL");
+    bf_begin_change(b);
     RPLSH_REQUIRE(b->n + num < (1ll << 31), "index size limited to 2^31 rows");
     DeviceScope scope(b->cfg.device);
     hipStream_t st = bf_stream(b);
@@ -766,6 +774,7 @@ rplsh_status rplsh_bf_set_precision(rplsh_bf* b, int precision, int kprime_facto
     RPLSH_REQUIRE(b, "index is NULL");
     (void)elem_of(precision);
     RPLSH_REQUIRE(filter_dpad(elem_of(precision), b->dim) > 0, "dimension not supported");
+    bf_begin_change(b);
     DeviceScope scope(b->cfg.device);
     if (b->cfg.precision != precision) b->tunes.clear();
     b->cfg.precision = precision;
@@ -971,6 +980,7 @@ rplsh_status rplsh_bf_autotune(rplsh_bf* b, const float* sample, int64_t nsample
     RPLSH_REQUIRE(b && sample && out, "NULL argument");
     RPLSH_REQUIRE(nsample > 0 && k > 0 && batch_nq > 0, "bad sizes");
     RPLSH_REQUIRE(b->n > 0, "autotune needs a non-empty index");
+    bf_begin_change(b);
     DeviceScope scope(b->cfg.device);
     bf_prepare(b);
     hipStream_t st = bf_stream(b);
@@ -1145,6 +1155,7 @@ rplsh_status rplsh_bf_set_tuning(rplsh_bf* b, const rplsh_tune_result* t) {
     RPLSH_REQUIRE(t->kprime_factor >= 0 && (int64_t)t->kprime_factor * t->k <= 4096, "k' too large");
     RPLSH_REQUIRE(t->variant >= kFilterLdsWide && t->variant <= kFilterLdsWide8, "unknown filter variant %d", t->variant);
     RPLSH_REQUIRE(t->rho >= 0 && t->rho <= 256, "bad rho %d", t->rho);
+    bf_begin_change(b);
     b->tunes.erase(std::remove_if(b->tunes.begin(), b->tunes.end(),
                                   [&](const rplsh_tune_result& x) {
                                     return x.k == t->k && x.batch_nq == t->batch_nq;
@@ -1157,6 +1168,7 @@ rplsh_status rplsh_bf_set_tuning(rplsh_bf* b, const rplsh_tune_result* t) {
 rplsh_status rplsh_bf_clear_tuning(rplsh_bf* b) {
   return guard([&] {
     RPLSH_REQUIRE(b, "index is NULL");
+    bf_begin_change(b);
     b->tunes.clear();
   });
 }

@@ -50,6 +50,7 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 struct Request {
   uint64_t id = 0;
   int index = 0;
+  rplsh_bf* b = nullptr;  // resolved under the lock at submit
   int64_t nq = 0;
   const float* q = nullptr;
   int k = 0;
@@ -61,12 +62,50 @@ struct Request {
   std::string err;
 };
 
-// A captured pipeline for (index, k, bucket): fixed device buffers for up to
-// `bucket` queries, the live count in nqd, its own workspaces.
-struct GraphEntry {
+// Everything a captured pipeline depends on besides its buffers: the index
+// and its generation (bumped by any change to it), k, the batch bucket and
+// the resolved search parameters. A batch replays a graph only if all match.
+struct GraphKey {
   int index = -1;
-  int k = 0;
-  int bucket = 0;
+  uint64_t gen = 0;
+  int k = 0, bucket = 0;
+  int kp = 0, c_factor = 0, target_wgs = 0, variant = 0, rho = 0, fp64_boundary = 0;
+  bool certify = false;
+  bool operator==(const GraphKey& o) const {
+    return index == o.index && gen == o.gen && k == o.k && bucket == o.bucket && kp == o.kp &&
+           c_factor == o.c_factor && target_wgs == o.target_wgs && variant == o.variant &&
+           rho == o.rho && fp64_boundary == o.fp64_boundary && certify == o.certify;
+  }
+  uint64_t hash() const {
+    uint64_t h = 1469598103934665603ull;
+    for (uint64_t v : {(uint64_t)index, gen, (uint64_t)k, (uint64_t)bucket, (uint64_t)kp,
+                       (uint64_t)c_factor, (uint64_t)target_wgs, (uint64_t)variant, (uint64_t)rho,
+                       (uint64_t)fp64_boundary, (uint64_t)certify})
+      h = (h ^ v) * 1099511628211ull;
+    return h;
+  }
+};
+
+GraphKey make_key(int index, const rplsh_bf* b, int k, int bucket, const SearchParams& p) {
+  GraphKey g;
+  g.index = index;
+  g.gen = b->generation.load();
+  g.k = k;
+  g.bucket = bucket;
+  g.kp = p.kp;
+  g.c_factor = p.c_factor;
+  g.target_wgs = p.target_wgs;
+  g.variant = p.variant;
+  g.rho = p.rho;
+  g.fp64_boundary = p.fp64_boundary;
+  g.certify = p.certify;
+  return g;
+}
+
+// A captured pipeline: fixed device buffers for up to key.bucket queries,
+// the live count in nqd, its own workspaces.
+struct GraphEntry {
+  GraphKey key;
   hipGraphExec_t exec = nullptr;
   std::vector<Workspace> wss;
   DevBuf q, I, D, nqd;
@@ -81,7 +120,10 @@ struct Slot {
   hipStream_t st = nullptr;
   std::vector<Workspace> wss;
   DevBuf qbuf, ibuf, dbuf;
-  std::vector<std::unique_ptr<GraphEntry>> graphs;
+  HostBuf hq, hi, hd;  // pinned staging of the batch's queries and results
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // device time of a batch
+  std::vector<std::unique_ptr<GraphEntry>> graphs;  // touched by this slot's worker only
+  size_t graph_bytes = 0;
   std::thread th;
 };
 
@@ -96,7 +138,8 @@ struct rplsh_server {
   std::deque<Request*> queue;
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
   uint64_t next_id = 1;
-  size_t reserved = 0;
+  size_t reserved = 0;     // workspaces of admitted batches (eager or capturing)
+  size_t graph_bytes = 0;  // workspaces held by cached graphs (all slots)
   int inflight = 0;
   bool stop = false;
   std::vector<std::unique_ptr<Slot>> slots;
@@ -108,19 +151,36 @@ struct rplsh_server {
   bool have_first = false;
   Clock::time_point first_submit, last_done;
   int64_t graph_launches = 0, graph_captures = 0;
-  size_t graph_bytes = 0;  // sum over slots of cached graph workspaces
   uint64_t use_clock = 0;
-  // Batches seen per (index, k, bucket): a graph is captured on a key's second
-  // batch, so one-off shapes never pay for a capture.
+  // Batches seen per graph key: a graph is captured on a key's second batch,
+  // so one-off shapes never pay for a capture.
   std::unordered_map<uint64_t, int64_t> shape_seen;
 };
 
 namespace {
 
+// Retried queries of one batch get at most this many queries' worth of
+// workspace in the admission count (a retry re-runs only the failed queries,
+// with 4x the survivor margin and k'; more failures than this still run, on
+// memory the budget did not see).
+constexpr int64_t kRetryReserveQueries = 64;
+
+size_t retry_bytes(const DbView& v, int64_t nq, const SearchParams& p) {
+  SearchParams p2 = p;
+  p2.c_factor = p.c_factor * 4;
+  p2.kp = (int)std::min<int64_t>(std::min<int64_t>(v.n, 4096), (int64_t)p.kp * 4);
+  const int64_t r = std::min<int64_t>(nq, kRetryReserveQueries);
+  return search_workspace_bytes(v, (int)r, p2) + (size_t)r * (v.d * 4 + (size_t)p.k * 8);
+}
+
+// Device bytes an eager batch of nq queries needs: its depth-0 workspace with
+// the parameters it will run with, its query/result buffers and the retry
+// reserve.
 size_t batch_bytes(rplsh_bf* b, int64_t nq, int k) {
   const DbView v = bf_db_view(b);
-  const SearchParams p = bf_search_params(b, k);
-  return search_workspace_bytes(v, (int)nq, p) + (size_t)nq * (v.d * 4 + (size_t)k * 8);
+  const SearchParams p = bf_search_params(b, k, nq);
+  return search_workspace_bytes(v, (int)nq, p) + (size_t)nq * (v.d * 4 + (size_t)k * 8) +
+         retry_bytes(v, nq, p);
 }
 
 void finish(rplsh_server* s, Request* r, int status) {
@@ -156,31 +216,74 @@ int bucket_of(int64_t total) {
   return b;
 }
 
-// Finds or captures the graph of (index, k, bucket) on this slot. Called
-// without the server lock; graph_bytes is adjusted under it by the caller.
-GraphEntry* slot_graph(rplsh_server* s, Slot& sl, int index, int k, int bucket, const DbView& v,
-                       const SearchParams& p, size_t& added, size_t& freed, bool& captured) {
-  added = freed = 0;
-  captured = false;
+// How a batch will run, decided at admission by the worker that admits it.
+struct Plan {
+  rplsh_bf* b = nullptr;
+  int index = 0, k = 0;
+  int64_t total = 0;
+  SearchParams p;
+  GraphEntry* g = nullptr;  // a cached graph of this slot to replay
+  bool capture = false;     // capture a graph for this batch's key
+  GraphKey key;
+  size_t need = 0;          // bytes reserved while the batch runs
+  size_t graph_part = 0;    // of need: the graph's workspace, moved to the cache once captured
+};
+
+GraphEntry* find_graph(Slot& sl, const GraphKey& key) {
   for (auto& g : sl.graphs)
-    if (g->index == index && g->k == k && g->bucket == bucket) return g.get();
+    if (g->key == key) return g.get();
+  return nullptr;
+}
+
+// Bytes a graph for this key would hold (its bucket-sized workspace and
+// buffers; the retry depths are allocated on use).
+size_t graph_bytes_for(const DbView& v, const GraphKey& key, const SearchParams& p) {
+  return search_workspace_bytes(v, key.bucket, p) + (size_t)key.bucket * (v.d * 4 + (size_t)key.k * 8) + 256;
+}
+
+// Drops this slot's graphs whose index changed since capture, then least
+// recently used ones until `want` more bytes fit in the slot's cache share.
+// Called with the server lock held, by the slot's own worker or while no
+// batch is in flight (a graph is only used by a running batch of its slot).
+void evict_graphs(rplsh_server* s, Slot& sl, size_t want) {
+  const size_t share = s->cfg.graph_cache_bytes / (size_t)std::max(1, s->cfg.slots);
+  for (auto it = sl.graphs.begin(); it != sl.graphs.end();) {
+    const GraphKey& k = (*it)->key;
+    if (k.gen != s->indexes[k.index]->generation.load()) {
+      sl.graph_bytes -= (*it)->bytes;
+      s->graph_bytes -= (*it)->bytes;
+      it = sl.graphs.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  while (!sl.graphs.empty() && sl.graph_bytes + want > share) {
+    auto it = std::min_element(sl.graphs.begin(), sl.graphs.end(),
+                               [](const auto& a, const auto& b) { return a->last_use < b->last_use; });
+    sl.graph_bytes -= (*it)->bytes;
+    s->graph_bytes -= (*it)->bytes;
+    sl.graphs.erase(it);
+  }
+}
+
+// Captures the pipeline for p.key on this slot (without the server lock).
+std::unique_ptr<GraphEntry> capture_graph(Slot& sl, const DbView& v, const Plan& pl) {
   auto g = std::make_unique<GraphEntry>();
-  g->index = index;
-  g->k = k;
-  g->bucket = bucket;
+  g->key = pl.key;
   g->wss.resize(4);
+  const int bucket = pl.key.bucket, k = pl.k;
   float* q = g->q.as<float>((size_t)bucket * v.d);
   int32_t* I = g->I.as<int32_t>((size_t)bucket * k);
   float* D = g->D.as<float>((size_t)bucket * k);
   int* nqd = g->nqd.as<int>(1);
   // Size the workspace with a run over zero live queries, then capture.
   launch_fill_i32(nqd, 1, 0, sl.st);
-  enqueue_batch(v, q, bucket, nqd, p, I, D, g->wss[0], sl.st, nullptr);
+  enqueue_batch(v, q, bucket, nqd, pl.p, I, D, g->wss[0], sl.st, nullptr);
   RPLSH_HIP(hipStreamSynchronize(sl.st));
   hipGraph_t graph = nullptr;
   RPLSH_HIP(hipStreamBeginCapture(sl.st, hipStreamCaptureModeThreadLocal));
   try {
-    enqueue_batch(v, q, bucket, nqd, p, I, D, g->wss[0], sl.st, nullptr);
+    enqueue_batch(v, q, bucket, nqd, pl.p, I, D, g->wss[0], sl.st, nullptr);
   } catch (...) {
     (void)hipStreamEndCapture(sl.st, &graph);
     if (graph) (void)hipGraphDestroy(graph);
@@ -191,97 +294,96 @@ GraphEntry* slot_graph(rplsh_server* s, Slot& sl, int index, int k, int bucket, 
   (void)hipGraphDestroy(graph);
   RPLSH_HIP(ie);
   g->bytes = workspace_bytes(g->wss) + g->q.bytes + g->I.bytes + g->D.bytes + g->nqd.bytes;
-  added = g->bytes;
-  captured = true;
-  // Evict this slot's least recently used graphs beyond its share of the cache.
-  const size_t share = s->cfg.graph_cache_bytes / std::max(1, s->cfg.slots);
-  size_t mine = added;
-  for (auto& e : sl.graphs) mine += e->bytes;
-  while (mine > share && !sl.graphs.empty()) {
-    auto it = std::min_element(sl.graphs.begin(), sl.graphs.end(),
-                               [](const auto& a, const auto& b) { return a->last_use < b->last_use; });
-    mine -= (*it)->bytes;
-    freed += (*it)->bytes;
-    sl.graphs.erase(it);
-  }
-  sl.graphs.push_back(std::move(g));
-  return sl.graphs.back().get();
+  return g;
 }
 
-// Runs one batch of requests on a slot (called without the lock).
-int run_batch(rplsh_server* s, Slot& sl, const std::vector<Request*>& batch, std::string& err) {
+// Runs one admitted batch on a slot (called without the lock). Request I/O
+// goes through the slot's pinned staging buffers: one host-to-device copy of
+// the batch's queries, one device-to-host copy of each result array.
+int run_batch(rplsh_server* s, Slot& sl, Plan& pl, const std::vector<Request*>& batch,
+              std::string& err, float& device_ms) {
   try {
-    rplsh_bf* b = s->indexes[batch[0]->index];
-    const int k = batch[0]->k;
-    const DbView v = bf_db_view(b);
-    int64_t total = 0;
-    for (auto* r : batch) total += r->nq;
-    const SearchParams p = bf_search_params(b, k, total);
-    bool graph = s->cfg.use_graphs && v.n > 0 && total <= s->cfg.graph_max_queries;
-    if (graph) {
-      const uint64_t key = ((uint64_t)batch[0]->index << 40) ^ ((uint64_t)k << 20) ^ (uint64_t)bucket_of(total);
+    const DbView v = bf_db_view(pl.b);
+    const int k = pl.k;
+    const int64_t total = pl.total;
+    if (pl.capture) {
+      std::unique_ptr<GraphEntry> g = capture_graph(sl, v, pl);
       std::lock_guard<std::mutex> lk(s->mu);
-      graph = ++s->shape_seen[key] >= 2;
+      evict_graphs(s, sl, g->bytes);
+      sl.graph_bytes += g->bytes;
+      s->graph_bytes += g->bytes;
+      s->reserved -= pl.graph_part;  // now counted in graph_bytes
+      pl.need -= pl.graph_part;
+      pl.graph_part = 0;
+      s->graph_captures += 1;
+      g->last_use = ++s->use_clock;
+      sl.graphs.push_back(std::move(g));
+      pl.g = sl.graphs.back().get();
     }
     float* q;
     int32_t* I;
     float* D;
-    GraphEntry* g = nullptr;
-    if (graph) {
-      size_t added = 0, freed = 0;
-      bool captured = false;
-      g = slot_graph(s, sl, batch[0]->index, k, bucket_of(total), v, p, added, freed, captured);
-      {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->graph_bytes += added;
-        s->graph_bytes -= freed;
-        s->graph_captures += captured ? 1 : 0;
-        s->graph_launches += 1;
-        g->last_use = ++s->use_clock;
-      }
-      q = static_cast<float*>(g->q.p);
-      I = static_cast<int32_t*>(g->I.p);
-      D = static_cast<float*>(g->D.p);
+    if (pl.g) {
+      q = static_cast<float*>(pl.g->q.p);
+      I = static_cast<int32_t*>(pl.g->I.p);
+      D = static_cast<float*>(pl.g->D.p);
     } else {
       q = sl.qbuf.as<float>((size_t)total * v.d);
       I = sl.ibuf.as<int32_t>((size_t)total * k);
       D = sl.dbuf.as<float>((size_t)total * k);
     }
+    float* hq = sl.hq.as<float>((size_t)total * v.d);
+    int32_t* hi = sl.hi.as<int32_t>((size_t)total * k);
+    float* hd = sl.hd.as<float>((size_t)total * k);
     int64_t off = 0;
     for (auto* r : batch) {
-      RPLSH_HIP(hipMemcpyAsync(q + off * v.d, r->q, (size_t)r->nq * v.d * sizeof(float),
-                               hipMemcpyHostToDevice, sl.st));
+      std::memcpy(hq + off * v.d, r->q, (size_t)r->nq * v.d * sizeof(float));
       off += r->nq;
     }
+    RPLSH_HIP(hipEventRecord(sl.ev0, sl.st));
+    RPLSH_HIP(hipMemcpyAsync(q, hq, (size_t)total * v.d * sizeof(float), hipMemcpyHostToDevice, sl.st));
     rplsh_search_stats stats{};
     if (v.n == 0) {
       // Empty index: FLT_MAX / -1 (search.cu:93-100).
       launch_fill_i32(I, total * k, -1, sl.st);
       launch_fill_f32(D, total * k, FLT_MAX, sl.st);
-    } else if (g) {
-      launch_fill_i32(static_cast<int*>(g->nqd.p), 1, (int)total, sl.st);
-      RPLSH_HIP(hipGraphLaunch(g->exec, sl.st));
-      finish_batch(v, q, (int)total, p, I, D, g->wss, 0, sl.st, stats, nullptr);
+    } else if (pl.g) {
+      launch_fill_i32(static_cast<int*>(pl.g->nqd.p), 1, (int)total, sl.st);
+      RPLSH_HIP(hipGraphLaunch(pl.g->exec, sl.st));
+      finish_batch(v, q, (int)total, pl.p, I, D, pl.g->wss, 0, sl.st, stats, nullptr);
     } else {
-      search_batch(v, q, (int)total, p, I, D, sl.wss, 0, sl.st, stats);
+      search_batch(v, q, (int)total, pl.p, I, D, sl.wss, 0, sl.st, stats);
     }
+    RPLSH_HIP(hipMemcpyAsync(hi, I, (size_t)total * k * sizeof(int32_t), hipMemcpyDeviceToHost, sl.st));
+    RPLSH_HIP(hipMemcpyAsync(hd, D, (size_t)total * k * sizeof(float), hipMemcpyDeviceToHost, sl.st));
+    RPLSH_HIP(hipEventRecord(sl.ev1, sl.st));
+    RPLSH_HIP(hipStreamSynchronize(sl.st));
+    RPLSH_HIP(hipEventElapsedTime(&device_ms, sl.ev0, sl.ev1));
     off = 0;
     for (auto* r : batch) {
-      RPLSH_HIP(hipMemcpyAsync(r->I, I + off * k, (size_t)r->nq * k * sizeof(int32_t),
-                               hipMemcpyDeviceToHost, sl.st));
-      RPLSH_HIP(hipMemcpyAsync(r->D, D + off * k, (size_t)r->nq * k * sizeof(float),
-                               hipMemcpyDeviceToHost, sl.st));
+      std::memcpy(r->I, hi + off * k, (size_t)r->nq * k * sizeof(int32_t));
+      std::memcpy(r->D, hd + off * k, (size_t)r->nq * k * sizeof(float));
       off += r->nq;
     }
-    RPLSH_HIP(hipStreamSynchronize(sl.st));
     return RPLSH_OK;
   } catch (const Error& e) {
     err = e.what();
     return e.status;
+  } catch (const std::bad_alloc&) {
+    err = "host allocation failed";
+    return RPLSH_ERR_OUT_OF_MEMORY;
   } catch (const std::exception& e) {
     err = e.what();
     return RPLSH_ERR_STATE;
   }
+}
+
+// Completes a request that can never be admitted (called with the lock held).
+void reject(rplsh_server* s, Request* r, int status, const std::string& why) {
+  r->t_admit = Clock::now();
+  r->err = why;
+  finish(s, r, status);
+  s->done_cv.notify_all();
 }
 
 void worker(rplsh_server* s, Slot* sl) {
@@ -289,7 +391,7 @@ void worker(rplsh_server* s, Slot* sl) {
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
     std::vector<Request*> batch;
-    size_t need = 0;
+    Plan pl;
     for (;;) {
       if (s->queue.empty()) {
         if (s->stop) return;
@@ -297,23 +399,28 @@ void worker(rplsh_server* s, Slot* sl) {
         continue;
       }
       Request* head = s->queue.front();
-      rplsh_bf* b = s->indexes[head->index];
+      rplsh_bf* b = head->b;
       size_t hb = 0;
+      std::string why;
       try {
         hb = batch_bytes(b, head->nq, head->k);
-      } catch (...) {
+      } catch (const std::exception& e) {
         hb = SIZE_MAX;
+        why = e.what();
       }
-      if (hb > s->cfg.hbm_budget_bytes) {
-        // Can never fit: complete it with a budget error.
+      if (hb == SIZE_MAX || hb > s->cfg.hbm_budget_bytes) {
         s->queue.pop_front();
-        head->t_admit = Clock::now();
-        head->err = "batch workspace exceeds the HBM budget";
-        finish(s, head, RPLSH_ERR_BUDGET);
-        s->done_cv.notify_all();
+        reject(s, head, hb == SIZE_MAX ? RPLSH_ERR_STATE : RPLSH_ERR_BUDGET,
+               hb == SIZE_MAX ? why : "batch workspace exceeds the HBM budget");
         continue;
       }
-      if (s->reserved + hb > s->cfg.hbm_budget_bytes) {
+      if (s->reserved + s->graph_bytes + hb > s->cfg.hbm_budget_bytes) {
+        if (s->inflight == 0) {
+          // Only cached graphs stand in the way (nothing runs, so none is in
+          // use): release them all.
+          for (auto& o : s->slots) evict_graphs(s, *o, SIZE_MAX / 2);
+          continue;
+        }
         s->cv.wait(lk);  // a finishing batch will release memory
         continue;
       }
@@ -321,13 +428,17 @@ void worker(rplsh_server* s, Slot* sl) {
       s->queue.pop_front();
       batch.push_back(head);
       int64_t total = head->nq;
-      need = hb;
+      size_t need = hb;
       for (auto it = s->queue.begin(); it != s->queue.end();) {
         Request* r = *it;
         if (r->index == head->index && r->k == head->k &&
             total + r->nq <= s->cfg.max_batch_queries) {
-          const size_t nb = batch_bytes(b, total + r->nq, head->k);
-          if (s->reserved + nb <= s->cfg.hbm_budget_bytes) {
+          size_t nb = SIZE_MAX;
+          try {
+            nb = batch_bytes(b, total + r->nq, head->k);
+          } catch (...) {
+          }
+          if (nb != SIZE_MAX && s->reserved + s->graph_bytes + nb <= s->cfg.hbm_budget_bytes) {
             batch.push_back(r);
             total += r->nq;
             need = nb;
@@ -337,34 +448,93 @@ void worker(rplsh_server* s, Slot* sl) {
         }
         ++it;
       }
+      pl.b = b;
+      pl.index = head->index;
+      pl.k = head->k;
+      pl.total = total;
+      pl.need = need;
+      try {
+        pl.p = bf_search_params(b, pl.k, total);
+        const DbView v = bf_db_view(b);
+        if (s->cfg.use_graphs && v.n > 0 && total <= s->cfg.graph_max_queries) {
+          pl.key = make_key(pl.index, b, pl.k, bucket_of(total), pl.p);
+          pl.g = find_graph(*sl, pl.key);
+          if (pl.g) {
+            pl.need = retry_bytes(v, total, pl.p);
+            pl.g->last_use = ++s->use_clock;
+          } else if (++s->shape_seen[pl.key.hash()] >= 2) {
+            // Capture if the graph's workspace fits beside what runs now.
+            const size_t gw = graph_bytes_for(v, pl.key, pl.p);
+            const size_t rb = retry_bytes(v, total, pl.p);
+            if (s->reserved + s->graph_bytes + gw + rb <= s->cfg.hbm_budget_bytes &&
+                gw <= s->cfg.graph_cache_bytes / (size_t)std::max(1, s->cfg.slots)) {
+              pl.capture = true;
+              pl.need = gw + rb;
+              pl.graph_part = gw;
+            }
+          }
+        }
+      } catch (...) {
+        pl.g = nullptr;
+        pl.capture = false;
+      }
       break;
     }
     const auto now = Clock::now();
     for (auto* r : batch) {
       r->t_admit = now;
       r->info.batch_requests = (int)batch.size();
+      r->info.batch_queries = pl.total;
     }
-    int64_t total = 0;
-    for (auto* r : batch) total += r->nq;
-    for (auto* r : batch) r->info.batch_queries = total;
-    s->reserved += need;
+    s->reserved += pl.need;
     s->inflight += 1;
     s->batches += 1;
-    s->peak_reserved = std::max(s->peak_reserved, s->reserved);
+    s->peak_reserved = std::max(s->peak_reserved, s->reserved + s->graph_bytes);
     s->peak_inflight = std::max(s->peak_inflight, s->inflight);
+    const bool replay = pl.g != nullptr || pl.capture;
+    if (replay) s->graph_launches += 1;
     lk.unlock();
     std::string err;
-    const int st = run_batch(s, *sl, batch, err);
+    float device_ms = 0.0f;
+    const int st = run_batch(s, *sl, pl, batch, err, device_ms);
     lk.lock();
-    s->reserved -= need;
+    s->reserved -= pl.need;
     s->inflight -= 1;
     for (auto* r : batch) {
       r->err = err;
+      r->info.device_ms = device_ms;
       finish(s, r, st);
     }
     s->done_cv.notify_all();
     s->cv.notify_all();
   }
+}
+
+void destroy_slots(rplsh_server* s) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  for (auto& sl : s->slots)
+    if (sl->th.joinable()) sl->th.join();
+  DeviceScope scope(s->cfg.device);
+  for (auto& sl : s->slots) {
+    if (sl->st) (void)hipStreamSynchronize(sl->st);
+    sl->graphs.clear();
+    sl->wss.clear();
+    sl->qbuf = DevBuf();
+    sl->ibuf = DevBuf();
+    sl->dbuf = DevBuf();
+    sl->hq = HostBuf();
+    sl->hi = HostBuf();
+    sl->hd = HostBuf();
+    if (sl->ev0) (void)hipEventDestroy(sl->ev0);
+    if (sl->ev1) (void)hipEventDestroy(sl->ev1);
+    if (sl->st) (void)hipStreamDestroy(sl->st);
+  }
+  for (rplsh_bf* b : s->indexes) b->serving.fetch_sub(1);
+  s->indexes.clear();
 }
 
 }  // namespace
@@ -408,13 +578,14 @@ rplsh_status rplsh_server_create(const rplsh_server_config* cfg, rplsh_server** 
       for (int i = 0; i < c.slots; ++i) {
         auto sl = std::make_unique<Slot>();
         RPLSH_HIP(hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking));
+        RPLSH_HIP(hipEventCreate(&sl->ev0));
+        RPLSH_HIP(hipEventCreate(&sl->ev1));
         sl->wss.resize(4);
         s->slots.push_back(std::move(sl));
       }
       for (auto& sl : s->slots) sl->th = std::thread(worker, s, sl.get());
     } catch (...) {
-      for (auto& sl : s->slots)
-        if (sl->st) (void)hipStreamDestroy(sl->st);
+      destroy_slots(s);  // stops and joins the workers already started
       delete s;
       throw;
     }
@@ -425,22 +596,7 @@ rplsh_status rplsh_server_create(const rplsh_server_config* cfg, rplsh_server** 
 rplsh_status rplsh_server_destroy(rplsh_server* s) {
   return guard([&] {
     if (!s) return;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->stop = true;
-    }
-    s->cv.notify_all();
-    for (auto& sl : s->slots)
-      if (sl->th.joinable()) sl->th.join();
-    DeviceScope scope(s->cfg.device);
-    for (auto& sl : s->slots) {
-      sl->graphs.clear();
-      sl->wss.clear();
-      sl->qbuf = DevBuf();
-      sl->ibuf = DevBuf();
-      sl->dbuf = DevBuf();
-      (void)hipStreamDestroy(sl->st);
-    }
+    destroy_slots(s);
     delete s;
   });
 }
@@ -451,7 +607,9 @@ rplsh_status rplsh_server_add_index(rplsh_server* s, rplsh_bf* index, int* index
     RPLSH_REQUIRE(bf_device_of(index) == s->cfg.device, "index lives on another device");
     bf_prepare(index);
     std::lock_guard<std::mutex> lk(s->mu);
+    RPLSH_REQUIRE(!s->stop, "server is stopping");
     s->indexes.push_back(index);
+    index->serving.fetch_add(1);
     *index_id = (int)s->indexes.size() - 1;
   });
 }
@@ -468,6 +626,7 @@ rplsh_status rplsh_server_submit(rplsh_server* s, int index_id, int64_t nq, cons
     auto r = std::make_unique<Request>();
     r->id = s->next_id++;
     r->index = index_id;
+    r->b = s->indexes[index_id];
     r->nq = nq;
     r->q = queries;
     r->k = k;
@@ -574,6 +733,7 @@ rplsh_status rplsh_server_reset_stats(rplsh_server* s) {
     s->peak_inflight = 0;
     s->have_first = false;
     s->graph_launches = s->graph_captures = 0;
+    s->peak_reserved = s->reserved + s->graph_bytes;
   });
 }
 

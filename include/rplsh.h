@@ -372,9 +372,17 @@ rplsh_status rplsh_lsh_hash_queries(rplsh_lsh* index, int64_t nq, const float* q
  * never fit is completed with RPLSH_ERR_BUDGET. Batches up to
  * graph_max_queries run as a replay of a hipGraph captured once per (index,
  * k, power-of-two batch bucket); the live query count is read on device, so
- * one graph serves every batch size of its bucket. Results equal those of
- * rplsh_bf_search on the same queries. Registered indexes must not be
- * modified (add/reset/set_precision) while the server runs. */
+ * one graph serves every batch size of its bucket; a graph is keyed by the
+ * index, k, the bucket and the search parameters the batch resolves to (the
+ * index's tuning entry nearest its size), so a replay runs exactly what an
+ * eager batch would. Results equal those of rplsh_bf_search on the same
+ * queries. The budget covers the workspaces of admitted batches (plus a
+ * reserve for re-running up to 64 failed queries each) and those held by
+ * cached graphs. Request queries and results pass through pinned staging
+ * buffers owned by each slot. While a server holds an index (from
+ * rplsh_server_add_index until rplsh_server_destroy), rplsh_bf_add, _reset,
+ * _set_precision, _autotune, _set_tuning, _clear_tuning and _destroy on it
+ * fail with RPLSH_ERR_STATE. */
 typedef struct rplsh_server rplsh_server;
 
 typedef struct rplsh_server_config {
@@ -395,6 +403,8 @@ typedef struct rplsh_request_info {
   double queue_ms;      /* submit -> admitted */
   int batch_requests;   /* requests in the batch it ran in */
   int64_t batch_queries;
+  float device_ms;      /* device time of that batch: query upload, search and
+                           result download (HIP events on the slot's stream) */
 } rplsh_request_info;
 
 typedef struct rplsh_server_stats {

@@ -130,3 +130,123 @@ def test_eager_server_matches(ca, indexes):
         assert np.array_equal(I, Ia) and np.array_equal(D.view(np.uint32), Da.view(np.uint32))
     assert srv.stats()["graph_launches"] == 0
     srv.close()
+
+
+def test_registered_index_refuses_changes(ca, res):
+    """While a server holds an index, changing it (rows, encoding, tuning)
+    fails with STATE instead of leaving cached graphs and in-flight batches
+    pointing at freed buffers; after the server is gone it works again."""
+    x = ca.gen_uniform(20_000, 64, 601, -1, 1)
+    idx = ca.IndexBF(res, 64, ca.IndexBFConfig(precision="bf16"))
+    idx.add(x)
+    srv = ca.Server(slots=2)
+    sid = srv.add_index(idx)
+    q = ca.gen_uniform(16, 64, 602, -1, 1)
+    I0, D0, _ = srv.wait(srv.submit(sid, q, 10), timeout_s=100.0)
+    for op in (lambda: idx.add(x[:100]), idx.reset, lambda: idx.set_precision("fp32"),
+               idx.clear_tuning, lambda: idx.autotune(q, 10, 64, 0.0)):
+        with pytest.raises(ca.RplshError) as e:
+            op()
+        assert e.value.status == 4
+    I1, D1, _ = srv.wait(srv.submit(sid, q, 10), timeout_s=100.0)
+    assert np.array_equal(I0, I1) and np.array_equal(D0.view(np.uint32), D1.view(np.uint32))
+    srv.close()
+    idx.add(x[:100])  # allowed again
+    assert idx.num_data() == 20_100
+
+
+def test_graph_key_follows_tuning_entries(ca, res):
+    """An index with two tuning entries for one k: batches of different sizes
+    resolve to different search parameters, so they must not share a cached
+    graph. Every served result equals IndexBF.search on the same queries (which
+    resolves the entry the same way)."""
+    x = ca.gen_uniform(200_000, 96, 611, -1, 1)
+    idx = ca.IndexBF(res, 96, ca.IndexBFConfig(precision="bf16"))
+    idx.add(x)
+    base = dict(k=10, kprime_factor=4, target_wgs=4096, c_factor=3, variant=0, rho=0,
+                recall=1.0, ms_default=0.0, ms_tuned=0.0)
+    idx.set_tuning(dict(base, batch_nq=16, kprime_factor=2, c_factor=2, rho=32))
+    idx.set_tuning(dict(base, batch_nq=24, kprime_factor=8, c_factor=4, rho=16, variant=1))
+    srv = ca.Server(slots=1)
+    sid = srv.add_index(idx)
+    qs = [ca.gen_uniform(nq, 96, 620 + j, -1, 1) for j, nq in enumerate([17, 23, 18, 22, 17, 23])]
+    for q in qs:  # buckets: all 32; tuning entries: nearest of 16 / 24
+        I, D, info = srv.wait(srv.submit(sid, q, 10), timeout_s=100.0)
+        Ia, Da = idx.search(q, 10)
+        st = idx.last_stats()
+        assert st["kprime"] == 10 * (2 if abs(np.log(q.shape[0] / 16)) < abs(np.log(q.shape[0] / 24)) else 8)
+        assert np.array_equal(I, Ia) and np.array_equal(D.view(np.uint32), Da.view(np.uint32))
+    s = srv.stats()
+    print("two-entry stats:", s)
+    assert s["graph_captures"] == 2 and s["graph_launches"] >= 4
+    srv.close()
+
+
+C5_PRECISIONS = ["bf16", "bf16", "fp32", "fp8"]
+
+
+def test_c5_full_scale(ca, orc, res):
+    """The serving configuration at its stated scale: 64 resident indexes with
+    n log-uniform in [1e4, 2e6] and d from {64, 96, 128, 256}, U[-1,1) rows;
+    1,024 requests with Poisson arrivals, each to a random index with nq
+    log-uniform in [1, 2048] and k from {1, 10, 100}. Every request equals its
+    alone IndexBF.search bit for bit; requests on FP32 indexes are checked
+    against the canonical oracle on their first queries, BF16 / FP8 ones for
+    recall against the FP64 oracle."""
+    rng = np.random.default_rng(4242)
+    specs = []
+    for i in range(64):
+        n = int(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        d = int(rng.choice([64, 96, 128, 256]))
+        specs.append((n, d, C5_PRECISIONS[i % 4]))
+    idxs = []
+    for i, (n, d, prec) in enumerate(specs):
+        x = ca.gen_uniform(n, d, 80_000 + i, -1.0, 1.0)
+        ix = ca.IndexBF(res, d, ca.IndexBFConfig(precision=prec))
+        ix.add(x)
+        idxs.append(ix)
+        del x
+    print(f"[c5] 64 indexes, {sum(s[0] for s in specs)} rows", flush=True)
+    reqs = []
+    for r in range(1024):
+        i = int(rng.integers(64))
+        nq = int(np.clip(np.floor(np.exp(rng.uniform(0.0, np.log(2049.0)))), 1, 2048))
+        k = int(rng.choice([1, 10, 100]))
+        reqs.append((i, ca.gen_uniform(nq, specs[i][1], 300_000 + r, -1.0, 1.0), k))
+    alone = [idxs[i].search(q, k) for i, q, k in reqs]
+    print(f"[c5] {len(reqs)} requests, {sum(q.shape[0] for _, q, _ in reqs)} queries; alone done", flush=True)
+    srv = ca.Server(slots=4)
+    ids = [srv.add_index(ix) for ix in idxs]
+    gaps = rng.exponential(1.0 / 2000.0, size=len(reqs))
+    handles = []
+    for (i, q, k), g in zip(reqs, gaps):
+        handles.append(srv.submit(ids[i], q, k))
+        time.sleep(float(g))
+    outs = [srv.wait(h, timeout_s=300.0) for h in handles]
+    st = srv.stats()
+    print("[c5] server stats:", st, flush=True)
+    for j, ((I, D, info), (Ia, Da), (i, q, k)) in enumerate(zip(outs, alone, reqs)):
+        assert info["status"] == 0
+        assert np.array_equal(I, Ia), f"request {j}: index {i} nq {q.shape[0]} k {k}"
+        assert np.array_equal(D.view(np.uint32), Da.view(np.uint32)), f"request {j}"
+    assert st["completed"] == len(reqs)
+    assert st["peak_inflight"] >= 2
+    assert sum(1 for _, _, info in outs if info["batch_requests"] > 1) > 0
+    # oracle on a sample: the first 8 queries of up to 12 requests per precision
+    seen = {"fp32": 0, "bf16": 0, "fp8": 0}
+    for (I, D, info), (i, q, k) in zip(outs, reqs):
+        n, d, prec = specs[i]
+        if seen[prec] >= 12:
+            continue
+        seen[prec] += 1
+        x = ca.gen_uniform(n, d, 80_000 + i, -1.0, 1.0)
+        qq = q[:8]
+        if prec == "fp32":
+            Io, Do = orc.search_canonical(x, qq, k)
+            assert np.array_equal(I[:8], Io), f"index {i} k {k}"
+            assert np.array_equal(D[:8].view(np.uint32), Do.view(np.uint32))
+        else:
+            I64, _ = orc.search_fp64(x, qq, k)
+            r = orc.recall(I[:8], I64)
+            assert r >= 0.99, f"{prec} index {i} (n {n}, d {d}) k {k}: recall {r}"
+    srv.close()
