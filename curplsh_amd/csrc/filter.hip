@@ -327,6 +327,187 @@ __global__ __launch_bounds__(kBlock, 2) void filter_kernel(FilterArgs a) {
   }
 }
 
+// K-looped FP32 filter for dimensions above the register-resident kernels'
+// range (dpad > 256, a multiple of kKPanel). Neither operand stays in
+// registers: each workgroup takes kLoopBQ queries x one slice of rows, and
+// walks the slice in chunks of kLoopBM rows; for every chunk the contraction
+// runs over the padded dimension in panels of kKPanel elements, each panel of
+// rows AND of queries staged through LDS (double buffered: the next panel's
+// loads are in flight in registers while the current one is multiplied). The
+// first MFMA of every chunk takes C = -||x||^2/2, as in filter_kernel, and the
+// epilogue (max-reduce, compare, survivor append) is the same.
+// Wave w owns query tile w (32 queries) and all four 32-row tiles of a chunk.
+constexpr int kKPanel = 32;   // elements per panel (128 bytes of FP32)
+constexpr int kLoopBM = 128;  // rows per chunk
+constexpr int kLoopBQ = 128;  // queries per workgroup
+
+__global__ __launch_bounds__(kBlock, 2) void filter_kloop_kernel(FilterArgs a, int dpad) {
+  constexpr int PB = kKPanel * 4;          // bytes of a row panel
+  constexpr int RS = PB + 16;              // LDS row stride
+  constexpr int NCP = PB / 32;             // 32-byte K-chunks per panel
+  constexpr int UROW = PB / 16;            // 16-byte units per row panel
+  constexpr int XU = kLoopBM * UROW / kBlock;  // row units per thread
+  constexpr int QU = kLoopBQ * UROW / kBlock;  // query units per thread
+  static_assert(kLoopBM * UROW % kBlock == 0 && kLoopBQ * UROW % kBlock == 0, "units per thread");
+
+  __shared__ __attribute__((aligned(16))) char sx[2][kLoopBM * RS];
+  __shared__ __attribute__((aligned(16))) char sq[2][kLoopBQ * RS];
+  __shared__ __attribute__((aligned(16))) float sn[kLoopBM];
+
+  const int per_xcd = (int)(gridDim.x >> 3);
+  const int logical = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (logical >= a.num_qtiles * a.num_slices) return;
+  const int qtile = logical % a.num_qtiles;
+  const int slice = logical / a.num_qtiles;
+  const int nq = a.nq_dev ? min(*a.nq_dev, a.nq) : a.nq;
+  if (qtile * kLoopBQ >= nq) return;  // whole workgroup leaves together
+  const int64_t r_begin = (int64_t)slice * a.rows_per_slice;
+  int64_t r_end = r_begin + a.rows_per_slice;
+  if (r_end > a.m) r_end = a.m;
+  if (r_begin >= r_end) return;  // whole workgroup leaves together
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int h = lane >> 5, col = lane & 31;
+  const int qr = qtile * kLoopBQ + wave * 32 + col;  // this lane's query column
+  const float thr = qr < nq ? fmaxf(-0.5f * a.tau[qr], -FLT_MAX) : INFINITY;
+
+  const char* xb = reinterpret_cast<const char*>(a.x);
+  const char* qb = reinterpret_cast<const char*>(a.q);
+  const int64_t xrow_bytes = a.ldx * 4;
+  const int64_t qrow_bytes = (int64_t)dpad * 4;
+  const int npanels = dpad / kKPanel;
+  i32x4 px[XU], pq[QU];
+
+  // Registers <- panel p of chunk c0 (rows past r_end and bytes past the
+  // row's data read as zero; queries past nq as zero).
+  auto gload = [&](int64_t c0, int p) {
+#pragma unroll
+    for (int t = 0; t < XU; ++t) {
+      const int u = tid + kBlock * t;
+      const int r = u / UROW;
+      const int cb = p * PB + (u - r * UROW) * 16;
+      const int64_t j = c0 + r;
+      i32x4 v = {0, 0, 0, 0};
+      if (j < r_end && cb < a.dvalid_bytes)
+        v = *reinterpret_cast<const i32x4*>(xb + j * a.stride * xrow_bytes + cb);
+      px[t] = v;
+    }
+#pragma unroll
+    for (int t = 0; t < QU; ++t) {
+      const int u = tid + kBlock * t;
+      const int r = u / UROW;
+      const int cb = p * PB + (u - r * UROW) * 16;
+      const int qq = qtile * kLoopBQ + r;
+      i32x4 v = {0, 0, 0, 0};
+      if (qq < nq) v = *reinterpret_cast<const i32x4*>(qb + (int64_t)qq * qrow_bytes + cb);
+      pq[t] = v;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < XU; ++t) {
+      const int u = tid + kBlock * t;
+      const int r = u / UROW;
+      *reinterpret_cast<i32x4*>(&sx[buf][r * RS + (u - r * UROW) * 16]) = px[t];
+    }
+#pragma unroll
+    for (int t = 0; t < QU; ++t) {
+      const int u = tid + kBlock * t;
+      const int r = u / UROW;
+      *reinterpret_cast<i32x4*>(&sq[buf][r * RS + (u - r * UROW) * 16]) = pq[t];
+    }
+  };
+
+  int64_t c0 = r_begin;
+  int p = 0;
+  gload(c0, 0);
+  sstore(0);
+  int buf = 0;
+  f32x16 acc[4];
+  for (;;) {
+    if (p == 0) {
+      // Norms of this chunk (the previous chunk's epilogue read them before
+      // the barrier that ended it).
+      if (tid < kLoopBM) {
+        const int64_t j = c0 + tid;
+        sn[tid] = j < r_end ? -0.5f * a.xnorm[j * a.stride] : -INFINITY;
+      }
+    }
+    __syncthreads();
+    // Prefetch the next panel (of this chunk or the next one).
+    int64_t nc0 = c0;
+    int np = p + 1;
+    if (np == npanels) {
+      np = 0;
+      nc0 = c0 + kLoopBM;
+    }
+    const bool has_next = nc0 < r_end;
+    if (has_next) gload(nc0, np);
+    if (p == 0) {
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const f32x4 v = *reinterpret_cast<const f32x4*>(&sn[mt * 32 + 8 * j + 4 * h]);
+          acc[mt][4 * j + 0] = v[0];
+          acc[mt][4 * j + 1] = v[1];
+          acc[mt][4 * j + 2] = v[2];
+          acc[mt][4 * j + 3] = v[3];
+        }
+    }
+    const char* bp = &sq[buf][(wave * 32 + col) * RS + 16 * h];
+#pragma unroll
+    for (int c = 0; c < NCP; ++c) {
+      const Frag<16> bf = load_frag<16>(bp + 32 * c);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const Frag<16> af = load_frag<16>(&sx[buf][(mt * 32 + col) * RS + 16 * h + 32 * c]);
+        acc[mt] = mfma_chunk<Elem::F32>(af, bf, acc[mt], 127, 127);
+      }
+    }
+    if (p + 1 == npanels) {
+      // Epilogue of the chunk: survivors are rows with acc >= thr.
+      float mx[4];
+      bool hit = false;
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        float m = acc[mt][0];
+#pragma unroll
+        for (int i = 1; i < 16; ++i) m = fmaxf(m, acc[mt][i]);
+        mx[mt] = m;
+        hit |= m >= thr;
+      }
+      if (__any(hit)) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+          if (mx[mt] >= thr) {
+            int* cnt = a.counts + qr;
+            uint64_t* dst = a.cand + (int64_t)qr * a.cap;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+              const float v = acc[mt][i];
+              if (v >= thr) {
+                const int64_t row = c0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+                const int pos = atomicAdd(cnt, 1);
+                if (pos < a.cap) dst[pos] = pack_cand(-2.0f * v, (int32_t)(row * a.stride));
+              }
+            }
+          }
+        }
+      }
+    }
+    if (!has_next) break;
+    // Everyone is done reading buf ^ 1 (the barrier at the top of this step),
+    // so the next panel goes there; the barrier at the top of the next step
+    // publishes it.
+    sstore(buf ^ 1);
+    buf ^= 1;
+    c0 = nc0;
+    p = np;
+  }
+}
+
 template <Elem E, int DPAD>
 constexpr int tn_for() {
   return DPAD * ElemTraits<E>::kBytes <= 384 ? 2 : 1;
@@ -341,8 +522,19 @@ void launch_one(const FilterArgs& a, hipStream_t st) {
   RPLSH_HIP(hipGetLastError());
 }
 
+void launch_kloop(int dpad, const FilterArgs& a, hipStream_t st) {
+  RPLSH_REQUIRE(dpad % kKPanel == 0 && dpad > 256, "filter: K-looped dimension %d", dpad);
+  const int total = a.num_qtiles * a.num_slices;
+  const unsigned grid = (unsigned)round_up(total, 8);
+  filter_kloop_kernel<<<grid, kBlock, 0, st>>>(a, dpad);
+  RPLSH_HIP(hipGetLastError());
+}
+
 template <Elem E>
 void launch_elem(int dpad, const FilterArgs& a, hipStream_t st) {
+  if constexpr (E == Elem::F32) {
+    if (dpad > 256) return launch_kloop(dpad, a, st);
+  }
   switch (dpad) {
     case 64: return launch_one<E, 64>(a, st);
     case 128: return launch_one<E, 128>(a, st);
@@ -364,7 +556,12 @@ void launch_elem(int dpad, const FilterArgs& a, hipStream_t st) {
 }  // namespace
 
 int filter_dpad(Elem e, int d) {
-  if (d <= 0 || d > 256) return 0;
+  if (d <= 0) return 0;
+  if (d > 256) {
+    // FP32 only: the K-looped kernel, dimension padded to whole panels.
+    if (e != Elem::F32 || d > kMaxKLoopDim) return 0;
+    return (int)round_up(d, kKPanel);
+  }
   if (e == Elem::FP8) {
     // 64-element K-chunks.
     if (d <= 64) return 64;
@@ -378,6 +575,7 @@ int filter_dpad(Elem e, int d) {
 }
 
 int filter_block_queries(Elem e, int dpad) {
+  if (e == Elem::F32 && dpad > 256) return kLoopBQ;
   const int bytes = e == Elem::F32 ? 4 : (e == Elem::BF16 ? 2 : 1);
   return 4 * 32 * (dpad * bytes <= 384 ? 2 : 1);
 }

@@ -28,7 +28,9 @@ void launch_max_reduce(const float* v, int64_t n, float* out, hipStream_t st);
 enum class Elem { F32 = 0, BF16 = 1, FP8 = 2 };
 
 // Padded row length (elements) the filter kernels use for dimension d, or 0
-// if d is not supported.
+// if d is not supported: up to 256 for every element kind (register-resident
+// query fragments), above that FP32 only (K-looped kernel) up to kMaxKLoopDim.
+constexpr int kMaxKLoopDim = 4096;
 int filter_dpad(Elem e, int d);
 // Queries per workgroup of the filter kernel for (e, dpad).
 int filter_block_queries(Elem e, int dpad);

@@ -673,8 +673,10 @@ rplsh_status rplsh_bf_create(rplsh_resources* res, int dim, const rplsh_bf_confi
     rplsh_device_count(&count);
     RPLSH_REQUIRE(c.device >= 0 && c.device < count, "device %d out of range [0, %d)", c.device, count);
     RPLSH_REQUIRE(c.memory_space == 0, "only MemorySpace::Device is supported");
-    RPLSH_REQUIRE(dim <= 256, "dimension %d > 256 is not supported", dim);
     (void)elem_of(c.precision);
+    RPLSH_REQUIRE(filter_dpad(elem_of(c.precision), dim) > 0,
+                  "dimension %d is not supported by precision %d (FP32: up to %d; BF16/FP8: up to 256)",
+                  dim, c.precision, kMaxKLoopDim);
     if (c.kprime_factor <= 0) c.kprime_factor = 4;
     if (rplsh_resources_init_device(res, c.device) != RPLSH_OK)
       fail(RPLSH_ERR_STATE, "%s", rplsh_last_error());
@@ -877,7 +879,7 @@ rplsh_status rplsh_search_l2_dev(const float* x, int64_t n, int d, const float* 
                                  const float* q, int64_t nq, int k, int32_t* indices,
                                  float* distances, int exact, void* stream) {
   return guard([&] {
-    RPLSH_REQUIRE(d > 0 && d <= 256, "dimension %d unsupported", d);
+    RPLSH_REQUIRE(d > 0 && filter_dpad(Elem::F32, d) > 0, "dimension %d unsupported (1..%d)", d, kMaxKLoopDim);
     RPLSH_REQUIRE(n >= 0 && nq >= 0, "bad sizes");
     RPLSH_REQUIRE(k > 0, "k must be > 0");
     if (nq == 0) return;

@@ -526,22 +526,27 @@ __host__ __device__ inline bool rerank_staged(const float* x, int64_t ldx, int d
 // NP = 0: one lane per candidate (rows not 16-byte aligned, or a batch too
 // small to fill the device, where the staged walk's serial steps per wave
 // cost more than they save); NP = 1, 2: score_rows_staged<NP> (d <= 128,
-// d <= 256).
+// d <= 256); NP = -1: one lane per candidate for d > 256, the query kept in
+// dynamic LDS after the candidate and band regions.
 template <int NP>
 __global__ __launch_bounds__(kSelBlock) void rerank_kernel(RerankArgs a) {
   extern __shared__ __attribute__((aligned(16))) uint64_t rerank_lds[];
-  __shared__ float sq[256];
+  __shared__ float sq_fixed[NP < 0 ? 4 : 256];
   __shared__ float sxn[kSelWaves * 64];
   __shared__ int sh[4];
   uint64_t* s = rerank_lds;
   const int q = blockIdx.x;
   if (q >= a.nq || (a.nq_dev && q >= *a.nq_dev)) return;
   const int d = a.d;
+  const int kp = a.kp;
+  const int P = pow2_at_least(kp);
+  float* sq = sq_fixed;
+  if constexpr (NP < 0)
+    sq = reinterpret_cast<float*>(reinterpret_cast<char*>(rerank_lds) +
+                                  rerank_lds_bytes(P, a.fp64_boundary != 0, d, false));
   for (int j = threadIdx.x; j < d; j += blockDim.x) sq[j] = a.q[(int64_t)q * a.ldq + j];
   __syncthreads();
   const bool vec4 = rerank_staged(a.x, a.ldx, d);  // 16-byte loads also in the FP64 band
-  const int kp = a.kp;
-  const int P = pow2_at_least(kp);
   const int band_cap = rerank_band_cap(P);
   uint64_t* bw = s + rerank_words(P);  // 16-byte aligned
   double* bd = reinterpret_cast<double*>(bw + band_cap);
@@ -757,8 +762,15 @@ void launch_select(const SelectArgs& a, hipStream_t st) {
 void launch_rerank(const RerankArgs& a, hipStream_t st) {
   if (a.nq <= 0) return;
   if (a.kp > kMaxSortK) fail(RPLSH_ERR_UNSUPPORTED, "re-rank list limited to %d", kMaxSortK);
-  if (a.d > 256) fail(RPLSH_ERR_UNSUPPORTED, "re-rank dimension limited to 256");
+  if (a.d > kMaxKLoopDim) fail(RPLSH_ERR_UNSUPPORTED, "re-rank dimension limited to %d", kMaxKLoopDim);
   const int P = pow2_at_least(a.kp < 1 ? 1 : a.kp);
+  if (a.d > 256) {
+    // Query in dynamic LDS behind the candidate / band regions (16-byte aligned).
+    const size_t lds = rerank_lds_bytes(P, a.fp64_boundary != 0, a.d, false) + (size_t)round_up(a.d, 4) * 4;
+    rerank_kernel<-1><<<a.nq, kSelBlock, lds, st>>>(a);
+    RPLSH_HIP(hipGetLastError());
+    return;
+  }
   const bool staged = rerank_staged(a.x, a.ldx, a.d) && a.nq >= kStagedMinQueries;
   const size_t lds = rerank_lds_bytes(P, a.fp64_boundary != 0, a.d, staged);
   if (!staged)

@@ -152,7 +152,7 @@ def test_registered_index_refuses_changes(ca, res):
     assert np.array_equal(I0, I1) and np.array_equal(D0.view(np.uint32), D1.view(np.uint32))
     srv.close()
     idx.add(x[:100])  # allowed again
-    assert idx.num_data() == 20_100
+    assert idx.num_data == 20_100
 
 
 def test_graph_key_follows_tuning_entries(ca, res):
