@@ -15,32 +15,57 @@
 
 namespace rplsh {
 
-// Growable device buffer (hipMalloc only when a call needs more than before).
+// A fixed block of device memory handed out front to back (256-byte
+// aligned pieces); never allocates. Used where a caller cannot allocate.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  void* take(size_t b) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    if (at > cap || b > cap - at)
+      fail(RPLSH_ERR_OUT_OF_MEMORY, "scratch exhausted: %zu more bytes needed beyond the %zu-byte scratch",
+           at + b - cap, cap);
+    used = at + b;
+    return base + at;
+  }
+};
+
+// Growable device buffer (hipMalloc only when a call needs more than before),
+// or, when `arena` is set, pieces of that arena (no allocation, no free).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  Arena* arena = nullptr;
   DevBuf() = default;
+  explicit DevBuf(Arena* a) : arena(a) {}
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), arena(o.arena) {
     o.p = nullptr;
     o.bytes = 0;
   }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
-      if (p) (void)hipFree(p);
+      if (p && !arena) (void)hipFree(p);
       p = o.p;
       bytes = o.bytes;
+      arena = o.arena;
       o.p = nullptr;
       o.bytes = 0;
     }
     return *this;
   }
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && !arena) (void)hipFree(p);
   }
   void* ensure(size_t b) {
     if (b <= bytes) return p;
+    if (arena) {
+      p = arena->take(b);
+      bytes = b;
+      return p;
+    }
     if (p) RPLSH_HIP(hipFree(p));
     p = nullptr;
     bytes = 0;
@@ -131,6 +156,14 @@ struct Workspace {
   DevBuf flist, ctr;  // failed-query list; BatchCounters
   DevBuf qsub, isub, dsub;  // a retried subset of queries
   HostBuf hctr;             // pinned read-back target of ctr
+  bool no_alloc = false;  // buffers from an arena; ctr read back into host stack memory; no events
+  Workspace() = default;
+  // Every device buffer taken from `a` (nothing allocated, counters read
+  // back into pageable stack memory).
+  explicit Workspace(Arena* a)
+      : qnorm(a), qenc(a), qscale(a), qpad(a), tau(a), tau2(a), counts(a), cand(a), topk(a),
+        bound(a), status(a), relk(a), flist(a), ctr(a), qsub(a), isub(a), dsub(a),
+        no_alloc(true) {}
 };
 
 // The database as the search kernels see it.
@@ -200,6 +233,21 @@ void search_batch(const DbView& db, const float* q, int nq, const SearchParams& 
 
 // Device bytes search_batch allocates for one batch (depth-0 workspace).
 size_t search_workspace_bytes(const DbView& db, int nq, const SearchParams& p);
+
+// searchL2Distance on device buffers (plain FP32 path, results on `st`,
+// synchronised). wss: workspaces per retry depth (4); `arena`, if set, backs
+// the temporary norms / padded rows and every workspace buffer (no
+// allocation anywhere); otherwise they come from the kept buffers passed in.
+struct L2Scratch {
+  DevBuf xpad, xn, xmax;
+  std::vector<Workspace> wss;
+};
+void search_l2_device(const float* x, int64_t n, int d, const float* xnorms, const float* q, int64_t nq,
+                      int k, int32_t* indices, float* distances, int exact, hipStream_t st,
+                      L2Scratch& scratch);
+// Device bytes search_l2_device needs for depth 0 plus a retry of
+// min(nq, 64) queries (what the no-allocation entry point checks up front).
+size_t search_l2_device_bytes(int64_t n, int d, bool have_norms, bool aligned, int64_t nq, int k);
 
 // The view and the parameters IndexBF::search uses for k on an index; the
 // candidate copy must already match the configured precision.

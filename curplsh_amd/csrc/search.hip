@@ -32,6 +32,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -56,21 +58,32 @@ struct Level {
   int rank_next;  // rho of the threshold for the next level; 0 on the last
 };
 
+// Rows the last level aims to keep per query: c_factor * k', but at least
+// kMinTarget, so that a small k' (k = 1: k' = 4) still leaves the last sample
+// level a stride of several rows and a threshold rank of at least 8; with
+// 3 * 4 = 12 rows the last sample collapsed to rank 1 and about a quarter of
+// the queries kept fewer than k' rows and were re-run.
+constexpr int64_t kMinTarget = 64;
+static int64_t survivor_target(int kp, int c_factor) {
+  return std::max<int64_t>((int64_t)c_factor * kp, kMinTarget);
+}
+
 // Rank of the last level's threshold within its sample. Automatic
 // (rho_final <= 0): larger ranks give a tighter survivor count for the same
 // risk but need a denser last sample (stride c*k'/rho); keep that stride >= 16
 // with rho in [8, 32].
 static int final_rank(int kp, int c_factor, int rho_final) {
   return rho_final > 0 ? rho_final
-                       : (int)std::min<int64_t>(32, std::max<int64_t>(8, (int64_t)c_factor * kp / 16));
+                       : (int)std::min<int64_t>(32, std::max<int64_t>(8, survivor_target(kp, c_factor) / 16));
 }
 
 // Probability that a query keeps fewer than k' rows at the full level: the
-// threshold is the rho-th smallest key of a sample of stride s = c*k'/rho, so
+// threshold is the rho-th smallest key of a sample of stride s = c*k'/rho
+// (c = the survivor target over k'), so
 // the full level keeps about s * Gamma(rho, 1) rows, fewer than k' when
 // Gamma(rho, 1) < rho / c, i.e. P(Poisson(rho / c) >= rho).
-static double short_probability(int rho, int c_factor) {
-  if (c_factor <= 1) return 1.0;
+static double short_probability(int rho, double c_factor) {
+  if (c_factor <= 1.0) return 1.0;
   const double lam = (double)rho / c_factor;
   double term = std::exp(-lam), below = 0.0;
   for (int i = 0; i < rho; ++i) {
@@ -92,7 +105,7 @@ static std::vector<Level> plan_levels(int64_t n, int kp, int c_factor, int rho_f
     L.push_back({1, n, 0});
     return L;
   }
-  const int64_t target = (int64_t)c_factor * kp;
+  const int64_t target = survivor_target(kp, c_factor);
   const int64_t s_pf = std::max<int64_t>(1, target / kRhoFinal);
   std::vector<int64_t> strides;
   if (s0 <= s_pf) {
@@ -124,7 +137,7 @@ static std::vector<Level> plan_levels(int64_t n, int kp, int c_factor, int rho_f
 static int64_t level_cap(const std::vector<Level>& L, int kp, int c_factor) {
   int64_t cap = 4096;
   cap = std::max<int64_t>(cap, L.front().stride > 1 ? L.front().m : 0);
-  cap = std::max<int64_t>(cap, 4 * (int64_t)c_factor * kp);
+  cap = std::max<int64_t>(cap, 4 * survivor_target(kp, c_factor));
   if (L.size() == 1) cap = std::max<int64_t>(cap, L.front().m);
   return cap;
 }
@@ -352,8 +365,10 @@ void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& 
   Workspace& ws = wss.at(depth);
   const int d = db.d;
   const int kp = (int)std::min<int64_t>(p.kp, db.n);
-  // One read-back of the batch's counters into the workspace's pinned buffer.
-  BatchCounters* hc = ws.hctr.as<BatchCounters>(1);
+  // One read-back of the batch's counters into the workspace's pinned buffer
+  // (or a stack copy when the workspace must not allocate).
+  BatchCounters stack_ctr{};
+  BatchCounters* hc = ws.no_alloc ? &stack_ctr : ws.hctr.as<BatchCounters>(1);
   RPLSH_HIP(hipMemcpyAsync(hc, ws.ctr.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
   RPLSH_HIP(hipStreamSynchronize(st));
   const int nfail = hc->failed, nfp64 = hc->fp64;
@@ -401,7 +416,7 @@ void search_batch(const DbView& db, const float* q, int nq, const SearchParams& 
                   hipStream_t st, rplsh_search_stats& stats) {
   if (nq <= 0) return;
   PhaseEvents pe;
-  if (depth == 0) pe.create();
+  if (depth == 0 && !wss.at(0).no_alloc) pe.create();
   enqueue_batch(db, q, nq, nullptr, p, out_i, out_d, wss.at(depth), st, depth == 0 ? &pe : nullptr);
   finish_batch(db, q, nq, p, out_i, out_d, wss, depth, st, stats, depth == 0 ? &pe : nullptr);
 }
@@ -874,71 +889,123 @@ rplsh_status rplsh_bf_last_stats(const rplsh_bf* b, rplsh_search_stats* out) {
   });
 }
 
-// searchL2Distance on caller device buffers, plain FP32 path.
+// searchL2Distance on caller device buffers, plain FP32 path. Temporaries
+// (norms when not given, rows padded to 16 bytes when x is not, and the
+// search workspaces) are kept per device between calls and only grow; a
+// call holds its device's set for its whole duration.
+namespace {
+struct KeptL2 {
+  std::mutex mu;
+  L2Scratch s;
+};
+KeptL2& kept_l2(int device) {
+  static std::mutex mu;
+  static std::vector<std::unique_ptr<KeptL2>> per_device;
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)per_device.size() <= device) per_device.resize(device + 1);
+  if (!per_device[device]) {
+    per_device[device] = std::make_unique<KeptL2>();
+    per_device[device]->s.wss.resize(4);
+  }
+  return *per_device[device];
+}
+}  // namespace
+
 rplsh_status rplsh_search_l2_dev(const float* x, int64_t n, int d, const float* xnorms,
                                  const float* q, int64_t nq, int k, int32_t* indices,
                                  float* distances, int exact, void* stream) {
   return guard([&] {
-    RPLSH_REQUIRE(d > 0 && filter_dpad(Elem::F32, d) > 0, "dimension %d unsupported (1..%d)", d, kMaxKLoopDim);
-    RPLSH_REQUIRE(n >= 0 && nq >= 0, "bad sizes");
-    RPLSH_REQUIRE(k > 0, "k must be > 0");
-    if (nq == 0) return;
-    RPLSH_REQUIRE(q && indices && distances && (n == 0 || x), "NULL pointer");
-    hipStream_t st = (hipStream_t)stream;
-    if (n == 0) {
-      launch_fill_i32(indices, nq * k, -1, st);
-      launch_fill_f32(distances, nq * k, FLT_MAX, st);
-      RPLSH_HIP(hipStreamSynchronize(st));
-      return;
-    }
-    RPLSH_REQUIRE(plain_kprime(k) <= 4096 || n <= 4096, "k too large");
-    DevBuf xpad, xn, xmax;
-    DbView v;
-    v.n = n;
-    v.d = d;
-    v.elem = Elem::F32;
-    v.dpad = filter_dpad(Elem::F32, d);
-    const bool aligned = (d % 4 == 0) && ((uintptr_t)x % 16 == 0);
-    if (aligned) {
-      v.x = x;
-      v.ldx = d;
-    } else {
-      const int ld = (int)round_up(d, 4);
-      float* t = xpad.as<float>((size_t)n * ld);
-      launch_pad_rows(x, n, d, d, ld, t, st);
-      v.x = t;
-      v.ldx = ld;
-    }
-    if (xnorms) {
-      v.xnorm = xnorms;
-    } else {
-      float* t = xn.as<float>(n);
-      launch_l2norm(v.x, n, d, v.ldx, t, 1, st);
-      v.xnorm = t;
-    }
-    float* mx = xmax.as<float>(1);
-    RPLSH_HIP(hipMemsetAsync(mx, 0, sizeof(float), st));
-    launch_max_reduce(v.xnorm, n, mx, st);
-    RPLSH_HIP(hipMemcpyAsync(&v.xn_max, mx, sizeof(float), hipMemcpyDeviceToHost, st));
-    RPLSH_HIP(hipStreamSynchronize(st));
-    v.xenc = v.x;
-    v.ldenc = v.ldx;
-    v.dvalid_bytes = (int)(v.ldx * 4);
-    SearchParams p;
-    p.k = k;
-    p.kp = plain_kprime(k);
-    p.exact = exact;
-    p.certify = true;
-    std::vector<Workspace> wss(4);
-    rplsh_search_stats stats{};
-    search_batch(v, q, (int)nq, p, indices, distances, wss, 0, st, stats);
-    RPLSH_HIP(hipStreamSynchronize(st));
+    KeptL2& kept = kept_l2(current_device());
+    std::lock_guard<std::mutex> lk(kept.mu);
+    search_l2_device(x, n, d, xnorms, q, nq, k, indices, distances, exact, (hipStream_t)stream, kept.s);
   });
 }
 
 }  // extern "C"
 
-// ---------------------------------------------------------------- autotuner
+namespace rplsh {
+
+void search_l2_device(const float* x, int64_t n, int d, const float* xnorms, const float* q, int64_t nq,
+                      int k, int32_t* indices, float* distances, int exact, hipStream_t st,
+                      L2Scratch& sc) {
+  RPLSH_REQUIRE(d > 0 && filter_dpad(Elem::F32, d) > 0, "dimension %d unsupported (1..%d)", d, kMaxKLoopDim);
+  RPLSH_REQUIRE(n >= 0 && nq >= 0, "bad sizes");
+  RPLSH_REQUIRE(k > 0, "k must be > 0");
+  RPLSH_REQUIRE(nq < (1ll << 31) && n < (1ll << 31), "sizes limited to 2^31 rows");
+  if (nq == 0) return;
+  RPLSH_REQUIRE(q && indices && distances && (n == 0 || x), "NULL pointer");
+  if (n == 0) {
+    launch_fill_i32(indices, nq * k, -1, st);
+    launch_fill_f32(distances, nq * k, FLT_MAX, st);
+    RPLSH_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  RPLSH_REQUIRE(plain_kprime(k) <= 4096 || n <= 4096, "k too large");
+  DbView v;
+  v.n = n;
+  v.d = d;
+  v.elem = Elem::F32;
+  v.dpad = filter_dpad(Elem::F32, d);
+  const bool aligned = (d % 4 == 0) && ((uintptr_t)x % 16 == 0);
+  if (aligned) {
+    v.x = x;
+    v.ldx = d;
+  } else {
+    const int ld = (int)round_up(d, 4);
+    float* t = sc.xpad.as<float>((size_t)n * ld);
+    launch_pad_rows(x, n, d, d, ld, t, st);
+    v.x = t;
+    v.ldx = ld;
+  }
+  if (xnorms) {
+    v.xnorm = xnorms;
+  } else {
+    float* t = sc.xn.as<float>(n);
+    launch_l2norm(v.x, n, d, v.ldx, t, 1, st);
+    v.xnorm = t;
+  }
+  float* mx = sc.xmax.as<float>(1);
+  launch_fill_f32(mx, 1, 0.0f, st);
+  launch_max_reduce(v.xnorm, n, mx, st);
+  RPLSH_HIP(hipMemcpyAsync(&v.xn_max, mx, sizeof(float), hipMemcpyDeviceToHost, st));
+  RPLSH_HIP(hipStreamSynchronize(st));
+  v.xenc = v.x;
+  v.ldenc = v.ldx;
+  v.dvalid_bytes = (int)(v.ldx * 4);
+  SearchParams p;
+  p.k = k;
+  p.kp = plain_kprime(k);
+  p.exact = exact;
+  p.certify = true;
+  rplsh_search_stats stats{};
+  search_batch(v, q, (int)nq, p, indices, distances, sc.wss, 0, st, stats);
+  RPLSH_HIP(hipStreamSynchronize(st));
+}
+
+size_t search_l2_device_bytes(int64_t n, int d, bool have_norms, bool aligned, int64_t nq, int k) {
+  if (n <= 0 || nq <= 0) return 0;
+  DbView v;
+  v.n = n;
+  v.d = d;
+  v.elem = Elem::F32;
+  v.dpad = filter_dpad(Elem::F32, d);
+  SearchParams p;
+  p.k = k;
+  p.kp = plain_kprime(k);
+  SearchParams p2 = p;
+  p2.c_factor = p.c_factor * 4;
+  p2.kp = (int)std::min<int64_t>(std::min<int64_t>(n, 4096), (int64_t)p.kp * 4);
+  const int64_t r = std::min<int64_t>(nq, 64);
+  constexpr size_t kSlop = 32 * 256;  // alignment of the pieces
+  size_t b = kSlop + 256;
+  if (!aligned) b += (size_t)n * round_up(d, 4) * 4;
+  if (!have_norms) b += (size_t)n * 4;
+  b += search_workspace_bytes(v, (int)nq, p) + kSlop;
+  b += search_workspace_bytes(v, (int)r, p2) + (size_t)r * ((size_t)d * 4 + (size_t)k * 8) + kSlop;
+  return b;
+}
+
+}  // namespace rplsh
 
 namespace {
 
@@ -1126,7 +1193,7 @@ rplsh_status rplsh_bf_autotune(rplsh_bf* b, const float* sample, int64_t nsample
           // already paid for one.
           if (stats.retried_queries == 0 && plan_levels(v.n, p.kp, p.c_factor, p.rho).size() > 1) {
             const int rho = final_rank(p.kp, p.c_factor, p.rho);
-            const double pf = short_probability(rho, p.c_factor);
+            const double pf = short_probability(rho, (double)survivor_target(p.kp, p.c_factor) / p.kp);
             const double p_any = 1.0 - std::pow(1.0 - pf, (double)batch_nq);
             if (p_any > 1e-3) ms += (float)(p_any * retry_ms(p));
           }
