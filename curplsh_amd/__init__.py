@@ -544,6 +544,23 @@ def search_l2_distance(bases: DeviceArray, bases_norm: DeviceArray | None, queri
          indices.ptr, distances.ptr, int(exact), stream)
 
 
+def search_l2_distance_noalloc(bases: DeviceArray, bases_norm: DeviceArray | None, queries: DeviceArray,
+                               k: int, indices: DeviceArray, distances: DeviceArray,
+                               exact: bool = True, stream: int = 0) -> None:
+    """searchL2Distance through rplsh_search_l2_dev_noalloc: same results,
+    every temporary from the library's fixed per-device scratch array (no
+    device allocation); sizes that do not fit raise INVALID_ARG."""
+    n, d = bases.shape if bases is not None else (0, queries.shape[1])
+    nq = queries.shape[0]
+    call("rplsh_search_l2_dev_noalloc", bases.ptr if bases is not None else None, n, d,
+         bases_norm.ptr if bases_norm is not None else None, queries.ptr, nq, int(k),
+         indices.ptr, distances.ptr, int(exact), stream)
+
+
+def noalloc_scratch_bytes() -> int:
+    return int(_lib.load().rplsh_search_l2_noalloc_scratch_bytes())
+
+
 def l2_select(products: DeviceArray, base_norms: DeviceArray, k: int, distances: DeviceArray,
               indices: DeviceArray, stream: int = 0) -> None:
     """l2Select (include/internal/select.h:29-34): per row the k smallest

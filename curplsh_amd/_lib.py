@@ -200,6 +200,8 @@ _SIGS = {
     "rplsh_encode_bf16_dev": [_vp, _i64, _i32, _i32, _vp, _vp],
     "rplsh_encode_e4m3_dev": [_vp, _i64, _i32, _i32, _vp, _vp, _vp],
     "rplsh_search_l2_dev": [_vp, _i64, _i32, _vp, _vp, _i64, _i32, _vp, _vp, _i32, _vp],
+    "rplsh_search_l2_dev_noalloc": [_vp, _i64, _i32, _vp, _vp, _i64, _i32, _vp, _vp, _i32, _vp],
+    "rplsh_search_l2_noalloc_scratch_bytes": [],
     "rplsh_l2_select_dev": [_vp, _i64, _i64, _i64, _vp, _i32, _vp, _vp, _vp],
     "rplsh_block_select_dev": [_vp, _vp, _i64, _i64, _i32, _i32, _vp, _vp, _vp],
     "rplsh_broadcast_rows_sum_dev": [_vp, _vp, _i64, _i64, _vp],
@@ -261,6 +263,7 @@ _RESTYPES = {
     "rplsh_bf_config_default": None,
     "rplsh_server_config_default": None,
     "rplsh_lsh_config_default": None,
+    "rplsh_search_l2_noalloc_scratch_bytes": C.c_size_t,
 }
 
 _lib = None

@@ -132,6 +132,23 @@ rplsh_status rplsh_search_l2_dev(const float* x, int64_t n, int d, const float* 
                                  const float* q, int64_t nq, int k, int32_t* indices,
                                  float* distances, int exact, void* stream);
 
+/* No reference counterpart: rplsh_search_l2_dev for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, semantics and results;
+ * every temporary comes from a fixed device scratch array of
+ * rplsh_search_l2_noalloc_scratch_bytes() (64 MiB) per device that the
+ * library holds until the process ends. No hipMalloc/hipFree, pinned-host
+ * allocation or HIP event creation. Sizes whose temporaries do not fit
+ * (depth-0 workspace plus a retry of up to 64 queries; norms when xnorms is
+ * NULL; padded rows when x is not 16-byte aligned with d % 4 == 0) are
+ * rejected with RPLSH_ERR_INVALID_ARG before any work; a deeper retry that
+ * runs out of scratch fails with RPLSH_ERR_OUT_OF_MEMORY. Calls that overlap
+ * on one device race on the shared scratch: serialise them. */
+rplsh_status rplsh_search_l2_dev_noalloc(const float* x, int64_t n, int d, const float* xnorms,
+                                         const float* q, int64_t nq, int k, int32_t* indices,
+                                         float* distances, int exact, void* stream);
+size_t rplsh_search_l2_noalloc_scratch_bytes(void);
+
 /* l2Select (include/internal/select.h:29-34, src/internal/select.cu:189-224):
  * per row r of products [rows][n] (row stride ldp), the k smallest
  * v = base_norms[i] + products[r][i] (one FP32 add), ascending, ties by the
