@@ -443,6 +443,44 @@ class IndexBF:
             pass
 
 
+class HbmBudget:
+    """rplsh_memory: a byte budget on one device with reusable blocks, the
+    implementation behind curplsh::DeviceMemory (include/device_memory.h:12-89).
+    reserve() returns a device pointer charged to the budget until release();
+    a Server admits its batches against its own budget or one given here."""
+
+    def __init__(self, device: int = 0, budget_bytes: int = 0, _handle: int | None = None):
+        load()
+        self._own = _handle is None
+        self._h = C.c_void_p(_handle) if _handle is not None else C.c_void_p()
+        if self._own:
+            call("rplsh_memory_create", int(device), int(budget_bytes), C.byref(self._h))
+
+    def reserve(self, nbytes: int, stream: int = 0, timeout_s: float = -1.0) -> int:
+        p = C.c_void_p()
+        call("rplsh_memory_reserve", self._h, stream, int(nbytes), float(timeout_s), C.byref(p))
+        return p.value or 0
+
+    def release(self, ptr: int) -> None:
+        call("rplsh_memory_release", self._h, C.c_void_p(ptr))
+
+    def stats(self) -> dict:
+        st = _lib.MemoryStats()
+        call("rplsh_memory_get_stats", self._h, C.byref(st))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def close(self) -> None:
+        if self._own and getattr(self, "_h", None) and self._h.value:
+            call("rplsh_memory_destroy", self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Server:
     """Continuous-batching request loop over resident indexes (rplsh_server_*).
 
@@ -453,13 +491,15 @@ class Server:
 
     def __init__(self, device: int = 0, slots: int = 4, hbm_budget_bytes: int = 0,
                  max_batch_queries: int = 4096, use_graphs: bool = True,
-                 graph_max_queries: int = 1024):
+                 graph_max_queries: int = 1024, memory: "HbmBudget | None" = None):
         load()
         cfg = _lib.ServerConfig()
         _lib.load().rplsh_server_config_default(C.byref(cfg))
         cfg.device, cfg.slots = device, slots
         cfg.hbm_budget_bytes, cfg.max_batch_queries = hbm_budget_bytes, max_batch_queries
         cfg.use_graphs, cfg.graph_max_queries = int(use_graphs), graph_max_queries
+        cfg.memory = memory._h.value if memory is not None else None
+        self._memory = memory
         self._h = C.c_void_p()
         call("rplsh_server_create", C.byref(cfg), C.byref(self._h))
         self._pending = {}
@@ -507,6 +547,12 @@ class Server:
 
     def reset_stats(self) -> None:
         call("rplsh_server_reset_stats", self._h)
+
+    def memory(self) -> HbmBudget:
+        """The budget the server admits against (not owned by the result)."""
+        m = C.c_void_p()
+        call("rplsh_server_memory", self._h, C.byref(m))
+        return HbmBudget(_handle=m.value)
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:

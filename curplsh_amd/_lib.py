@@ -96,6 +96,22 @@ class ServerConfig(C.Structure):
         ("use_graphs", C.c_int),
         ("graph_max_queries", C.c_int),
         ("graph_cache_bytes", C.c_size_t),
+        ("memory", C.c_void_p),
+    ]
+
+
+class MemoryStats(C.Structure):
+    _fields_ = [
+        ("device", C.c_int),
+        ("budget_bytes", C.c_size_t),
+        ("reserved_bytes", C.c_size_t),
+        ("peak_reserved_bytes", C.c_size_t),
+        ("cached_bytes", C.c_size_t),
+        ("available_without_malloc", C.c_size_t),
+        ("high_water_malloc_bytes", C.c_size_t),
+        ("reservations", C.c_int64),
+        ("waits", C.c_int64),
+        ("mallocs", C.c_int64),
     ]
 
 
@@ -221,6 +237,12 @@ _SIGS = {
     "rplsh_bf_autotune": [_vp, _vp, _i64, _i32, _i64, _f32, C.POINTER(TuneResult)],
     "rplsh_bf_set_tuning": [_vp, C.POINTER(TuneResult)],
     "rplsh_bf_clear_tuning": [_vp],
+    "rplsh_memory_create": [_i32, C.c_size_t, C.POINTER(_vp)],
+    "rplsh_memory_destroy": [_vp],
+    "rplsh_memory_reserve": [_vp, _vp, C.c_size_t, C.c_double, C.POINTER(_vp)],
+    "rplsh_memory_release": [_vp, _vp],
+    "rplsh_memory_get_stats": [_vp, C.POINTER(MemoryStats)],
+    "rplsh_server_memory": [_vp, C.POINTER(_vp)],
     "rplsh_server_config_default": [C.POINTER(ServerConfig)],
     "rplsh_server_create": [C.POINTER(ServerConfig), C.POINTER(_vp)],
     "rplsh_server_destroy": [_vp],

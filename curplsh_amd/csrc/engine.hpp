@@ -304,4 +304,18 @@ namespace rplsh {
 // Fails with RPLSH_ERR_STATE while a server holds the index; otherwise bumps
 // its generation (the caller is about to change it).
 void bf_begin_change(rplsh_bf* b);
+
+// HBM budget (memory.hip): accounting-only charges for buffers their users
+// own (the serving loop's batch workspaces and cached graphs), beside the
+// block reservations of rplsh_memory_reserve.
+rplsh_memory* memory_create(int device, size_t budget);
+void memory_destroy(rplsh_memory* m);
+bool memory_try_charge(rplsh_memory* m, size_t bytes);  // false if it would pass the budget
+void memory_force_charge(rplsh_memory* m, size_t bytes);
+void memory_uncharge(rplsh_memory* m, size_t bytes);
+size_t memory_budget(const rplsh_memory* m);
+int memory_device(const rplsh_memory* m);
+size_t memory_reserved(rplsh_memory* m);
+size_t memory_peak(rplsh_memory* m);
+void memory_reset_peak(rplsh_memory* m);
 }  // namespace rplsh

@@ -138,7 +138,11 @@ struct rplsh_server {
   std::deque<Request*> queue;
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
   uint64_t next_id = 1;
-  size_t reserved = 0;     // workspaces of admitted batches (eager or capturing)
+  // The HBM budget batches are admitted against (own, or given in the
+  // config): admitted batches charge their workspaces to it while they run,
+  // cached graphs theirs while cached.
+  rplsh_memory* mem = nullptr;
+  bool own_mem = false;
   size_t graph_bytes = 0;  // workspaces held by cached graphs (all slots)
   int inflight = 0;
   bool stop = false;
@@ -146,7 +150,6 @@ struct rplsh_server {
   // statistics
   std::vector<double> lat;
   int64_t submitted = 0, completed = 0, rejected = 0, queries = 0, batches = 0;
-  size_t peak_reserved = 0;
   int peak_inflight = 0;
   bool have_first = false;
   Clock::time_point first_submit, last_done;
@@ -252,6 +255,7 @@ void evict_graphs(rplsh_server* s, Slot& sl, size_t want) {
     if (k.gen != s->indexes[k.index]->generation.load()) {
       sl.graph_bytes -= (*it)->bytes;
       s->graph_bytes -= (*it)->bytes;
+      memory_uncharge(s->mem, (*it)->bytes);
       it = sl.graphs.erase(it);
     } else {
       ++it;
@@ -262,6 +266,7 @@ void evict_graphs(rplsh_server* s, Slot& sl, size_t want) {
                                [](const auto& a, const auto& b) { return a->last_use < b->last_use; });
     sl.graph_bytes -= (*it)->bytes;
     s->graph_bytes -= (*it)->bytes;
+    memory_uncharge(s->mem, (*it)->bytes);
     sl.graphs.erase(it);
   }
 }
@@ -312,7 +317,9 @@ int run_batch(rplsh_server* s, Slot& sl, Plan& pl, const std::vector<Request*>& 
       evict_graphs(s, sl, g->bytes);
       sl.graph_bytes += g->bytes;
       s->graph_bytes += g->bytes;
-      s->reserved -= pl.graph_part;  // now counted in graph_bytes
+      // The estimate charged at admission becomes the graph's real size.
+      memory_force_charge(s->mem, g->bytes);
+      memory_uncharge(s->mem, pl.graph_part);
       pl.need -= pl.graph_part;
       pl.graph_part = 0;
       s->graph_captures += 1;
@@ -408,20 +415,24 @@ void worker(rplsh_server* s, Slot* sl) {
         hb = SIZE_MAX;
         why = e.what();
       }
-      if (hb == SIZE_MAX || hb > s->cfg.hbm_budget_bytes) {
+      const size_t budget = memory_budget(s->mem);
+      if (hb == SIZE_MAX || hb > budget) {
         s->queue.pop_front();
         reject(s, head, hb == SIZE_MAX ? RPLSH_ERR_STATE : RPLSH_ERR_BUDGET,
                hb == SIZE_MAX ? why : "batch workspace exceeds the HBM budget");
         continue;
       }
-      if (s->reserved + s->graph_bytes + hb > s->cfg.hbm_budget_bytes) {
-        if (s->inflight == 0) {
-          // Only cached graphs stand in the way (nothing runs, so none is in
-          // use): release them all.
+      size_t avail = budget - std::min(budget, memory_reserved(s->mem));
+      if (hb > avail) {
+        if (s->inflight == 0 && s->graph_bytes > 0) {
+          // Cached graphs stand in the way and nothing runs, so none is in
+          // use: release them all.
           for (auto& o : s->slots) evict_graphs(s, *o, SIZE_MAX / 2);
           continue;
         }
-        s->cv.wait(lk);  // a finishing batch will release memory
+        // A finishing batch, or a reservation released elsewhere on a shared
+        // budget, makes room: look again shortly.
+        s->cv.wait_for(lk, std::chrono::milliseconds(2));
         continue;
       }
       // Admit the head and the same-(index, k) requests queued behind it.
@@ -438,7 +449,7 @@ void worker(rplsh_server* s, Slot* sl) {
             nb = batch_bytes(b, total + r->nq, head->k);
           } catch (...) {
           }
-          if (nb != SIZE_MAX && s->reserved + s->graph_bytes + nb <= s->cfg.hbm_budget_bytes) {
+          if (nb != SIZE_MAX && nb <= avail) {
             batch.push_back(r);
             total += r->nq;
             need = nb;
@@ -466,8 +477,7 @@ void worker(rplsh_server* s, Slot* sl) {
             // Capture if the graph's workspace fits beside what runs now.
             const size_t gw = graph_bytes_for(v, pl.key, pl.p);
             const size_t rb = retry_bytes(v, total, pl.p);
-            if (s->reserved + s->graph_bytes + gw + rb <= s->cfg.hbm_budget_bytes &&
-                gw <= s->cfg.graph_cache_bytes / (size_t)std::max(1, s->cfg.slots)) {
+            if (gw + rb <= avail && gw <= s->cfg.graph_cache_bytes / (size_t)std::max(1, s->cfg.slots)) {
               pl.capture = true;
               pl.need = gw + rb;
               pl.graph_part = gw;
@@ -477,6 +487,16 @@ void worker(rplsh_server* s, Slot* sl) {
       } catch (...) {
         pl.g = nullptr;
         pl.capture = false;
+        pl.need = need;
+        pl.graph_part = 0;
+      }
+      if (!memory_try_charge(s->mem, pl.need)) {
+        // Taken meanwhile by a reservation on a shared budget: requeue.
+        for (auto rit = batch.rbegin(); rit != batch.rend(); ++rit) s->queue.push_front(*rit);
+        batch.clear();
+        pl = Plan();
+        s->cv.wait_for(lk, std::chrono::milliseconds(2));
+        continue;
       }
       break;
     }
@@ -486,10 +506,8 @@ void worker(rplsh_server* s, Slot* sl) {
       r->info.batch_requests = (int)batch.size();
       r->info.batch_queries = pl.total;
     }
-    s->reserved += pl.need;
     s->inflight += 1;
     s->batches += 1;
-    s->peak_reserved = std::max(s->peak_reserved, s->reserved + s->graph_bytes);
     s->peak_inflight = std::max(s->peak_inflight, s->inflight);
     const bool replay = pl.g != nullptr || pl.capture;
     if (replay) s->graph_launches += 1;
@@ -498,7 +516,7 @@ void worker(rplsh_server* s, Slot* sl) {
     float device_ms = 0.0f;
     const int st = run_batch(s, *sl, pl, batch, err, device_ms);
     lk.lock();
-    s->reserved -= pl.need;
+    memory_uncharge(s->mem, pl.need);
     s->inflight -= 1;
     for (auto* r : batch) {
       r->err = err;
@@ -567,13 +585,23 @@ rplsh_status rplsh_server_create(const rplsh_server_config* cfg, rplsh_server** 
     rplsh_device_count(&count);
     RPLSH_REQUIRE(c.device >= 0 && c.device < count, "device %d out of range [0, %d)", c.device, count);
     DeviceScope scope(c.device);
-    if (c.hbm_budget_bytes == 0) {
+    if (c.memory) {
+      RPLSH_REQUIRE(memory_device(c.memory) == c.device, "budget lives on device %d, server on %d",
+                    memory_device(c.memory), c.device);
+      c.hbm_budget_bytes = memory_budget(c.memory);
+    } else if (c.hbm_budget_bytes == 0) {
       size_t fr = 0, tot = 0;
       RPLSH_HIP(hipMemGetInfo(&fr, &tot));
       c.hbm_budget_bytes = fr / 4;
     }
     auto* s = new rplsh_server();
     s->cfg = c;
+    if (c.memory) {
+      s->mem = c.memory;
+    } else {
+      s->mem = memory_create(c.device, c.hbm_budget_bytes);
+      s->own_mem = true;
+    }
     try {
       for (int i = 0; i < c.slots; ++i) {
         auto sl = std::make_unique<Slot>();
@@ -586,6 +614,7 @@ rplsh_status rplsh_server_create(const rplsh_server_config* cfg, rplsh_server** 
       for (auto& sl : s->slots) sl->th = std::thread(worker, s, sl.get());
     } catch (...) {
       destroy_slots(s);  // stops and joins the workers already started
+      if (s->own_mem) memory_destroy(s->mem);
       delete s;
       throw;
     }
@@ -597,6 +626,7 @@ rplsh_status rplsh_server_destroy(rplsh_server* s) {
   return guard([&] {
     if (!s) return;
     destroy_slots(s);
+    if (s->own_mem) memory_destroy(s->mem);
     delete s;
   });
 }
@@ -697,7 +727,7 @@ rplsh_status rplsh_server_get_stats(rplsh_server* s, rplsh_server_stats* out) {
     out->rejected = s->rejected;
     out->queries = s->queries;
     out->batches = s->batches;
-    out->peak_reserved_bytes = s->peak_reserved;
+    out->peak_reserved_bytes = memory_peak(s->mem);
     out->peak_inflight = s->peak_inflight;
     out->graph_launches = s->graph_launches;
     out->graph_captures = s->graph_captures;
@@ -729,11 +759,17 @@ rplsh_status rplsh_server_reset_stats(rplsh_server* s) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->lat.clear();
     s->submitted = s->completed = s->rejected = s->queries = s->batches = 0;
-    s->peak_reserved = 0;
     s->peak_inflight = 0;
     s->have_first = false;
     s->graph_launches = s->graph_captures = 0;
-    s->peak_reserved = s->reserved + s->graph_bytes;
+    memory_reset_peak(s->mem);
+  });
+}
+
+rplsh_status rplsh_server_memory(rplsh_server* s, rplsh_memory** out) {
+  return guard([&] {
+    RPLSH_REQUIRE(s && out, "NULL argument");
+    *out = s->mem;
   });
 }
 

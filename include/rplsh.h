@@ -378,6 +378,42 @@ rplsh_status rplsh_lsh_tables(const rplsh_lsh* index, uint32_t* codes, uint32_t*
 rplsh_status rplsh_lsh_hash_queries(rplsh_lsh* index, int64_t nq, const float* queries,
                                     uint32_t* codes, float* margins);
 
+/* --------------------------------------------------------- HBM budget -- */
+/* DeviceMemory / DeviceMemoryReservation (include/device_memory.h:12-89):
+ * a byte budget on one device with reusable blocks. rplsh_memory_reserve
+ * returns a block of at least `bytes` (a cached free block of at most twice
+ * the size when there is one, else a new hipMalloc), charged to the budget
+ * until released; it waits up to timeout_s (< 0: without limit, 0: not at
+ * all) while other reservations hold the budget and fails with
+ * RPLSH_ERR_BUDGET when the bytes never or not yet fit. Cached blocks count
+ * against the budget and are freed when a new allocation needs the room.
+ * rplsh_memory_release waits for the stream the block was reserved on, then
+ * keeps the block for reuse. A serving loop admits its batches against such a
+ * budget (its own, or one passed in rplsh_server_config.memory), so
+ * reservations taken from it and the loop's batches share one bound.
+ * Thread-safe. */
+typedef struct rplsh_memory rplsh_memory;
+
+typedef struct rplsh_memory_stats {
+  int device;
+  size_t budget_bytes;
+  size_t reserved_bytes;            /* charged now (blocks and accounting charges) */
+  size_t peak_reserved_bytes;
+  size_t cached_bytes;              /* free blocks kept for reuse */
+  size_t available_without_malloc;  /* largest reservation served from the cache now */
+  size_t high_water_malloc_bytes;   /* peak bytes obtained from hipMalloc */
+  int64_t reservations, waits, mallocs;
+} rplsh_memory_stats;
+
+/* budget_bytes 0: a quarter of the device's free memory. */
+rplsh_status rplsh_memory_create(int device, size_t budget_bytes, rplsh_memory** out);
+/* Fails with RPLSH_ERR_STATE while reservations are held. */
+rplsh_status rplsh_memory_destroy(rplsh_memory* m);
+rplsh_status rplsh_memory_reserve(rplsh_memory* m, void* stream, size_t bytes, double timeout_s,
+                                  void** ptr);
+rplsh_status rplsh_memory_release(rplsh_memory* m, void* ptr);
+rplsh_status rplsh_memory_get_stats(rplsh_memory* m, rplsh_memory_stats* out);
+
 /* ----------------------------------------------------------- serving -- */
 /* Continuous-batching request loop over resident indexes (no reference
  * counterpart). `slots` worker threads each own a HIP stream and a workspace;
@@ -412,6 +448,8 @@ typedef struct rplsh_server_config {
   int graph_max_queries;   /* largest batch bucket captured (default 1024) */
   size_t graph_cache_bytes;/* device bytes of cached graph workspaces per server
                               (default 8 GiB); least recently used are evicted */
+  rplsh_memory* memory;    /* budget to admit against (not owned, must outlive the
+                              server); NULL: the server's own, of hbm_budget_bytes */
 } rplsh_server_config;
 
 typedef struct rplsh_request_info {
@@ -452,6 +490,8 @@ rplsh_status rplsh_server_wait(rplsh_server* s, uint64_t request, double timeout
 rplsh_status rplsh_server_drain(rplsh_server* s);
 rplsh_status rplsh_server_get_stats(rplsh_server* s, rplsh_server_stats* out);
 rplsh_status rplsh_server_reset_stats(rplsh_server* s);
+/* The budget the server admits against (its own or the one it was given). */
+rplsh_status rplsh_server_memory(rplsh_server* s, rplsh_memory** out);
 /* Device bytes one batch of nq queries with this k needs on an index. */
 rplsh_status rplsh_bf_batch_bytes(rplsh_bf* index, int64_t nq, int k, size_t* bytes);
 

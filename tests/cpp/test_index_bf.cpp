@@ -19,6 +19,9 @@
 //  7. The internal selection functions: blockSelect (smallest / largest,
 //     implicit and given labels), l2Select + broadcastAlongRowsSum on a small
 //     fixture with exact ties (lower label first).
+//  8. DeviceMemory (include/device_memory.h:12-89) over the HBM budget:
+//     reservations used as tensor storage through narrow()ed views, blocks
+//     reused after release, the budget respected.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -28,6 +31,7 @@
 #include <unistd.h>
 
 #include "curplsh/dataset.h"
+#include "curplsh/device_memory.h"
 #include "curplsh/index_bf.h"
 #include "curplsh/index_rplsh.h"
 #include "curplsh/internal/broadcast.h"
@@ -274,6 +278,46 @@ static int test_rplsh(DeviceResources& res) {
   return 0;
 }
 
+static int test_device_memory(DeviceResources& res) {
+  const int n = 3000, dim = 64;
+  Stream s = res.getDefaultStream(0);
+  BudgetDeviceMemory mem(0, (size_t)64 << 20);
+  CHECK(mem.getDevice() == 0);
+  std::vector<float> h((size_t)n * dim), out(n);
+  for (size_t i = 0; i < h.size(); ++i) h[i] = (float)((i * 7) % 13) - 6.0f;
+  void* first = nullptr;
+  for (int round = 0; round < 2; ++round) {
+    DeviceMemoryReservation rx = mem.getMemory(s, h.size() * sizeof(float));
+    DeviceMemoryReservation rn = mem.getMemory(s, (size_t)n * sizeof(float));
+    CHECK(rx.get() && rn.get() && rx.size() == h.size() * sizeof(float));
+    if (round == 0) first = rx.get();
+    else CHECK(rx.get() == first);  // the released block came back
+    DeviceTensor<float, 2> x(static_cast<float*>(rx.get()), {n, dim}, MemorySpace::Device);
+    DeviceTensor<float, 1> nr(static_cast<float*>(rn.get()), {n}, MemorySpace::Device);
+    CHECK(!x.owner());
+    x.copyFrom(h.data(), s);
+    // norms of the second half only, through narrowed views of both tensors
+    Tensor<float, 2> xh = x.narrow(0, n / 2, n - n / 2);
+    Tensor<float, 1> nh = nr.narrow(0, n / 2, n - n / 2);
+    CHECK(xh.data() == x.data() + (size_t)(n / 2) * dim && xh.getSize(0) == n - n / 2);
+    nr.zero(s);
+    computeL2Norm(xh, nh, true, s);
+    nr.copyTo(out.data(), s);
+    streamSynchronize(s);
+    for (int i = 0; i < n; ++i) {
+      float e = 0.0f;
+      if (i >= n / 2)
+        for (int j = 0; j < dim; ++j) e = std::fma(h[(size_t)i * dim + j], h[(size_t)i * dim + j], e);
+      CHECK(out[i] == e);
+    }
+  }
+  const rplsh_memory_stats st = mem.stats();
+  CHECK(st.mallocs == 2 && st.reservations == 4 && st.reserved_bytes == 0);
+  CHECK(st.high_water_malloc_bytes >= (size_t)n * dim * 4 && mem.getSizeAvailable() >= (size_t)n * dim * 4);
+  std::printf("DeviceMemory ok: %s\n", mem.toString().c_str());
+  return 0;
+}
+
 int main() {
   if (getNumDevices() < 1) {
     std::fprintf(stderr, "no GPU\n");
@@ -287,6 +331,7 @@ int main() {
   if (test_dataset(res)) return 1;
   if (test_rplsh(res)) return 1;
   if (test_internal_select(res)) return 1;
+  if (test_device_memory(res)) return 1;
   std::printf("all C++ tests passed\n");
   return 0;
 }
