@@ -230,6 +230,7 @@ struct Plan {
   GraphKey key;
   size_t need = 0;          // bytes reserved while the batch runs
   size_t graph_part = 0;    // of need: the graph's workspace, moved to the cache once captured
+  std::unique_ptr<GraphEntry> temp_graph;  // captured but not cached (no room in the budget)
 };
 
 GraphEntry* find_graph(Slot& sl, const GraphKey& key) {
@@ -315,17 +316,31 @@ int run_batch(rplsh_server* s, Slot& sl, Plan& pl, const std::vector<Request*>& 
       std::unique_ptr<GraphEntry> g = capture_graph(sl, v, pl);
       std::lock_guard<std::mutex> lk(s->mu);
       evict_graphs(s, sl, g->bytes);
-      sl.graph_bytes += g->bytes;
-      s->graph_bytes += g->bytes;
-      // The estimate charged at admission becomes the graph's real size.
-      memory_force_charge(s->mem, g->bytes);
+      // The estimate charged at admission becomes the graph's real size;
+      // this slot's other cached graphs (none in use) make room if needed.
       memory_uncharge(s->mem, pl.graph_part);
       pl.need -= pl.graph_part;
       pl.graph_part = 0;
+      while (!memory_try_charge(s->mem, g->bytes)) {
+        if (sl.graphs.empty()) {
+          // No room to keep it: charged while this batch runs, then dropped.
+          memory_force_charge(s->mem, g->bytes);
+          pl.need += g->bytes;
+          pl.temp_graph = std::move(g);
+          break;
+        }
+        evict_graphs(s, sl, SIZE_MAX / 2);
+      }
       s->graph_captures += 1;
-      g->last_use = ++s->use_clock;
-      sl.graphs.push_back(std::move(g));
-      pl.g = sl.graphs.back().get();
+      if (pl.temp_graph) {
+        pl.g = pl.temp_graph.get();
+      } else {
+        sl.graph_bytes += g->bytes;
+        s->graph_bytes += g->bytes;
+        g->last_use = ++s->use_clock;
+        sl.graphs.push_back(std::move(g));
+        pl.g = sl.graphs.back().get();
+      }
     }
     float* q;
     int32_t* I;
