@@ -247,8 +247,8 @@ def main() -> int:
     # a device-to-device copy.
     measured = {"mfma_tflops_random": {}, "mfma_tflops_constant": {}}
     for f in ("bf16", "fp8", "fp32"):
-        measured["mfma_tflops_random"][f] = round(ca.probe_mfma_peak(f, True), 1)
-        measured["mfma_tflops_constant"][f] = round(ca.probe_mfma_peak(f, False), 1)
+        measured["mfma_tflops_random"][f] = round(mfma_peak(f, True), 1)
+        measured["mfma_tflops_constant"][f] = round(mfma_peak(f, False), 1)
     measured["hbm_copy_gbs"] = round(copy_bandwidth(ca, 1 << 30, 3), 1)
     log(f"[bench] measured peaks {measured}")
 
@@ -408,6 +408,21 @@ def pmc_traffic(args, tuning) -> dict:
             "fetch_kib_raw": got[("FETCH_SIZE", "filter")], "write_kib_raw": got[("WRITE_SIZE", "filter")],
             "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
             "calibration": "hipMemcpy device-to-device of 1 GiB: scale = 1 GiB / counter bytes"}
+
+
+def mfma_peak(kind: str, random_operands: bool) -> float:
+    """Measured MFMA throughput (TFLOP/s) from the benchmark-only probe library
+    (curplsh_amd/bench/probe.hip): back-to-back MFMAs, no memory traffic."""
+    import ctypes as C
+
+    lib = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "curplsh_amd",
+                              "libcurplsh_probe.so"))
+    v = C.c_double(0.0)
+    st = lib.rplsh_bench_mfma_peak(C.c_int({"bf16": 0, "fp8": 1, "fp32": 2}[kind]),
+                                   C.c_int(1 if random_operands else 0), C.byref(v))
+    if st != 0:
+        raise RuntimeError(f"rplsh_bench_mfma_peak failed with status {st}")
+    return float(v.value)
 
 
 def copy_bandwidth(ca, nbytes: int, reps: int) -> float:

@@ -665,16 +665,6 @@ def gen_mixture(n: int, d: int, centers: int, sigma: float, center_seed: int, po
     return out
 
 
-def probe_mfma_peak(kind: str = "bf16", random_operands: bool = True) -> float:
-    """Measured MFMA throughput of the current device (TFLOP/s) for "bf16",
-    "fp8" or "fp32": back-to-back MFMAs, no memory traffic; random operands
-    give the power-limited ceiling real data sees."""
-    v = C.c_double(0.0)
-    call("rplsh_probe_mfma_peak", {"bf16": 0, "fp8": 1, "fp32": 2}[kind], 1 if random_operands else 0,
-         C.byref(v))
-    return float(v.value)
-
-
 # ---- datasets (include/dataset.h) -------------------------------------------
 
 _XVECS_ELEM = {np.dtype(np.float32): 4, np.dtype(np.int32): 4, np.dtype(np.uint8): 1}

@@ -257,7 +257,6 @@ _SIGS = {
     "rplsh_gen_uniform": [_vp, _i64, _i32, _u64, _f32, _f32],
     "rplsh_gen_gaussian": [_vp, _i64, _i32, _u64, _i32],
     "rplsh_gen_mixture": [_vp, _i64, _i32, _i32, _f32, _u64, _u64, C.c_double],
-    "rplsh_probe_mfma_peak": [_i32, _i32, C.POINTER(C.c_double)],
     "rplsh_bf_save": [_vp, C.c_char_p],
     "rplsh_bf_load": [_vp, C.c_char_p, _i32, C.POINTER(_vp)],
     "rplsh_xvecs_info": [C.c_char_p, _i32, C.POINTER(_i64), C.POINTER(_i32)],

@@ -4,6 +4,11 @@
 // waves per CU. Operands are random finite values (`random`) or a constant
 // pattern: this part's MFMA throughput is power-limited, so random operands
 // (what real data feeds the filter) give the lower, realistic ceiling.
+//
+// Built into libcurplsh_probe.so for the benchmark only; not part of the
+// library's C ABI:
+//   rplsh_status rplsh_bench_mfma_peak(int kind, int random_operands, double* tflops);
+// kind 0 = bf16 (32x32x16), 1 = fp8 e4m3 (32x32x64 scaled), 2 = fp32 (32x32x2).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -112,7 +117,7 @@ double mfma_flops(int kind) { return kind == 0 ? 2.0 * 32 * 32 * 16 : (kind == 1
 
 using namespace rplsh;
 
-extern "C" rplsh_status rplsh_probe_mfma_peak(int kind, int random_operands, double* tflops) {
+extern "C" rplsh_status rplsh_bench_mfma_peak(int kind, int random_operands, double* tflops) {
   return guard([&] {
     RPLSH_REQUIRE(kind >= 0 && kind <= 2, "kind: 0 bf16, 1 fp8 e4m3, 2 fp32");
     RPLSH_REQUIRE(tflops, "NULL output");

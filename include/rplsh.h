@@ -529,13 +529,6 @@ rplsh_status rplsh_xvecs_write(const char* path, int elem_bytes, const void* dat
 rplsh_status rplsh_recall_at(const int32_t* result, int64_t nq, int ld_result, const int32_t* truth,
                              int ld_truth, int k, int32_t* missed, double* recall);
 
-/* ---- measurement ----------------------------------------------------------
- * No reference counterpart. Measured MFMA throughput of the current device in
- * TFLOP/s for kind 0 = bf16 (32x32x16), 1 = fp8 e4m3 (32x32x64 scaled),
- * 2 = fp32 (32x32x2): back-to-back MFMAs on four accumulators per wave, two
- * waves per SIMD, no memory traffic. random_operands != 0 feeds random finite
- * values (the realistic, power-limited ceiling), 0 a constant pattern. */
-rplsh_status rplsh_probe_mfma_peak(int kind, int random_operands, double* tflops);
 
 #ifdef __cplusplus
 }
