@@ -491,7 +491,8 @@ class Server:
 
     def __init__(self, device: int = 0, slots: int = 4, hbm_budget_bytes: int = 0,
                  max_batch_queries: int = 4096, use_graphs: bool = True,
-                 graph_max_queries: int = 1024, memory: "HbmBudget | None" = None):
+                 graph_max_queries: int = 1024, memory: "HbmBudget | None" = None,
+                 persistent_max_queries: int = 256):
         load()
         cfg = _lib.ServerConfig()
         _lib.load().rplsh_server_config_default(C.byref(cfg))
@@ -499,6 +500,7 @@ class Server:
         cfg.hbm_budget_bytes, cfg.max_batch_queries = hbm_budget_bytes, max_batch_queries
         cfg.use_graphs, cfg.graph_max_queries = int(use_graphs), graph_max_queries
         cfg.memory = memory._h.value if memory is not None else None
+        cfg.persistent_max_queries = int(persistent_max_queries)
         self._memory = memory
         self._h = C.c_void_p()
         call("rplsh_server_create", C.byref(cfg), C.byref(self._h))

@@ -97,6 +97,7 @@ class ServerConfig(C.Structure):
         ("graph_max_queries", C.c_int),
         ("graph_cache_bytes", C.c_size_t),
         ("memory", C.c_void_p),
+        ("persistent_max_queries", C.c_int),
     ]
 
 
@@ -123,6 +124,7 @@ class RequestInfo(C.Structure):
         ("batch_requests", C.c_int),
         ("batch_queries", C.c_int64),
         ("device_ms", C.c_float),
+        ("kernel_batches", C.c_int),
     ]
 
 
@@ -144,6 +146,9 @@ class ServerStats(C.Structure):
         ("graph_launches", C.c_int64),
         ("graph_captures", C.c_int64),
         ("graph_cache_bytes", C.c_size_t),
+        ("persistent_launches", C.c_int64),
+        ("persistent_batches", C.c_int64),
+        ("persistent_aborts", C.c_int64),
     ]
 
 

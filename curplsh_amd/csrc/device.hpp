@@ -139,4 +139,57 @@ struct RerankArgs {
   int* fp64_count;       // total boundary candidates re-ranked in FP64
 };
 
+// Persistent serving kernel (persist.hip). Query preparation of one batch:
+// canonical FP32 norms, the encoded copy for the candidate contraction
+// (BF16, or e4m3 + power-of-two row scale), threshold +inf, status 0, the
+// first level's count preset, and the batch counters cleared.
+struct PrepArgs {
+  const float* q;   // FP32 queries [nq][d] (device)
+  int nq;
+  int d;
+  int dpad;
+  int fp8;          // 0: BF16, 1: FP8 e4m3
+  float* qnorm;
+  void* qenc;       // [nq][dpad]
+  float* qscale;    // FP8 only
+  float* tau;
+  int* status;
+  int* counts;
+  int count_init;   // counts[q] before the first level (its row count when dense)
+  int* ctr;         // BatchCounters, four ints
+};
+
+// The re-rank of a batch plus what its failed-query list and survivor total
+// need (collect_failed and sum_counts of the launched pipeline).
+struct RerankTailArgs {
+  RerankArgs ra;
+  const int* counts;   // last level's survivors per query
+  int* flist;          // failed queries
+  int* failed;         // BatchCounters::failed
+  long long* survivors;
+};
+
+// Longest candidate list the persistent kernel selects (a wave per query).
+constexpr int kPersistMaxKp = 512;
+
+// A phase of one batch in a persistent program: `units` consecutive work
+// units from `begin`, each waiting for every unit of phase `dep` (-1: none).
+enum PersistKind : int { kPkPrep = 0, kPkFilter = 1, kPkSelect = 2, kPkRerank = 3 };
+struct PersistPhase {
+  int kind;
+  int begin;        // first unit
+  int units;
+  int dep;          // phase index it waits for, or -1
+  int arg;          // byte offset of its argument record in the program
+  int count_reset;  // select phases: counts[q] after the select (-1: unchanged)
+  int pad[2];
+};
+// Program layout: PersistHeader, PersistPhase[nphases], argument records
+// (16-byte aligned).
+struct PersistHeader {
+  int nphases;
+  int total_units;
+  int pad[2];
+};
+
 }  // namespace rplsh

@@ -219,10 +219,12 @@ struct SearchParams {
 void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
                    const SearchParams& p, int32_t* out_i, float* out_d, Workspace& ws,
                    hipStream_t st, PhaseEvents* pe);
-// Waits for st, reads the counters, re-runs failed queries eagerly.
+// Waits for st, reads the counters (unless `ready` already holds them),
+// re-runs failed queries eagerly.
 void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
                   int32_t* out_i, float* out_d, std::vector<Workspace>& wss, int depth,
-                  hipStream_t st, rplsh_search_stats& stats, const PhaseEvents* pe);
+                  hipStream_t st, rplsh_search_stats& stats, const PhaseEvents* pe,
+                  const BatchCounters* ready = nullptr);
 
 // One batch of nq device-resident FP32 queries [nq][d] -> device outputs
 // [nq][k], on stream st, with the workspaces wss (one per retry depth).
@@ -230,6 +232,40 @@ void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& 
 void search_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
                   int32_t* out_i, float* out_d, std::vector<Workspace>& wss, int depth,
                   hipStream_t st, rplsh_search_stats& stats);
+
+// A batch as argument records for the persistent kernel (search.hip).
+struct PersistBatchPlan {
+  PrepArgs prep;
+  std::vector<FilterArgs> filter;  // per level
+  std::vector<SelectArgs> select;  // per level
+  std::vector<int> count_reset;    // per level: counts after its select (-1: unchanged)
+  RerankTailArgs rerank;
+};
+void plan_persistent_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
+                           int32_t* out_i, float* out_d, Workspace& ws, int64_t target_units,
+                           PersistBatchPlan& out);
+
+// Persistent serving kernel (persist.hip): whether it takes a batch, and
+// one launch over several batches of the same element kind and padded
+// dimension. Each batch's depth-0 results, status, failed list and counters
+// land where enqueue_batch puts them (finish_batch completes the batch).
+// Returns false if the kernel gave up (a dependency wait passed its bound):
+// the batches' outputs are then not valid and must be recomputed. On return
+// the stream is synchronised and each batch's counters are in ws->hctr.
+bool persist_supported(const DbView& db, const SearchParams& p);
+struct PersistState;  // per-stream program and synchronisation buffers
+PersistState* persist_state_create();
+void persist_state_destroy(PersistState* s);
+struct PersistBatch {
+  const DbView* db;
+  const float* q;  // device FP32 queries
+  int nq;
+  const SearchParams* p;
+  int32_t* out_i;
+  float* out_d;
+  Workspace* ws;
+};
+bool persist_run(PersistState& state, const std::vector<PersistBatch>& batches, hipStream_t st);
 
 // Device bytes search_batch allocates for one batch (depth-0 workspace).
 size_t search_workspace_bytes(const DbView& db, int nq, const SearchParams& p);

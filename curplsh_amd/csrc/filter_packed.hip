@@ -26,89 +26,16 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "encode.hpp"
 #include "kernels.hpp"
+#include "packed_mma.hpp"
 
 namespace rplsh {
 
 namespace {
 
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef int i32x4 __attribute__((ext_vector_type(4)));
-typedef int i32x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
 constexpr int kBlock = 256;
 
-template <Elem E>
-struct PackTraits;
-template <>
-struct PackTraits<Elem::BF16> {
-  static constexpr int kBytes = 2;
-  static constexpr int kLaneBytes = 16;  // 8 bf16
-  static constexpr int kChunkElems = 16;
-};
-template <>
-struct PackTraits<Elem::FP8> {
-  static constexpr int kBytes = 1;
-  static constexpr int kLaneBytes = 32;  // 32 e4m3
-  static constexpr int kChunkElems = 64;
-};
-
-template <int LB>
-struct Frag {
-  i32x4 v[LB / 16];
-};
-
-template <int LB>
-__device__ inline Frag<LB> load_frag(const char* p) {
-  Frag<LB> f;
-#pragma unroll
-  for (int i = 0; i < LB / 16; ++i) f.v[i] = *reinterpret_cast<const i32x4*>(p + 16 * i);
-  return f;
-}
-
-template <int LB>
-__device__ inline Frag<LB> zero_frag() {
-  Frag<LB> f;
-#pragma unroll
-  for (int i = 0; i < LB / 16; ++i) f.v[i] = i32x4{0, 0, 0, 0};
-  return f;
-}
-
-template <Elem E>
-__device__ inline f32x16 mfma_chunk(const Frag<PackTraits<E>::kLaneBytes>& a,
-                                    const Frag<PackTraits<E>::kLaneBytes>& b, f32x16 c, int sa,
-                                    int sb);
-
-template <>
-__device__ inline f32x16 mfma_chunk<Elem::BF16>(const Frag<16>& a, const Frag<16>& b, f32x16 c,
-                                                int, int) {
-  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a.v[0]),
-                                                  __builtin_bit_cast(bf16x8, b.v[0]), c, 0, 0, 0);
-}
-
-template <>
-__device__ inline f32x16 mfma_chunk<Elem::FP8>(const Frag<32>& a, const Frag<32>& b, f32x16 c,
-                                               int sa, int sb) {
-  i32x8 av, bv;
-  av.lo = a.v[0];
-  av.hi = a.v[1];
-  bv.lo = b.v[0];
-  bv.hi = b.v[1];
-  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, c, 0, 0, 0, sa, 0, sb);
-}
-
-__device__ inline int e8m0_of(float s) { return (int)((__float_as_uint(s) >> 23) & 0xff); }
-
-template <Elem E, int DPAD>
-struct Geometry {
-  static constexpr int LB = PackTraits<E>::kLaneBytes;
-  static constexpr int CHUNK = 64 * LB;                       // bytes of one K-chunk of a tile
-  static constexpr int NC = DPAD / PackTraits<E>::kChunkElems;  // K-chunks per row
-  static constexpr int TILE = NC * CHUNK;                     // bytes of a 32-row tile
-  static constexpr int ROWB = DPAD * PackTraits<E>::kBytes;   // bytes of a query row
-};
 
 template <Elem E, int DPAD, int TN>
 __global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) {
@@ -944,32 +871,6 @@ __global__ __launch_bounds__(kBlock, (RingShape<E, DPAD, WIDE>::WAVES_PER_SIMD))
 }
 
 // Packed encoders: one thread per 16-byte output piece.
-__device__ inline uint16_t bf16_rne(float v) {
-  uint32_t u = __float_as_uint(v);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-
-__device__ inline uint8_t e4m3_rne_sat(float v) {
-  if (isnan(v)) return 0x7f;
-  const uint8_t sign = signbit(v) ? 0x80 : 0x00;
-  const float a = fabsf(v);
-  if (a >= 448.0f) return sign | 0x7e;
-  if (a < 0.015625f) return sign | (uint8_t)(int)rintf(a * 512.0f);
-  int e2;
-  frexpf(a, &e2);
-  int eb = e2 - 1;
-  int r = (int)rintf(ldexpf(a, 3 - eb));
-  if (r == 16) {
-    r = 8;
-    ++eb;
-  }
-  int code = ((eb + 7) << 3) | (r - 8);
-  if (code > 0x7e) code = 0x7e;
-  return sign | (uint8_t)code;
-}
-
 template <Elem E, int DPAD>
 __global__ void encode_packed_kernel(const float* __restrict__ x, int64_t n, int d, int64_t ldx,
                                      int64_t tile_from, int64_t tile_to, char* __restrict__ out,
@@ -1022,14 +923,7 @@ __global__ __launch_bounds__(256) void e4m3_scales_kernel(const float* __restric
   for (int j = lane; j < d; j += 64) mx = fmaxf(mx, fabsf(x[row * ldx + j]));
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-  float s = 1.0f;
-  if (mx > 0.0f) {
-    const float s0 = mx / 448.0f;
-    int e;
-    const float f = frexpf(s0, &e);
-    s = f == 0.5f ? s0 : ldexpf(1.0f, e);
-  }
-  if (lane == 0) scales[row] = s;
+  if (lane == 0) scales[row] = e4m3_row_scale(mx);
 }
 
 // Query tiles per wave of the direct variant: as many as keep the query

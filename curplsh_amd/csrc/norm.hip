@@ -16,6 +16,7 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "encode.hpp"
 #include "kernels.hpp"
 
 namespace rplsh {
@@ -68,14 +69,6 @@ __global__ __launch_bounds__(256) void l2norm_kernel(const float* __restrict__ x
   if (row < n) out[row] = squared ? s : sqrtf(s);
 }
 
-// bf16 round-to-nearest-even of one float (NaN stays a quiet NaN).
-__device__ inline uint16_t bf16_rne(float v) {
-  uint32_t u = __float_as_uint(v);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-
 __global__ void encode_bf16_kernel(const float* __restrict__ x, int64_t n, int d, int64_t ldx,
                                    int ld, uint16_t* __restrict__ out) {
   const int64_t total = n * ld;
@@ -85,31 +78,6 @@ __global__ void encode_bf16_kernel(const float* __restrict__ x, int64_t n, int d
     const int j = (int)(e - i * ld);
     out[e] = j < d ? bf16_rne(x[i * ldx + j]) : (uint16_t)0;
   }
-}
-
-// OCP FP8 e4m3 (FN: no infinities, max 448) round-to-nearest-even with
-// saturation. Scaling by powers of two is exact, so rintf on the scaled
-// magnitude is the RNE step of the target binade.
-__device__ inline uint8_t e4m3_rne_sat(float v) {
-  if (isnan(v)) return 0x7f;
-  const uint8_t sign = signbit(v) ? 0x80 : 0x00;
-  const float a = fabsf(v);
-  if (a >= 448.0f) return sign | 0x7e;
-  if (a < 0.015625f) {  // below 2^-6: subnormal quantum 2^-9; m == 8 is 2^-6
-    const int m = (int)rintf(a * 512.0f);
-    return sign | (uint8_t)m;
-  }
-  int e2;
-  frexpf(a, &e2);  // a = f * 2^e2, f in [0.5, 1): binade 2^(e2-1)
-  int eb = e2 - 1;
-  int r = (int)rintf(ldexpf(a, 3 - eb));  // in [8, 16]
-  if (r == 16) {
-    r = 8;
-    ++eb;
-  }
-  int code = ((eb + 7) << 3) | (r - 8);
-  if (code > 0x7e) code = 0x7e;
-  return sign | (uint8_t)code;
 }
 
 // One wave per row: power-of-two scale s >= max|x| / 448, then codes of x / s.
@@ -125,15 +93,7 @@ __global__ __launch_bounds__(256) void encode_e4m3_kernel(const float* __restric
   for (int j = lane; j < d; j += 64) mx = fmaxf(mx, fabsf(xr[j]));
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-  // Smallest power of two >= max|x| / 448, so that max|x| / s <= 448 and the
-  // scale is exact as an E8M0 exponent.
-  float s = 1.0f;
-  if (mx > 0.0f) {
-    const float s0 = mx / 448.0f;
-    int e;
-    const float f = frexpf(s0, &e);  // s0 = f * 2^e, f in [0.5, 1)
-    s = f == 0.5f ? s0 : ldexpf(1.0f, e);
-  }
+  const float s = e4m3_row_scale(mx);
   for (int j = lane; j < ld; j += 64) out[row * ld + j] = j < d ? e4m3_rne_sat(xr[j] / s) : 0;
   if (lane == 0) scales[row] = s;
 }

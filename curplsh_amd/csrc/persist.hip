@@ -1,0 +1,446 @@
+// Persistent serving kernel: one launch runs whole searches -- query
+// preparation, every filter level with its select, and the exact re-rank --
+// for several batches at once, against indexes of different sizes (one
+// kernel per element kind and padded dimension).
+//
+// The host writes a program: per batch, its phases in order (prep, filter and
+// select of each level, re-rank), each a run of work units with the argument
+// record of that phase. Resident workgroups take units from one device-wide
+// counter in program order; a unit first waits until every unit of the phase
+// it depends on (the batch's previous phase) has completed. Units are handed
+// out in order and a unit only waits for units handed out before it, which
+// are held by running workgroups, so the waits always end; a bounded spin
+// (kSpinLimit) still stops the launch (abort flag, every workgroup leaves)
+// rather than hang if that were ever violated, and the host then recomputes
+// the batches. Completion is published with an agent-scope release fence and
+// a relaxed counter add; waiters take an agent-scope acquire fence, so the
+// lists, thresholds and candidates written on other XCDs are visible.
+//
+// Units per phase: prep and select take four queries (a wave each); a filter
+// unit takes 32 queries x rows_per_slice rows, its four waves taking the
+// slice's 32-row tiles in turn, with the query operands in registers and the
+// row operands loaded straight from the packed copy (the tile layout is the
+// MFMA operand order, so every load is 16 or 32 contiguous bytes per lane);
+// a re-rank unit takes one query. The per-tile arithmetic, the selects and the
+// re-rank are the same device code and instruction sequence as the launched
+// pipeline's (packed_mma.hpp, select_body.hpp, encode.hpp), so results are
+// bit-identical to rplsh_bf_search.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "encode.hpp"
+#include "engine.hpp"
+#include "kernels.hpp"
+#include "packed_mma.hpp"
+#include "select_body.hpp"
+
+namespace rplsh {
+
+namespace {
+
+constexpr int kPBlock = 256;
+constexpr unsigned kSpinLimit = 1u << 26;  // ~4 s of s_sleep(2) before giving up
+
+// Bytes of dynamic LDS a launch carries: the larger of the selects' (four
+// waves x (256-bin histogram + kWaveSelK words)) and the re-rank's region at
+// the longest list, with the FP64 band and the staged rows of d = 256.
+constexpr size_t kSelectLds = (size_t)kWaveSelQ * (256 * 4 + kWaveSelK * 8);
+inline size_t persist_lds_bytes() {
+  const size_t rr = rerank_lds_bytes(kPersistMaxKp, true, 256, true);
+  return rr > kSelectLds ? rr : kSelectLds;
+}
+
+// ---- phase bodies --------------------------------------------------------
+
+__device__ void prep_unit(const PrepArgs& a, int local) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int q = local * 4 + w;
+  if (local == 0 && threadIdx.x < 4) a.ctr[threadIdx.x] = 0;
+  if (q >= a.nq) return;
+  const float* row = a.q + (int64_t)q * a.d;
+  if (lane == 0) {
+    // Canonical norm: one fmaf chain over j (as l2norm_kernel).
+    float s = 0.0f;
+    for (int j = 0; j < a.d; ++j) s = fmaf(row[j], row[j], s);
+    a.qnorm[q] = s;
+    a.tau[q] = INFINITY;
+    a.status[q] = 0;
+    a.counts[q] = a.count_init;
+  }
+  if (!a.fp8) {
+    uint16_t* out = static_cast<uint16_t*>(a.qenc) + (int64_t)q * a.dpad;
+    for (int j = lane; j < a.dpad; j += 64) out[j] = j < a.d ? bf16_rne(row[j]) : (uint16_t)0;
+  } else {
+    float mx = 0.0f;
+    for (int j = lane; j < a.d; j += 64) mx = fmaxf(mx, fabsf(row[j]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    const float s = e4m3_row_scale(mx);
+    uint8_t* out = static_cast<uint8_t*>(a.qenc) + (int64_t)q * a.dpad;
+    for (int j = lane; j < a.dpad; j += 64) out[j] = j < a.d ? e4m3_rne_sat(row[j] / s) : 0;
+    if (lane == 0) a.qscale[q] = s;
+  }
+}
+
+template <Elem E, int DPAD>
+__device__ void filter_unit(const FilterArgs& a, int local) {
+  using G = Geometry<E, DPAD>;
+  constexpr int LB = G::LB;
+  constexpr int NC = G::NC;
+  const int qtile = local % a.num_qtiles;
+  const int slice = local / a.num_qtiles;
+  const int64_t r_begin = (int64_t)slice * a.rows_per_slice;
+  const int64_t r_end = min(r_begin + a.rows_per_slice, a.m);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5, col = lane & 31;
+  const int qr = qtile * 32 + col;
+  const int nq = a.nq;
+
+  Frag<LB> qf[NC];
+  float thr = INFINITY;
+  int qsc = 127;
+  if (qr < nq) {
+    const char* qp = reinterpret_cast<const char*>(a.q) + (int64_t)qr * G::ROWB;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) qf[c] = load_frag<LB>(qp + 2 * LB * c + LB * h);
+    thr = fmaxf(-0.5f * a.tau[qr], -FLT_MAX);
+    if (E == Elem::FP8) qsc = e8m0_of(a.qscale[qr]);
+  } else {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) qf[c] = zero_frag<LB>();
+  }
+  const char* xb = reinterpret_cast<const char*>(a.x);
+  const int64_t stride = a.stride;
+  int* cnt = a.counts + qr;
+  uint64_t* dst = a.cand + (int64_t)qr * a.cap;
+
+#pragma unroll 1
+  for (int64_t t0 = r_begin + 32 * wave; t0 < r_end; t0 += 128) {
+    // Operand A: this lane's row of the tile (the same 32 rows for both
+    // halves of the wave, each half its part of every K-chunk).
+    const int64_t j = t0 + col;
+    const bool valid = j < r_end;
+    const int64_t p = valid ? j * stride : 0;
+    const char* ap = xb + (p >> 5) * G::TILE + ((p & 31) + 32 * h) * LB;
+    // C = -||x||^2 / 2 of rows t0 + 8g + 4h + i (-inf past the end).
+    f32x16 ci;
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t r = t0 + 8 * g + 4 * h + i;
+        ci[4 * g + i] = r < r_end ? -0.5f * a.xnorm[r * stride] : -INFINITY;
+      }
+    const int xsc = (E == Elem::FP8 && valid) ? e8m0_of(a.xscale[p]) : 127;
+    f32x16 acc;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const Frag<LB> af = valid ? load_frag<LB>(ap + c * G::CHUNK) : zero_frag<LB>();
+      acc = mfma_chunk<E>(af, qf[c], c == 0 ? ci : acc, xsc, qsc);
+    }
+    if (a.dense) {
+      // Every row survives (tau = +inf): row r's word goes to slot r.
+      if (qr < nq) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int64_t r = t0 + 8 * (i >> 2) + 4 * h + (i & 3);
+          if (r < r_end) dst[r] = pack_cand(-2.0f * acc[i], (int32_t)(r * stride));
+        }
+      }
+      continue;
+    }
+    float m = acc[0];
+#pragma unroll
+    for (int i = 1; i < 16; ++i) m = fmaxf(m, acc[i]);
+    if (__any(m >= thr)) {
+      if (m >= thr) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const float v = acc[i];
+          if (v >= thr) {
+            const int64_t r = t0 + 8 * (i >> 2) + 4 * h + (i & 3);
+            const int pos = atomicAdd(cnt, 1);
+            if (pos < a.cap) dst[pos] = pack_cand(-2.0f * v, (int32_t)(r * stride));
+          }
+        }
+      }
+    }
+  }
+}
+
+__device__ void select_unit(const SelectArgs& a, int local, int count_reset, char* lds) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int q = local * 4 + w;
+  if (q >= a.nq) return;
+  int* hist = reinterpret_cast<int*>(lds) + w * 256;
+  uint64_t* out = reinterpret_cast<uint64_t*>(lds + kWaveSelQ * 256 * 4) + w * kWaveSelK;
+  select_wave_query(a, q, hist, out);
+  // The wave has read this query's count and list; the next level appends
+  // to it from count_reset.
+  if (count_reset >= 0 && lane == 0) const_cast<int*>(a.counts)[q] = count_reset;
+}
+
+template <int NP>
+__device__ void rerank_unit(const RerankTailArgs& t, int q, uint64_t* lds, float* sq, float* sxn, int* sh) {
+  rerank_query<NP>(t.ra, q, lds, sq, sxn, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // collect_failed and sum_counts of the launched pipeline.
+    if (__hip_atomic_load(t.ra.status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+      const int pos = atomicAdd(t.failed, 1);
+      t.flist[pos] = q;
+    }
+    atomicAdd(reinterpret_cast<unsigned long long*>(t.survivors), (unsigned long long)t.counts[q]);
+  }
+}
+
+// ---- the kernel ----------------------------------------------------------
+
+// sync: [0] unit counter, [1] abort flag, [2 + i] completed units of phase i
+// (zeroed before the launch).
+template <Elem E, int DPAD>
+__global__ __launch_bounds__(kPBlock, 2) void persistent_search_kernel(const char* __restrict__ prog,
+                                                                       int* __restrict__ sync) {
+  extern __shared__ __attribute__((aligned(16))) uint64_t plds[];
+  __shared__ float sq_fixed[256];
+  __shared__ float sxn[kSelWaves * 64];
+  __shared__ int sh[4];
+  __shared__ int s_unit, s_go;
+  const PersistHeader* hd = reinterpret_cast<const PersistHeader*>(prog);
+  const PersistPhase* ph = reinterpret_cast<const PersistPhase*>(prog + sizeof(PersistHeader));
+  const int total = hd->total_units;
+  const int nph = hd->nphases;
+  int* claim = sync;
+  int* abort = sync + 1;
+  int* done = sync + 2;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int u = atomicAdd(claim, 1);
+      if (u < total && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) u = total;
+      s_unit = u;
+    }
+    __syncthreads();
+    const int u = __builtin_amdgcn_readfirstlane(s_unit);
+    if (u >= total) break;
+    // The phase holding unit u (phases are sorted by their first unit).
+    int lo = 0, hi = nph - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (ph[mid].begin <= u) lo = mid;
+      else hi = mid - 1;
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    const PersistPhase P = ph[lo];
+    if (P.dep >= 0) {
+      if (threadIdx.x == 0) {
+        const int need = ph[P.dep].units;
+        int ok = 1;
+        unsigned spins = 0;
+        while (__hip_atomic_load(done + P.dep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            ok = 0;
+            break;
+          }
+          if (++spins > kSpinLimit) {
+            __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = 0;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+        s_go = ok;
+      }
+      __syncthreads();
+      if (!s_go) break;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    const int local = u - P.begin;
+    const char* arg = prog + P.arg;
+    switch (P.kind) {
+      case kPkPrep:
+        prep_unit(*reinterpret_cast<const PrepArgs*>(arg), local);
+        break;
+      case kPkFilter:
+        filter_unit<E, DPAD>(*reinterpret_cast<const FilterArgs*>(arg), local);
+        break;
+      case kPkSelect:
+        select_unit(*reinterpret_cast<const SelectArgs*>(arg), local, P.count_reset,
+                    reinterpret_cast<char*>(plds));
+        break;
+      default: {
+        const RerankTailArgs& t = *reinterpret_cast<const RerankTailArgs*>(arg);
+        if (rerank_staged(t.ra.x, t.ra.ldx, t.ra.d)) {
+          if (t.ra.d <= 128) rerank_unit<1>(t, local, plds, sq_fixed, sxn, sh);
+          else rerank_unit<2>(t, local, plds, sq_fixed, sxn, sh);
+        } else {
+          rerank_unit<0>(t, local, plds, sq_fixed, sxn, sh);
+        }
+        break;
+      }
+    }
+    // Publish this unit's writes, then count it.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(done + lo, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+typedef void (*PersistKernel)(const char*, int*);
+
+template <Elem E, int DPAD>
+PersistKernel kernel_for() {
+  return persistent_search_kernel<E, DPAD>;
+}
+
+PersistKernel persist_kernel(Elem e, int dpad) {
+  if (e == Elem::BF16) {
+    switch (dpad) {
+      case 32: return kernel_for<Elem::BF16, 32>();
+      case 64: return kernel_for<Elem::BF16, 64>();
+      case 96: return kernel_for<Elem::BF16, 96>();
+      case 128: return kernel_for<Elem::BF16, 128>();
+      case 160: return kernel_for<Elem::BF16, 160>();
+      case 192: return kernel_for<Elem::BF16, 192>();
+      case 256: return kernel_for<Elem::BF16, 256>();
+      default: return nullptr;
+    }
+  }
+  if (e == Elem::FP8) {
+    switch (dpad) {
+      case 64: return kernel_for<Elem::FP8, 64>();
+      case 128: return kernel_for<Elem::FP8, 128>();
+      case 256: return kernel_for<Elem::FP8, 256>();
+      default: return nullptr;
+    }
+  }
+  return nullptr;
+}
+
+}  // namespace
+
+struct PersistState {
+  DevBuf prog, sync;
+  HostBuf hprog;  // program staging
+  HostBuf hflag;  // abort flag read-back
+  int resident = 0;  // workgroups the device holds at once (per kernel)
+};
+
+PersistState* persist_state_create() { return new PersistState(); }
+void persist_state_destroy(PersistState* s) { delete s; }
+
+bool persist_supported(const DbView& db, const SearchParams& p) {
+  return db.packed && db.n > 0 && db.elem != Elem::F32 && db.d <= 256 &&
+         persist_kernel(db.elem, db.dpad) != nullptr && std::min<int64_t>(p.kp, db.n) <= kPersistMaxKp;
+}
+
+bool persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipStream_t st) {
+  RPLSH_REQUIRE(!batches.empty(), "persistent kernel: no batches");
+  const Elem e = batches[0].db->elem;
+  const int dpad = batches[0].db->dpad;
+  PersistKernel kern = persist_kernel(e, dpad);
+  RPLSH_REQUIRE(kern, "persistent kernel: element kind / dimension not instantiated");
+  const size_t lds = persist_lds_bytes();
+  if (S.resident == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    RPLSH_HIP(hipGetDevice(&dev));
+    RPLSH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    RPLSH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),
+                                                           kPBlock, lds));
+    S.resident = std::max(1, cus * std::max(1, per_cu));
+  }
+  const int64_t target_units = S.resident;
+
+  // Plans, then the program: phases ordered by position (all batches' preps,
+  // then their first filters, ...), each depending on its batch's previous
+  // phase.
+  std::vector<PersistBatchPlan> plans(batches.size());
+  size_t max_phases = 0;
+  for (size_t b = 0; b < batches.size(); ++b) {
+    const PersistBatch& pb = batches[b];
+    RPLSH_REQUIRE(pb.db->elem == e && pb.db->dpad == dpad, "persistent kernel: mixed element kinds / dimensions");
+    plan_persistent_batch(*pb.db, pb.q, pb.nq, *pb.p, pb.out_i, pb.out_d, *pb.ws, target_units, plans[b]);
+    max_phases = std::max(max_phases, 2 + 2 * plans[b].filter.size());
+  }
+  struct Rec {
+    int kind, units, count_reset;
+    const void* arg;
+    size_t bytes;
+  };
+  std::vector<std::vector<Rec>> per(batches.size());
+  for (size_t b = 0; b < batches.size(); ++b) {
+    const PersistBatchPlan& pl = plans[b];
+    const int nq = batches[b].nq;
+    const int qunits = (int)div_up(nq, 4);
+    per[b].push_back({kPkPrep, qunits, -1, &pl.prep, sizeof(PrepArgs)});
+    for (size_t l = 0; l < pl.filter.size(); ++l) {
+      const FilterArgs& fa = pl.filter[l];
+      per[b].push_back({kPkFilter, fa.num_qtiles * fa.num_slices, -1, &fa, sizeof(FilterArgs)});
+      per[b].push_back({kPkSelect, qunits, pl.count_reset[l], &pl.select[l], sizeof(SelectArgs)});
+    }
+    per[b].push_back({kPkRerank, nq, -1, &pl.rerank, sizeof(RerankTailArgs)});
+  }
+  std::vector<PersistPhase> phases;
+  std::vector<std::pair<const void*, size_t>> args;
+  std::vector<int> last(batches.size(), -1);
+  int units = 0;
+  for (size_t pos = 0; pos < max_phases; ++pos) {
+    for (size_t b = 0; b < batches.size(); ++b) {
+      if (pos >= per[b].size()) continue;
+      const Rec& r = per[b][pos];
+      PersistPhase P{};
+      P.kind = r.kind;
+      P.begin = units;
+      P.units = r.units;
+      P.dep = last[b];
+      P.count_reset = r.count_reset;
+      P.arg = (int)args.size();  // index for now, byte offset below
+      args.push_back({r.arg, r.bytes});
+      last[b] = (int)phases.size();
+      units += r.units;
+      phases.push_back(P);
+    }
+  }
+  size_t off = round_up(sizeof(PersistHeader) + phases.size() * sizeof(PersistPhase), 16);
+  std::vector<size_t> arg_off(args.size());
+  for (size_t i = 0; i < args.size(); ++i) {
+    arg_off[i] = off;
+    off = round_up(off + args[i].second, 16);
+  }
+  RPLSH_REQUIRE(off < (size_t)INT32_MAX, "persistent program too large");
+  char* hp = S.hprog.as<char>(off);
+  PersistHeader hdr{};
+  hdr.nphases = (int)phases.size();
+  hdr.total_units = units;
+  std::memcpy(hp, &hdr, sizeof(hdr));
+  for (size_t i = 0; i < phases.size(); ++i) {
+    phases[i].arg = (int)arg_off[phases[i].arg];
+    std::memcpy(hp + sizeof(PersistHeader) + i * sizeof(PersistPhase), &phases[i], sizeof(PersistPhase));
+  }
+  for (size_t i = 0; i < args.size(); ++i) std::memcpy(hp + arg_off[i], args[i].first, args[i].second);
+
+  char* dprog = S.prog.as<char>(off);
+  int* sync = S.sync.as<int>(2 + phases.size());
+  RPLSH_HIP(hipMemcpyAsync(dprog, hp, off, hipMemcpyHostToDevice, st));
+  launch_fill_i32(sync, 2 + (int64_t)phases.size(), 0, st);
+  const int grid = (int)std::min<int64_t>(units, S.resident);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kPBlock), lds, st, (const char*)dprog, sync);
+  RPLSH_HIP(hipGetLastError());
+  // One synchronisation for the abort flag and every batch's counters.
+  int* hflag = S.hflag.as<int>(1);
+  RPLSH_HIP(hipMemcpyAsync(hflag, sync + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  for (const PersistBatch& pb : batches) {
+    BatchCounters* hc = pb.ws->hctr.as<BatchCounters>(1);
+    RPLSH_HIP(hipMemcpyAsync(hc, pb.ws->ctr.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
+  }
+  RPLSH_HIP(hipStreamSynchronize(st));
+  return *hflag == 0;
+}
+
+}  // namespace rplsh

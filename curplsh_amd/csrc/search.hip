@@ -151,11 +151,9 @@ static int64_t level_cap(const std::vector<Level>& L, int kp, int c_factor) {
 // level scored are skipped; `row_begin` > 0 (sample level after a carried
 // one): rows below it are not scored. Either way the lists already hold the
 // carried survivors.
-static void run_filter(const DbView& db, const Level& lv, bool dense, const void* qenc,
-                       const float* qscale, const float* tau, int nq, const int* nq_dev, int cap,
-                       int* counts, uint64_t* cand, int64_t target_wgs, int variant,
-                       int64_t skip_tiles, int64_t row_begin, hipStream_t st) {
-  FilterArgs fa{};
+// The rows a level reads: the index (every stride-th row) or, for a sample
+// level of a packed index, the first lv.m rows of its sample copy.
+static void filter_source(FilterArgs& fa, const DbView& db, const Level& lv) {
   fa.x = db.xenc;
   fa.ldx = db.ldenc;
   fa.dvalid_bytes = db.dvalid_bytes;
@@ -173,6 +171,14 @@ static void run_filter(const DbView& db, const Level& lv, bool dense, const void
     fa.xnh_inf = db.snh_inf;
     fa.stride = 1;
   }
+}
+
+static void run_filter(const DbView& db, const Level& lv, bool dense, const void* qenc,
+                       const float* qscale, const float* tau, int nq, const int* nq_dev, int cap,
+                       int* counts, uint64_t* cand, int64_t target_wgs, int variant,
+                       int64_t skip_tiles, int64_t row_begin, hipStream_t st) {
+  FilterArgs fa{};
+  filter_source(fa, db, lv);
   fa.dense = dense && db.packed && variant != kFilterDirect && lv.m <= cap ? 1 : 0;
   fa.q = qenc;
   fa.qscale = qscale;
@@ -357,20 +363,154 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
   launch_collect_failed(status, nq, nq_dev, flist, &ctr->failed, st);
 }
 
+// The same batch as enqueue_batch, as argument records for the persistent
+// kernel (persist.hip) instead of launches: the same levels, thresholds,
+// capacities, selects and re-rank. Its filter scores every row of a level
+// (no carry between levels, like the direct variant), which keeps exactly the
+// rows the carrying variants keep, so the selected candidates and the results
+// are the same. Filter units are 32 queries x rows_per_slice rows (a multiple
+// of 128: the four waves of a unit take 32-row tiles in turn), about
+// target_units per level.
+void plan_persistent_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
+                           int32_t* out_i, float* out_d, Workspace& ws, int64_t target_units,
+                           PersistBatchPlan& out) {
+  RPLSH_REQUIRE(db.packed && db.elem != Elem::F32, "persistent kernel: packed BF16 / FP8 copies only");
+  RPLSH_REQUIRE(nq > 0, "persistent kernel: empty batch");
+  const int d = db.d;
+  const int kp = (int)std::min<int64_t>(p.kp, db.n);
+  RPLSH_REQUIRE(kp <= kPersistMaxKp, "persistent kernel: k' %d above %d", kp, kPersistMaxKp);
+  const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
+  const int64_t cap64 = level_cap(L, kp, p.c_factor);
+  RPLSH_REQUIRE(cap64 < (1ll << 31), "candidate capacity overflow");
+  const int cap = (int)cap64;
+  const int eb = db.elem == Elem::BF16 ? 2 : 1;
+
+  float* qn = ws.qnorm.as<float>(nq);
+  void* qenc = ws.qenc.ensure((size_t)nq * db.dpad * eb);
+  float* qscale = db.elem == Elem::FP8 ? ws.qscale.as<float>(nq) : nullptr;
+  float* tau = ws.tau.as<float>(nq);
+  float* tau2 = ws.tau2.as<float>(nq);
+  int* counts = ws.counts.as<int>(nq);
+  uint64_t* cand = ws.cand.as<uint64_t>((size_t)nq * cap);
+  uint64_t* topk = ws.topk.as<uint64_t>((size_t)nq * kp);
+  float* bound = ws.bound.as<float>(nq);
+  int* status = ws.status.as<int>(nq);
+  float* relk = ws.relk.as<float>(nq);
+  int* flist = ws.flist.as<int>(nq);
+  BatchCounters* ctr = ws.ctr.as<BatchCounters>(1);
+
+  PrepArgs& pr = out.prep;
+  pr = PrepArgs{};
+  pr.q = q;
+  pr.nq = nq;
+  pr.d = d;
+  pr.dpad = db.dpad;
+  pr.fp8 = db.elem == Elem::FP8 ? 1 : 0;
+  pr.qnorm = qn;
+  pr.qenc = qenc;
+  pr.qscale = qscale;
+  pr.tau = tau;
+  pr.status = status;
+  pr.counts = counts;
+  pr.ctr = reinterpret_cast<int*>(ctr);
+
+  out.filter.clear();
+  out.select.clear();
+  out.count_reset.clear();
+  for (size_t li = 0; li < L.size(); ++li) {
+    const Level& lv = L[li];
+    FilterArgs fa{};
+    filter_source(fa, db, lv);
+    fa.dense = li == 0 && lv.m <= cap ? 1 : 0;
+    if (li == 0) pr.count_init = fa.dense ? (int)lv.m : 0;
+    fa.q = qenc;
+    fa.qscale = qscale;
+    fa.tau = tau;
+    fa.nq = nq;
+    fa.num_qtiles = (int)div_up(nq, 32);
+    int64_t slices = std::max<int64_t>(1, div_up(target_units, fa.num_qtiles));
+    slices = std::min<int64_t>(slices, div_up(lv.m, 128));
+    fa.rows_per_slice = round_up(div_up(lv.m, slices), 128);
+    fa.num_slices = (int)div_up(lv.m, fa.rows_per_slice);
+    fa.cap = cap;
+    fa.counts = counts;
+    fa.cand = cand;
+    out.filter.push_back(fa);
+
+    SelectArgs sa{};
+    sa.cand = cand;
+    sa.counts = counts;
+    sa.cap = cap;
+    sa.nq = nq;
+    sa.tau_in = tau;
+    if (lv.rank_next > 0) {
+      sa.K = lv.rank_next;
+      sa.topk_mode = 0;
+      sa.tau_out = tau2;
+      sa.status = nullptr;
+      std::swap(tau, tau2);
+      out.count_reset.push_back(0);  // the next level appends from zero
+    } else {
+      sa.K = kp;
+      sa.topk_mode = 1;
+      sa.topk_out = topk;
+      sa.bound_out = bound;
+      sa.status = status;
+      out.count_reset.push_back(-1);  // the survivor total reads them
+    }
+    out.select.push_back(sa);
+  }
+
+  RerankTailArgs& rt = out.rerank;
+  rt = RerankTailArgs{};
+  RerankArgs& ra = rt.ra;
+  ra.x = db.x;
+  ra.ldx = db.ldx;
+  ra.xnorm = db.xnorm;
+  ra.d = d;
+  ra.q = q;
+  ra.ldq = d;
+  ra.qnorm = qn;
+  ra.topk = topk;
+  ra.kp = kp;
+  ra.k = p.k;
+  ra.nq = nq;
+  ra.exact = p.exact;
+  ra.bound = p.certify ? bound : nullptr;
+  ra.eps_coeff = 4.0f * (float)(d + 4) * kUnitRoundoff;
+  ra.band_coeff = 2.0f * (float)(d + 2) * kUnitRoundoff;
+  ra.xn_max = db.xn_max;
+  ra.fp64_boundary = p.fp64_boundary;
+  ra.out_idx = out_i;
+  ra.out_dist = out_d;
+  ra.status = status;
+  ra.relk_out = relk;
+  ra.fp64_count = &ctr->fp64;
+  rt.counts = counts;
+  rt.flist = flist;
+  rt.failed = &ctr->failed;
+  rt.survivors = &ctr->survivors;
+}
+
 // Host side after enqueue_batch: waits for st, reads the counters and re-runs
 // the failed queries (eagerly, with wider margins) into out_i / out_d.
 void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& p,
                   int32_t* out_i, float* out_d, std::vector<Workspace>& wss, int depth,
-                  hipStream_t st, rplsh_search_stats& stats, const PhaseEvents* pe) {
+                  hipStream_t st, rplsh_search_stats& stats, const PhaseEvents* pe,
+                  const BatchCounters* ready) {
   Workspace& ws = wss.at(depth);
   const int d = db.d;
   const int kp = (int)std::min<int64_t>(p.kp, db.n);
   // One read-back of the batch's counters into the workspace's pinned buffer
   // (or a stack copy when the workspace must not allocate).
   BatchCounters stack_ctr{};
-  BatchCounters* hc = ws.no_alloc ? &stack_ctr : ws.hctr.as<BatchCounters>(1);
-  RPLSH_HIP(hipMemcpyAsync(hc, ws.ctr.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
-  RPLSH_HIP(hipStreamSynchronize(st));
+  const BatchCounters* hc = ready;
+  if (!hc) {
+    BatchCounters* t = ws.no_alloc ? &stack_ctr : ws.hctr.as<BatchCounters>(1);
+    RPLSH_HIP(hipMemcpyAsync(t, ws.ctr.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
+    RPLSH_HIP(hipStreamSynchronize(st));
+    hc = t;
+  }
   const int nfail = hc->failed, nfp64 = hc->fp64;
   const long long nsurv = hc->survivors;
   stats.boundary_fp64 += nfp64;

@@ -124,6 +124,8 @@ struct Slot {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // device time of a batch
   std::vector<std::unique_ptr<GraphEntry>> graphs;  // touched by this slot's worker only
   size_t graph_bytes = 0;
+  PersistState* pst = nullptr;                      // persistent kernel program buffers
+  std::vector<std::vector<Workspace>> pws;          // workspaces per batch of a persistent launch
   std::thread th;
 };
 
@@ -154,6 +156,7 @@ struct rplsh_server {
   bool have_first = false;
   Clock::time_point first_submit, last_done;
   int64_t graph_launches = 0, graph_captures = 0;
+  int64_t persistent_launches = 0, persistent_batches = 0, persistent_aborts = 0;
   uint64_t use_clock = 0;
   // Batches seen per graph key: a graph is captured on a key's second batch,
   // so one-off shapes never pay for a capture.
@@ -167,6 +170,9 @@ namespace {
 // with 4x the survivor margin and k'; more failures than this still run, on
 // memory the budget did not see).
 constexpr int64_t kRetryReserveQueries = 64;
+
+// Batches one persistent launch takes at most.
+constexpr int kMaxPersistBatches = 16;
 
 size_t retry_bytes(const DbView& v, int64_t nq, const SearchParams& p) {
   SearchParams p2 = p;
@@ -231,6 +237,17 @@ struct Plan {
   size_t need = 0;          // bytes reserved while the batch runs
   size_t graph_part = 0;    // of need: the graph's workspace, moved to the cache once captured
   std::unique_ptr<GraphEntry> temp_graph;  // captured but not cached (no room in the budget)
+  // Persistent kernel: the head batch plus `extra` batches of other (index,
+  // k) pairs of the same element kind and padded dimension, in one launch.
+  bool persistent = false;
+  struct Extra {
+    rplsh_bf* b = nullptr;
+    int index = 0, k = 0;
+    int64_t total = 0;
+    SearchParams p;
+    std::vector<Request*> reqs;
+  };
+  std::vector<Extra> extra;
 };
 
 GraphEntry* find_graph(Slot& sl, const GraphKey& key) {
@@ -400,6 +417,95 @@ int run_batch(rplsh_server* s, Slot& sl, Plan& pl, const std::vector<Request*>& 
   }
 }
 
+// Runs a persistent-kernel group on a slot (called without the lock): every
+// batch's queries in one host-to-device copy, one launch for all batches,
+// the rare failed queries re-run eagerly per batch, results back in one
+// device-to-host copy per array. If the kernel gave up (its bounded waits),
+// the batches are recomputed on the launched pipeline.
+int run_persistent(rplsh_server* s, Slot& sl, Plan& pl, const std::vector<Request*>& head,
+                   std::string& err, float& device_ms, int& aborted) {
+  aborted = 0;
+  try {
+    struct B {
+      rplsh_bf* b;
+      int k;
+      int64_t nq;
+      SearchParams p;
+      const std::vector<Request*>* reqs;
+      DbView v;
+      size_t qoff, ooff;
+    };
+    std::vector<B> bs;
+    bs.push_back({pl.b, pl.k, pl.total, pl.p, &head, bf_db_view(pl.b), 0, 0});
+    for (const auto& e : pl.extra) bs.push_back({e.b, e.k, e.total, e.p, &e.reqs, bf_db_view(e.b), 0, 0});
+    size_t qn = 0, on = 0;
+    for (auto& x : bs) {
+      x.qoff = qn;
+      x.ooff = on;
+      qn += (size_t)x.nq * x.v.d;
+      on += (size_t)x.nq * x.k;
+    }
+    float* q = sl.qbuf.as<float>(qn);
+    int32_t* I = sl.ibuf.as<int32_t>(on);
+    float* D = sl.dbuf.as<float>(on);
+    float* hq = sl.hq.as<float>(qn);
+    int32_t* hi = sl.hi.as<int32_t>(on);
+    float* hd = sl.hd.as<float>(on);
+    for (auto& x : bs) {
+      size_t off = x.qoff;
+      for (auto* r : *x.reqs) {
+        std::memcpy(hq + off, r->q, (size_t)r->nq * x.v.d * sizeof(float));
+        off += (size_t)r->nq * x.v.d;
+      }
+    }
+    if (sl.pws.size() < bs.size()) sl.pws.resize(bs.size());
+    std::vector<PersistBatch> list;
+    for (size_t i = 0; i < bs.size(); ++i) {
+      if (sl.pws[i].size() < 4) sl.pws[i].resize(4);
+      const B& x = bs[i];
+      list.push_back({&x.v, q + x.qoff, (int)x.nq, &x.p, I + x.ooff, D + x.ooff, &sl.pws[i][0]});
+    }
+    RPLSH_HIP(hipEventRecord(sl.ev0, sl.st));
+    RPLSH_HIP(hipMemcpyAsync(q, hq, qn * sizeof(float), hipMemcpyHostToDevice, sl.st));
+    const bool ok = persist_run(*sl.pst, list, sl.st);
+    rplsh_search_stats stats{};
+    for (size_t i = 0; i < bs.size(); ++i) {
+      const B& x = bs[i];
+      if (ok) {
+        finish_batch(x.v, q + x.qoff, (int)x.nq, x.p, I + x.ooff, D + x.ooff, sl.pws[i], 0, sl.st, stats,
+                     nullptr, static_cast<const BatchCounters*>(sl.pws[i][0].hctr.p));
+      } else {
+        search_batch(x.v, q + x.qoff, (int)x.nq, x.p, I + x.ooff, D + x.ooff, sl.wss, 0, sl.st, stats);
+      }
+    }
+    aborted = ok ? 0 : 1;
+    RPLSH_HIP(hipMemcpyAsync(hi, I, on * sizeof(int32_t), hipMemcpyDeviceToHost, sl.st));
+    RPLSH_HIP(hipMemcpyAsync(hd, D, on * sizeof(float), hipMemcpyDeviceToHost, sl.st));
+    RPLSH_HIP(hipEventRecord(sl.ev1, sl.st));
+    RPLSH_HIP(hipStreamSynchronize(sl.st));
+    RPLSH_HIP(hipEventElapsedTime(&device_ms, sl.ev0, sl.ev1));
+    for (auto& x : bs) {
+      size_t off = x.ooff;
+      for (auto* r : *x.reqs) {
+        std::memcpy(r->I, hi + off, (size_t)r->nq * x.k * sizeof(int32_t));
+        std::memcpy(r->D, hd + off, (size_t)r->nq * x.k * sizeof(float));
+        off += (size_t)r->nq * x.k;
+      }
+    }
+    (void)s;
+    return RPLSH_OK;
+  } catch (const Error& e) {
+    err = e.what();
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    err = "host allocation failed";
+    return RPLSH_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    err = e.what();
+    return RPLSH_ERR_STATE;
+  }
+}
+
 // Completes a request that can never be admitted (called with the lock held).
 void reject(rplsh_server* s, Request* r, int status, const std::string& why) {
   r->t_admit = Clock::now();
@@ -482,7 +588,60 @@ void worker(rplsh_server* s, Slot* sl) {
       try {
         pl.p = bf_search_params(b, pl.k, total);
         const DbView v = bf_db_view(b);
-        if (s->cfg.use_graphs && v.n > 0 && total <= s->cfg.graph_max_queries) {
+        if (s->cfg.persistent_max_queries > 0 && total <= s->cfg.persistent_max_queries &&
+            persist_supported(v, pl.p)) {
+          // Persistent kernel: add queued batches of other (index, k) pairs of
+          // the same element kind and padded dimension, each within the
+          // per-batch query limit, while the group's workspaces fit.
+          pl.persistent = true;
+          size_t group_need = need;
+          int64_t group_q = total;
+          const int64_t max_group_q = 4 * (int64_t)s->cfg.persistent_max_queries;
+          for (auto it = s->queue.begin(); it != s->queue.end();) {
+            Request* r = *it;
+            bool taken = false;
+            if (group_q + r->nq <= max_group_q && r->nq <= s->cfg.persistent_max_queries) {
+              const DbView rv = bf_db_view(r->b);
+              if (rv.elem == v.elem && rv.dpad == v.dpad && rv.n > 0) {
+                Plan::Extra* e = nullptr;
+                for (auto& x : pl.extra)
+                  if (x.index == r->index && x.k == r->k) e = &x;
+                const bool fresh = e == nullptr;
+                if (fresh && pl.extra.size() + 1 >= (size_t)kMaxPersistBatches) {
+                  ++it;
+                  continue;
+                }
+                const int64_t nt = (fresh ? 0 : e->total) + r->nq;
+                try {
+                  const SearchParams rp = bf_search_params(r->b, r->k, nt);
+                  if (nt <= s->cfg.persistent_max_queries && persist_supported(rv, rp)) {
+                    const size_t before = fresh ? 0 : batch_bytes(r->b, e->total, r->k);
+                    const size_t after = batch_bytes(r->b, nt, r->k);
+                    if (group_need - before + after <= avail) {
+                      if (fresh) {
+                        pl.extra.emplace_back();
+                        e = &pl.extra.back();
+                        e->b = r->b;
+                        e->index = r->index;
+                        e->k = r->k;
+                      }
+                      e->total = nt;
+                      e->p = rp;
+                      e->reqs.push_back(r);
+                      group_need = group_need - before + after;
+                      group_q += r->nq;
+                      taken = true;
+                    }
+                  }
+                } catch (...) {
+                }
+              }
+            }
+            if (taken) it = s->queue.erase(it);
+            else ++it;
+          }
+          pl.need = group_need;
+        } else if (s->cfg.use_graphs && v.n > 0 && total <= s->cfg.graph_max_queries) {
           pl.key = make_key(pl.index, b, pl.k, bucket_of(total), pl.p);
           pl.g = find_graph(*sl, pl.key);
           if (pl.g) {
@@ -504,9 +663,15 @@ void worker(rplsh_server* s, Slot* sl) {
         pl.capture = false;
         pl.need = need;
         pl.graph_part = 0;
+        for (auto eit = pl.extra.rbegin(); eit != pl.extra.rend(); ++eit)
+          for (auto rit = eit->reqs.rbegin(); rit != eit->reqs.rend(); ++rit) s->queue.push_front(*rit);
+        pl.extra.clear();
+        pl.persistent = false;
       }
       if (!memory_try_charge(s->mem, pl.need)) {
         // Taken meanwhile by a reservation on a shared budget: requeue.
+        for (auto eit = pl.extra.rbegin(); eit != pl.extra.rend(); ++eit)
+          for (auto rit = eit->reqs.rbegin(); rit != eit->reqs.rend(); ++rit) s->queue.push_front(*rit);
         for (auto rit = batch.rbegin(); rit != batch.rend(); ++rit) s->queue.push_front(*rit);
         batch.clear();
         pl = Plan();
@@ -516,28 +681,50 @@ void worker(rplsh_server* s, Slot* sl) {
       break;
     }
     const auto now = Clock::now();
+    const int group = pl.persistent ? 1 + (int)pl.extra.size() : 0;
     for (auto* r : batch) {
       r->t_admit = now;
       r->info.batch_requests = (int)batch.size();
       r->info.batch_queries = pl.total;
+      r->info.kernel_batches = group;
     }
+    for (auto& e : pl.extra)
+      for (auto* r : e.reqs) {
+        r->t_admit = now;
+        r->info.batch_requests = (int)e.reqs.size();
+        r->info.batch_queries = e.total;
+        r->info.kernel_batches = group;
+      }
     s->inflight += 1;
-    s->batches += 1;
+    s->batches += 1 + (int64_t)pl.extra.size();
     s->peak_inflight = std::max(s->peak_inflight, s->inflight);
     const bool replay = pl.g != nullptr || pl.capture;
     if (replay) s->graph_launches += 1;
+    if (pl.persistent) {
+      s->persistent_launches += 1;
+      s->persistent_batches += group;
+    }
     lk.unlock();
     std::string err;
     float device_ms = 0.0f;
-    const int st = run_batch(s, *sl, pl, batch, err, device_ms);
+    int aborted = 0;
+    const int st = pl.persistent ? run_persistent(s, *sl, pl, batch, err, device_ms, aborted)
+                                 : run_batch(s, *sl, pl, batch, err, device_ms);
     lk.lock();
     memory_uncharge(s->mem, pl.need);
     s->inflight -= 1;
+    s->persistent_aborts += aborted;
     for (auto* r : batch) {
       r->err = err;
       r->info.device_ms = device_ms;
       finish(s, r, st);
     }
+    for (auto& e : pl.extra)
+      for (auto* r : e.reqs) {
+        r->err = err;
+        r->info.device_ms = device_ms;
+        finish(s, r, st);
+      }
     s->done_cv.notify_all();
     s->cv.notify_all();
   }
@@ -556,6 +743,9 @@ void destroy_slots(rplsh_server* s) {
     if (sl->st) (void)hipStreamSynchronize(sl->st);
     sl->graphs.clear();
     sl->wss.clear();
+    sl->pws.clear();
+    if (sl->pst) persist_state_destroy(sl->pst);
+    sl->pst = nullptr;
     sl->qbuf = DevBuf();
     sl->ibuf = DevBuf();
     sl->dbuf = DevBuf();
@@ -584,6 +774,7 @@ void rplsh_server_config_default(rplsh_server_config* cfg) {
   cfg->use_graphs = 1;
   cfg->graph_max_queries = 1024;
   cfg->graph_cache_bytes = (size_t)8 << 30;
+  cfg->persistent_max_queries = 256;
 }
 
 rplsh_status rplsh_server_create(const rplsh_server_config* cfg, rplsh_server** out) {
@@ -624,6 +815,7 @@ rplsh_status rplsh_server_create(const rplsh_server_config* cfg, rplsh_server** 
         RPLSH_HIP(hipEventCreate(&sl->ev0));
         RPLSH_HIP(hipEventCreate(&sl->ev1));
         sl->wss.resize(4);
+        sl->pst = persist_state_create();
         s->slots.push_back(std::move(sl));
       }
       for (auto& sl : s->slots) sl->th = std::thread(worker, s, sl.get());
@@ -747,6 +939,9 @@ rplsh_status rplsh_server_get_stats(rplsh_server* s, rplsh_server_stats* out) {
     out->graph_launches = s->graph_launches;
     out->graph_captures = s->graph_captures;
     out->graph_cache_bytes = s->graph_bytes;
+    out->persistent_launches = s->persistent_launches;
+    out->persistent_batches = s->persistent_batches;
+    out->persistent_aborts = s->persistent_aborts;
     if (!s->lat.empty()) {
       std::vector<double> v = s->lat;
       std::sort(v.begin(), v.end());
@@ -777,6 +972,7 @@ rplsh_status rplsh_server_reset_stats(rplsh_server* s) {
     s->peak_inflight = 0;
     s->have_first = false;
     s->graph_launches = s->graph_captures = 0;
+    s->persistent_launches = s->persistent_batches = s->persistent_aborts = 0;
     memory_reset_peak(s->mem);
   });
 }
