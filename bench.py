@@ -88,6 +88,8 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=4096, help="serving: number of requests")
     ap.add_argument("--indexes", type=int, default=64, help="serving: resident indexes")
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent batches")
+    ap.add_argument("--persistent-max-queries", type=int, default=256,
+                    help="serving: batches up to this many queries run in the persistent kernel (0: off)")
     ap.add_argument("--serving-precision", default="bf16", choices=["bf16", "fp8", "fp32"])
     args = ap.parse_args()
 
@@ -495,7 +497,7 @@ def serving(args, ca) -> int:
         k = int(rng.choice([1, 10, 100]))
         reqs.append((i, ca.gen_uniform(nq, dims[i], 900_000 + r, -1.0, 1.0), k))
     gaps = rng.exponential(1.0 / args.rate, size=args.requests)
-    srv = ca.Server(slots=args.slots)
+    srv = ca.Server(slots=args.slots, persistent_max_queries=args.persistent_max_queries)
     ids = [srv.add_index(ix) for ix in idxs]
     # warm-up: one small request per index (workspaces, code paths)
     for i in range(args.indexes):
@@ -534,7 +536,8 @@ def serving(args, ca) -> int:
         "data": "synthetic: seeded U[-1,1) indexes and queries",
         "config": {"workload": f"{args.indexes} indexes (n log-uniform 1e4..2e6, d in 64/96/128/256), "
                                f"{args.requests} requests, nq log-uniform 1..2048, k in 1/10/100",
-                   "rate_rps": args.rate, "slots": args.slots, "rows": total_rows},
+                   "rate_rps": args.rate, "slots": args.slots, "rows": total_rows,
+                   "persistent_max_queries": args.persistent_max_queries},
         "requests_per_s": round(len(reqs) / wall, 1),
         "queries": nqs,
         "p50_ms": round(st["p50_ms"], 3),
@@ -546,6 +549,8 @@ def serving(args, ca) -> int:
         "device_ms_p99": round(float(np.percentile([inf["device_ms"] for inf in infos], 99)), 3),
         "queue_ms_p50": round(float(np.percentile([inf["queue_ms"] for inf in infos], 50)), 3),
         "graph_launches": st["graph_launches"],
+        "persistent_runs": st["persistent_launches"],
+        "persistent_batches": st["persistent_batches"],
         "graph_captures": st["graph_captures"],
         "co_batched_requests": sum(1 for inf in infos if inf["batch_requests"] > 1),
         "peak_inflight": st["peak_inflight"],

@@ -148,7 +148,6 @@ class ServerStats(C.Structure):
         ("graph_cache_bytes", C.c_size_t),
         ("persistent_launches", C.c_int64),
         ("persistent_batches", C.c_int64),
-        ("persistent_aborts", C.c_int64),
     ]
 
 

@@ -173,16 +173,15 @@ struct RerankTailArgs {
 constexpr int kPersistMaxKp = 512;
 
 // A phase of one batch in a persistent program: `units` consecutive work
-// units from `begin`, each waiting for every unit of phase `dep` (-1: none).
+// units from `begin`.
 enum PersistKind : int { kPkPrep = 0, kPkFilter = 1, kPkSelect = 2, kPkRerank = 3 };
 struct PersistPhase {
   int kind;
   int begin;        // first unit
   int units;
-  int dep;          // phase index it waits for, or -1
   int arg;          // byte offset of its argument record in the program
   int count_reset;  // select phases: counts[q] after the select (-1: unchanged)
-  int pad[2];
+  int pad[3];
 };
 // Program layout: PersistHeader, PersistPhase[nphases], argument records
 // (16-byte aligned).

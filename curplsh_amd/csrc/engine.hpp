@@ -246,12 +246,11 @@ void plan_persistent_batch(const DbView& db, const float* q, int nq, const Searc
                            PersistBatchPlan& out);
 
 // Persistent serving kernel (persist.hip): whether it takes a batch, and
-// one launch over several batches of the same element kind and padded
-// dimension. Each batch's depth-0 results, status, failed list and counters
-// land where enqueue_batch puts them (finish_batch completes the batch).
-// Returns false if the kernel gave up (a dependency wait passed its bound):
-// the batches' outputs are then not valid and must be recomputed. On return
-// the stream is synchronised and each batch's counters are in ws->hctr.
+// one run over several batches of the same element kind and padded
+// dimension (one launch per step of the batches' phases). Each batch's
+// depth-0 results, status, failed list and counters land where enqueue_batch
+// puts them (finish_batch completes the batch). On return the stream is
+// synchronised and each batch's counters are in ws->hctr.
 bool persist_supported(const DbView& db, const SearchParams& p);
 struct PersistState;  // per-stream program and synchronisation buffers
 PersistState* persist_state_create();
@@ -265,7 +264,7 @@ struct PersistBatch {
   float* out_d;
   Workspace* ws;
 };
-bool persist_run(PersistState& state, const std::vector<PersistBatch>& batches, hipStream_t st);
+void persist_run(PersistState& state, const std::vector<PersistBatch>& batches, hipStream_t st);
 
 // Device bytes search_batch allocates for one batch (depth-0 workspace).
 size_t search_workspace_bytes(const DbView& db, int nq, const SearchParams& p);

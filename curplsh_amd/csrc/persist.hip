@@ -1,20 +1,17 @@
-// Persistent serving kernel: one launch runs whole searches -- query
-// preparation, every filter level with its select, and the exact re-rank --
-// for several batches at once, against indexes of different sizes (one
-// kernel per element kind and padded dimension).
+// Persistent serving kernel: whole searches -- query preparation, every
+// filter level with its select, and the exact re-rank -- for several batches
+// at once, against indexes of different sizes (one kernel per element kind
+// and padded dimension).
 //
 // The host writes a program: per batch, its phases in order (prep, filter and
 // select of each level, re-rank), each a run of work units with the argument
-// record of that phase. Resident workgroups take units from one device-wide
-// counter in program order; a unit first waits until every unit of the phase
-// it depends on (the batch's previous phase) has completed. Units are handed
-// out in order and a unit only waits for units handed out before it, which
-// are held by running workgroups, so the waits always end; a bounded spin
-// (kSpinLimit) still stops the launch (abort flag, every workgroup leaves)
-// rather than hang if that were ever violated, and the host then recomputes
-// the batches. Completion is published with an agent-scope release fence and
-// a relaxed counter add; waiters take an agent-scope acquire fence, so the
-// lists, thresholds and candidates written on other XCDs are visible.
+// record of that phase. Step s of a group is the s-th phase of every batch
+// that has one; the steps run as consecutive launches of the kernel on one
+// stream, so a step sees everything the previous one wrote and no workgroup
+// ever waits for another. Within a launch the resident workgroups take the
+// step's units from a device counter until none is left (batches of uneven
+// sizes share the workgroups), with a grid of at most what the device holds at
+// once.
 //
 // Units per phase: prep and select take four queries (a wave each); a filter
 // unit takes 32 queries x rows_per_slice rows, its four waves taking the
@@ -46,7 +43,6 @@ namespace rplsh {
 namespace {
 
 constexpr int kPBlock = 256;
-constexpr unsigned kSpinLimit = 1u << 26;  // ~4 s of s_sleep(2) before giving up
 
 // Bytes of dynamic LDS a launch carries: the larger of the selects' (four
 // waves x (256-bin histogram + kWaveSelK words)) and the re-rank's region at
@@ -203,34 +199,26 @@ __device__ void rerank_unit(const RerankTailArgs& t, int q, uint64_t* lds, float
 
 // ---- the kernel ----------------------------------------------------------
 
-// sync: [0] unit counter, [1] abort flag, [2 + i] completed units of phase i
-// (zeroed before the launch).
+// One step: units [unit_begin, unit_end) of phases [phase_begin, phase_end),
+// taken from *claim (zeroed before the launch).
 template <Elem E, int DPAD>
-__global__ __launch_bounds__(kPBlock, 2) void persistent_search_kernel(const char* __restrict__ prog,
-                                                                       int* __restrict__ sync) {
+__global__ __launch_bounds__(kPBlock, 2) void persistent_step_kernel(const char* __restrict__ prog,
+                                                                     int phase_begin, int phase_end,
+                                                                     int unit_begin, int unit_end,
+                                                                     int* __restrict__ claim) {
   extern __shared__ __attribute__((aligned(16))) uint64_t plds[];
   __shared__ float sq_fixed[256];
   __shared__ float sxn[kSelWaves * 64];
   __shared__ int sh[4];
-  __shared__ int s_unit, s_go;
-  const PersistHeader* hd = reinterpret_cast<const PersistHeader*>(prog);
+  __shared__ int s_unit;
   const PersistPhase* ph = reinterpret_cast<const PersistPhase*>(prog + sizeof(PersistHeader));
-  const int total = hd->total_units;
-  const int nph = hd->nphases;
-  int* claim = sync;
-  int* abort = sync + 1;
-  int* done = sync + 2;
   for (;;) {
-    if (threadIdx.x == 0) {
-      int u = atomicAdd(claim, 1);
-      if (u < total && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) u = total;
-      s_unit = u;
-    }
+    if (threadIdx.x == 0) s_unit = unit_begin + atomicAdd(claim, 1);
     __syncthreads();
     const int u = __builtin_amdgcn_readfirstlane(s_unit);
-    if (u >= total) break;
+    if (u >= unit_end) break;
     // The phase holding unit u (phases are sorted by their first unit).
-    int lo = 0, hi = nph - 1;
+    int lo = phase_begin, hi = phase_end - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (ph[mid].begin <= u) lo = mid;
@@ -238,29 +226,6 @@ __global__ __launch_bounds__(kPBlock, 2) void persistent_search_kernel(const cha
     }
     lo = __builtin_amdgcn_readfirstlane(lo);
     const PersistPhase P = ph[lo];
-    if (P.dep >= 0) {
-      if (threadIdx.x == 0) {
-        const int need = ph[P.dep].units;
-        int ok = 1;
-        unsigned spins = 0;
-        while (__hip_atomic_load(done + P.dep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-          if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            ok = 0;
-            break;
-          }
-          if (++spins > kSpinLimit) {
-            __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ok = 0;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(2);
-        }
-        s_go = ok;
-      }
-      __syncthreads();
-      if (!s_go) break;
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
     const int local = u - P.begin;
     const char* arg = prog + P.arg;
     switch (P.kind) {
@@ -285,18 +250,15 @@ __global__ __launch_bounds__(kPBlock, 2) void persistent_search_kernel(const cha
         break;
       }
     }
-    // Publish this unit's writes, then count it.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(done + lo, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();  // s_unit is rewritten by the next claim
   }
 }
 
-typedef void (*PersistKernel)(const char*, int*);
+typedef void (*PersistKernel)(const char*, int, int, int, int, int*);
 
 template <Elem E, int DPAD>
 PersistKernel kernel_for() {
-  return persistent_search_kernel<E, DPAD>;
+  return persistent_step_kernel<E, DPAD>;
 }
 
 PersistKernel persist_kernel(Elem e, int dpad) {
@@ -326,10 +288,9 @@ PersistKernel persist_kernel(Elem e, int dpad) {
 }  // namespace
 
 struct PersistState {
-  DevBuf prog, sync;
-  HostBuf hprog;  // program staging
-  HostBuf hflag;  // abort flag read-back
-  int resident = 0;  // workgroups the device holds at once (per kernel)
+  DevBuf prog, sync;  // program; one unit counter per step
+  HostBuf hprog;      // program staging
+  int resident = 0;   // workgroups the device holds at once (per kernel)
 };
 
 PersistState* persist_state_create() { return new PersistState(); }
@@ -340,7 +301,7 @@ bool persist_supported(const DbView& db, const SearchParams& p) {
          persist_kernel(db.elem, db.dpad) != nullptr && std::min<int64_t>(p.kp, db.n) <= kPersistMaxKp;
 }
 
-bool persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipStream_t st) {
+void persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipStream_t st) {
   RPLSH_REQUIRE(!batches.empty(), "persistent kernel: no batches");
   const Elem e = batches[0].db->elem;
   const int dpad = batches[0].db->dpad;
@@ -358,8 +319,7 @@ bool persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipS
   const int64_t target_units = S.resident;
 
   // Plans, then the program: phases ordered by position (all batches' preps,
-  // then their first filters, ...), each depending on its batch's previous
-  // phase.
+  // then their first filters, ...); position s is step s.
   std::vector<PersistBatchPlan> plans(batches.size());
   size_t max_phases = 0;
   for (size_t b = 0; b < batches.size(); ++b) {
@@ -388,7 +348,7 @@ bool persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipS
   }
   std::vector<PersistPhase> phases;
   std::vector<std::pair<const void*, size_t>> args;
-  std::vector<int> last(batches.size(), -1);
+  std::vector<int> step_phase{0}, step_unit{0};  // boundaries of the steps
   int units = 0;
   for (size_t pos = 0; pos < max_phases; ++pos) {
     for (size_t b = 0; b < batches.size(); ++b) {
@@ -398,14 +358,14 @@ bool persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipS
       P.kind = r.kind;
       P.begin = units;
       P.units = r.units;
-      P.dep = last[b];
       P.count_reset = r.count_reset;
       P.arg = (int)args.size();  // index for now, byte offset below
       args.push_back({r.arg, r.bytes});
-      last[b] = (int)phases.size();
       units += r.units;
       phases.push_back(P);
     }
+    step_phase.push_back((int)phases.size());
+    step_unit.push_back(units);
   }
   size_t off = round_up(sizeof(PersistHeader) + phases.size() * sizeof(PersistPhase), 16);
   std::vector<size_t> arg_off(args.size());
@@ -425,22 +385,27 @@ bool persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipS
   }
   for (size_t i = 0; i < args.size(); ++i) std::memcpy(hp + arg_off[i], args[i].first, args[i].second);
 
+  const int nsteps = (int)max_phases;
   char* dprog = S.prog.as<char>(off);
-  int* sync = S.sync.as<int>(2 + phases.size());
+  int* claims = S.sync.as<int>(nsteps);
   RPLSH_HIP(hipMemcpyAsync(dprog, hp, off, hipMemcpyHostToDevice, st));
-  launch_fill_i32(sync, 2 + (int64_t)phases.size(), 0, st);
-  const int grid = (int)std::min<int64_t>(units, S.resident);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(kPBlock), lds, st, (const char*)dprog, sync);
-  RPLSH_HIP(hipGetLastError());
-  // One synchronisation for the abort flag and every batch's counters.
-  int* hflag = S.hflag.as<int>(1);
-  RPLSH_HIP(hipMemcpyAsync(hflag, sync + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  launch_fill_i32(claims, nsteps, 0, st);
+  for (int sidx = 0; sidx < nsteps; ++sidx) {
+    const int ub = step_unit[sidx], ue = step_unit[sidx + 1];
+    if (ue <= ub) continue;
+    const char* cprog = dprog;
+    int pb = step_phase[sidx], pe = step_phase[sidx + 1];
+    int* claim = claims + sidx;
+    void* kargs[] = {&cprog, &pb, &pe, const_cast<int*>(&ub), const_cast<int*>(&ue), &claim};
+    const int grid = std::min(ue - ub, S.resident);
+    RPLSH_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kern), dim3(grid), dim3(kPBlock), kargs, lds, st));
+  }
+  // One synchronisation for every batch's counters.
   for (const PersistBatch& pb : batches) {
     BatchCounters* hc = pb.ws->hctr.as<BatchCounters>(1);
     RPLSH_HIP(hipMemcpyAsync(hc, pb.ws->ctr.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
   }
   RPLSH_HIP(hipStreamSynchronize(st));
-  return *hflag == 0;
 }
 
 }  // namespace rplsh

@@ -156,7 +156,7 @@ struct rplsh_server {
   bool have_first = false;
   Clock::time_point first_submit, last_done;
   int64_t graph_launches = 0, graph_captures = 0;
-  int64_t persistent_launches = 0, persistent_batches = 0, persistent_aborts = 0;
+  int64_t persistent_launches = 0, persistent_batches = 0;
   uint64_t use_clock = 0;
   // Batches seen per graph key: a graph is captured on a key's second batch,
   // so one-off shapes never pay for a capture.
@@ -418,13 +418,11 @@ int run_batch(rplsh_server* s, Slot& sl, Plan& pl, const std::vector<Request*>& 
 }
 
 // Runs a persistent-kernel group on a slot (called without the lock): every
-// batch's queries in one host-to-device copy, one launch for all batches,
-// the rare failed queries re-run eagerly per batch, results back in one
-// device-to-host copy per array. If the kernel gave up (its bounded waits),
-// the batches are recomputed on the launched pipeline.
+// batch's queries in one host-to-device copy, the kernel's steps over all
+// batches, the rare failed queries re-run eagerly per batch, results back in
+// one device-to-host copy per array.
 int run_persistent(rplsh_server* s, Slot& sl, Plan& pl, const std::vector<Request*>& head,
-                   std::string& err, float& device_ms, int& aborted) {
-  aborted = 0;
+                   std::string& err, float& device_ms) {
   try {
     struct B {
       rplsh_bf* b;
@@ -467,18 +465,13 @@ int run_persistent(rplsh_server* s, Slot& sl, Plan& pl, const std::vector<Reques
     }
     RPLSH_HIP(hipEventRecord(sl.ev0, sl.st));
     RPLSH_HIP(hipMemcpyAsync(q, hq, qn * sizeof(float), hipMemcpyHostToDevice, sl.st));
-    const bool ok = persist_run(*sl.pst, list, sl.st);
+    persist_run(*sl.pst, list, sl.st);
     rplsh_search_stats stats{};
     for (size_t i = 0; i < bs.size(); ++i) {
       const B& x = bs[i];
-      if (ok) {
-        finish_batch(x.v, q + x.qoff, (int)x.nq, x.p, I + x.ooff, D + x.ooff, sl.pws[i], 0, sl.st, stats,
-                     nullptr, static_cast<const BatchCounters*>(sl.pws[i][0].hctr.p));
-      } else {
-        search_batch(x.v, q + x.qoff, (int)x.nq, x.p, I + x.ooff, D + x.ooff, sl.wss, 0, sl.st, stats);
-      }
+      finish_batch(x.v, q + x.qoff, (int)x.nq, x.p, I + x.ooff, D + x.ooff, sl.pws[i], 0, sl.st, stats,
+                   nullptr, static_cast<const BatchCounters*>(sl.pws[i][0].hctr.p));
     }
-    aborted = ok ? 0 : 1;
     RPLSH_HIP(hipMemcpyAsync(hi, I, on * sizeof(int32_t), hipMemcpyDeviceToHost, sl.st));
     RPLSH_HIP(hipMemcpyAsync(hd, D, on * sizeof(float), hipMemcpyDeviceToHost, sl.st));
     RPLSH_HIP(hipEventRecord(sl.ev1, sl.st));
@@ -707,13 +700,11 @@ void worker(rplsh_server* s, Slot* sl) {
     lk.unlock();
     std::string err;
     float device_ms = 0.0f;
-    int aborted = 0;
-    const int st = pl.persistent ? run_persistent(s, *sl, pl, batch, err, device_ms, aborted)
+    const int st = pl.persistent ? run_persistent(s, *sl, pl, batch, err, device_ms)
                                  : run_batch(s, *sl, pl, batch, err, device_ms);
     lk.lock();
     memory_uncharge(s->mem, pl.need);
     s->inflight -= 1;
-    s->persistent_aborts += aborted;
     for (auto* r : batch) {
       r->err = err;
       r->info.device_ms = device_ms;
@@ -941,7 +932,6 @@ rplsh_status rplsh_server_get_stats(rplsh_server* s, rplsh_server_stats* out) {
     out->graph_cache_bytes = s->graph_bytes;
     out->persistent_launches = s->persistent_launches;
     out->persistent_batches = s->persistent_batches;
-    out->persistent_aborts = s->persistent_aborts;
     if (!s->lat.empty()) {
       std::vector<double> v = s->lat;
       std::sort(v.begin(), v.end());
@@ -972,7 +962,7 @@ rplsh_status rplsh_server_reset_stats(rplsh_server* s) {
     s->peak_inflight = 0;
     s->have_first = false;
     s->graph_launches = s->graph_captures = 0;
-    s->persistent_launches = s->persistent_batches = s->persistent_aborts = 0;
+    s->persistent_launches = s->persistent_batches = 0;
     memory_reset_peak(s->mem);
   });
 }

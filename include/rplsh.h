@@ -453,8 +453,9 @@ typedef struct rplsh_server_config {
   int persistent_max_queries; /* BF16 / FP8 batches of up to this many queries (with
                               k' <= 512) run in the persistent kernel, which takes up to
                               16 queued batches of one element kind and padded
-                              dimension -- different indexes and k -- in one launch;
-                              0: off (default 256) */
+                              dimension -- different indexes and k -- in one run (a
+                              launch per step: prep, each filter level and its select,
+                              re-rank); 0: off (default 256) */
 } rplsh_server_config;
 
 typedef struct rplsh_request_info {
@@ -465,8 +466,8 @@ typedef struct rplsh_request_info {
   int64_t batch_queries;
   float device_ms;      /* device time of that batch: query upload, search and
                            result download (HIP events on the slot's stream) */
-  int kernel_batches;   /* batches that shared its persistent-kernel launch (0: it
-                           ran on the launched pipeline) */
+  int kernel_batches;   /* batches that shared its persistent-kernel run (0: it ran
+                           on the launched pipeline) */
 } rplsh_request_info;
 
 typedef struct rplsh_server_stats {
@@ -479,9 +480,8 @@ typedef struct rplsh_server_stats {
   int64_t graph_launches;                 /* batches run by replaying a captured graph */
   int64_t graph_captures;                 /* graphs captured */
   size_t graph_cache_bytes;               /* device bytes held by cached graphs */
-  int64_t persistent_launches;            /* persistent-kernel launches */
+  int64_t persistent_launches;            /* persistent-kernel runs (groups of batches) */
   int64_t persistent_batches;             /* batches they ran */
-  int64_t persistent_aborts;              /* launches that gave up (batches recomputed) */
 } rplsh_server_stats;
 
 void rplsh_server_config_default(rplsh_server_config* cfg);

@@ -102,7 +102,7 @@ def test_request_larger_than_budget_is_rejected(ca, indexes):
 def test_graph_replays_match_eager(ca, indexes):
     """The same (index, k, bucket) graph replayed many times, with eager
     searches and other graphs in between, always equals the eager result."""
-    srv = ca.Server(slots=2)
+    srv = ca.Server(slots=2, persistent_max_queries=0)
     ids = [srv.add_index(ix) for ix in indexes]
     q = ca.gen_uniform(37, 96, 77, -1, 1)
     Ia, Da = indexes[1].search(q, 10)
@@ -167,7 +167,7 @@ def test_graph_key_follows_tuning_entries(ca, res):
                 recall=1.0, ms_default=0.0, ms_tuned=0.0)
     idx.set_tuning(dict(base, batch_nq=16, kprime_factor=2, c_factor=2, rho=32))
     idx.set_tuning(dict(base, batch_nq=24, kprime_factor=8, c_factor=4, rho=16, variant=1))
-    srv = ca.Server(slots=1)
+    srv = ca.Server(slots=1, persistent_max_queries=0)
     sid = srv.add_index(idx)
     qs = [ca.gen_uniform(nq, 96, 620 + j, -1, 1) for j, nq in enumerate([17, 23, 18, 22, 17, 23])]
     for q in qs:  # buckets: all 32; tuning entries: nearest of 16 / 24
