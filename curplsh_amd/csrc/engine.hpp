@@ -249,8 +249,8 @@ void plan_persistent_batch(const DbView& db, const float* q, int nq, const Searc
 // one run over several batches of the same element kind and padded
 // dimension (one launch per step of the batches' phases). Each batch's
 // depth-0 results, status, failed list and counters land where enqueue_batch
-// puts them (finish_batch completes the batch). On return the stream is
-// synchronised and each batch's counters are in ws->hctr.
+// puts them (finish_batch completes the batch). Everything is enqueued on st;
+// once st is synchronised each batch's counters are in ws->hctr.
 bool persist_supported(const DbView& db, const SearchParams& p);
 struct PersistState;  // per-stream program and synchronisation buffers
 PersistState* persist_state_create();

@@ -127,13 +127,23 @@ __device__ void filter_unit(const FilterArgs& a, int local) {
     const char* ap = xb + (p >> 5) * G::TILE + ((p & 31) + 32 * h) * LB;
     // C = -||x||^2 / 2 of rows t0 + 8g + 4h + i (-inf past the end).
     f32x16 ci;
+    if (stride == 1 && t0 + 32 <= r_end) {
+      // Whole tile at stride 1: four aligned 16-byte loads (t0 is a multiple of 32).
 #pragma unroll
-    for (int g = 0; g < 4; ++g)
+      for (int g = 0; g < 4; ++g) {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(a.xnorm + t0 + 8 * g + 4 * h);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int64_t r = t0 + 8 * g + 4 * h + i;
-        ci[4 * g + i] = r < r_end ? -0.5f * a.xnorm[r * stride] : -INFINITY;
+        for (int i = 0; i < 4; ++i) ci[4 * g + i] = -0.5f * v[i];
       }
+    } else {
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int64_t r = t0 + 8 * g + 4 * h + i;
+          ci[4 * g + i] = r < r_end ? -0.5f * a.xnorm[r * stride] : -INFINITY;
+        }
+    }
     const int xsc = (E == Elem::FP8 && valid) ? e8m0_of(a.xscale[p]) : 127;
     f32x16 acc;
 #pragma unroll
@@ -400,12 +410,11 @@ void persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipS
     const int grid = std::min(ue - ub, S.resident);
     RPLSH_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kern), dim3(grid), dim3(kPBlock), kargs, lds, st));
   }
-  // One synchronisation for every batch's counters.
+  // Every batch's counters come back with the caller's next synchronisation.
   for (const PersistBatch& pb : batches) {
     BatchCounters* hc = pb.ws->hctr.as<BatchCounters>(1);
     RPLSH_HIP(hipMemcpyAsync(hc, pb.ws->ctr.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
   }
-  RPLSH_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace rplsh

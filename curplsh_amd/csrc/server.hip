@@ -466,16 +466,29 @@ int run_persistent(rplsh_server* s, Slot& sl, Plan& pl, const std::vector<Reques
     RPLSH_HIP(hipEventRecord(sl.ev0, sl.st));
     RPLSH_HIP(hipMemcpyAsync(q, hq, qn * sizeof(float), hipMemcpyHostToDevice, sl.st));
     persist_run(*sl.pst, list, sl.st);
-    rplsh_search_stats stats{};
-    for (size_t i = 0; i < bs.size(); ++i) {
-      const B& x = bs[i];
-      finish_batch(x.v, q + x.qoff, (int)x.nq, x.p, I + x.ooff, D + x.ooff, sl.pws[i], 0, sl.st, stats,
-                   nullptr, static_cast<const BatchCounters*>(sl.pws[i][0].hctr.p));
-    }
+    // Results and counters in one synchronisation; the rare batch with failed
+    // queries re-runs them (finish_batch) and copies its results again.
     RPLSH_HIP(hipMemcpyAsync(hi, I, on * sizeof(int32_t), hipMemcpyDeviceToHost, sl.st));
     RPLSH_HIP(hipMemcpyAsync(hd, D, on * sizeof(float), hipMemcpyDeviceToHost, sl.st));
     RPLSH_HIP(hipEventRecord(sl.ev1, sl.st));
     RPLSH_HIP(hipStreamSynchronize(sl.st));
+    rplsh_search_stats stats{};
+    bool redo = false;
+    for (size_t i = 0; i < bs.size(); ++i) {
+      const B& x = bs[i];
+      const BatchCounters* hc = static_cast<const BatchCounters*>(sl.pws[i][0].hctr.p);
+      if (hc->failed == 0) continue;
+      finish_batch(x.v, q + x.qoff, (int)x.nq, x.p, I + x.ooff, D + x.ooff, sl.pws[i], 0, sl.st, stats,
+                   nullptr, hc);
+      const size_t cnt = (size_t)x.nq * x.k;
+      RPLSH_HIP(hipMemcpyAsync(hi + x.ooff, I + x.ooff, cnt * sizeof(int32_t), hipMemcpyDeviceToHost, sl.st));
+      RPLSH_HIP(hipMemcpyAsync(hd + x.ooff, D + x.ooff, cnt * sizeof(float), hipMemcpyDeviceToHost, sl.st));
+      redo = true;
+    }
+    if (redo) {
+      RPLSH_HIP(hipEventRecord(sl.ev1, sl.st));
+      RPLSH_HIP(hipStreamSynchronize(sl.st));
+    }
     RPLSH_HIP(hipEventElapsedTime(&device_ms, sl.ev0, sl.ev1));
     for (auto& x : bs) {
       size_t off = x.ooff;

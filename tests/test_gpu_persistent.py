@@ -79,3 +79,28 @@ def test_persistent_groups_batches(ca, pindexes):
     assert grouped >= 2
     assert st["persistent_batches"] > st["persistent_launches"]
     srv.close()
+
+
+def test_persistent_retry_path(ca, pindexes):
+    """A tuning entry with a survivor multiple of 1 (about k' rows kept per
+    query) makes many queries keep fewer than k' rows: the persistent run's
+    failed queries are re-run on the launched pipeline with wider margins,
+    and the results still equal IndexBF.search's (which retries the same
+    way)."""
+    idx = pindexes[2]  # 150,000 x 128 BF16
+    idx.set_tuning(dict(k=100, batch_nq=64, kprime_factor=4, target_wgs=4096, c_factor=1, variant=0,
+                        rho=8, recall=1.0, ms_default=0.0, ms_tuned=0.0))
+    try:
+        q = ca.gen_uniform(64, 128, 777, -1, 1)
+        Ia, Da = idx.search(q, 100)
+        retried = idx.last_stats()["retried_queries"]
+        print("retried alone:", retried)
+        assert retried > 0
+        srv = ca.Server(slots=1, use_graphs=False)
+        sid = srv.add_index(idx)
+        I, D, info = srv.wait(srv.submit(sid, q, 100), timeout_s=60.0)
+        assert info["kernel_batches"] == 1
+        assert np.array_equal(I, Ia) and np.array_equal(D.view(np.uint32), Da.view(np.uint32))
+        srv.close()
+    finally:
+        idx.clear_tuning()
