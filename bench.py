@@ -71,11 +71,17 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (grid, survivor target, filter variant; k' stays 4k)")
-    ap.add_argument("--mode", default="headline", choices=["headline", "serving", "copy", "pmc-pass", "rplsh"],
+    ap.add_argument("--mode", default="headline",
+                    choices=["headline", "serving", "copy", "pmc-pass", "rplsh", "smallbatch"],
                     help="headline: the metric's workload; serving: the 64-index request loop; "
                          "copy: device-to-device copies of --copy-gib GiB (HBM bandwidth and the "
                          "reference for scaling PMC byte counters); pmc-pass: one search of the "
-                         "headline form (what the traffic passes profile)")
+                         "headline form (what the traffic passes profile); smallbatch: nq in "
+                         "--sb-nq against 2M-row indexes, the database stream's GB/s (HBM-bound regime)")
+    ap.add_argument("--sb-shapes", default="2000000x256,2000000x64", help="smallbatch: NxD,... indexes")
+    ap.add_argument("--sb-nq", default="1,8,64,512", help="smallbatch: batch sizes")
+    ap.add_argument("--sb-forms", default="bf16,fp32", help="smallbatch: precisions")
+    ap.add_argument("--sb-steps", type=int, default=30, help="smallbatch: timed searches per point")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 --pmc passes that measure the filter's HBM traffic")
     ap.add_argument("--tuning", default="", help="pmc-pass: JSON tuning entry to install")
@@ -111,6 +117,8 @@ def main() -> int:
         return pmc_pass(args, ca)
     if args.mode == "rplsh":
         return rplsh_front_end(args, ca)
+    if args.mode == "smallbatch":
+        return small_batch(args, ca)
     n, nq, d, k = args.n, args.nq, args.d, args.k
     log(f"[bench] generating {n}x{d} database and {nq} queries")
     t0 = time.time()
@@ -410,6 +418,71 @@ def pmc_traffic(args, tuning) -> dict:
             "fetch_kib_raw": got[("FETCH_SIZE", "filter")], "write_kib_raw": got[("WRITE_SIZE", "filter")],
             "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
             "calibration": "hipMemcpy device-to-device of 1 GiB: scale = 1 GiB / counter bytes"}
+
+
+def small_batch(args, ca) -> int:
+    """The HBM-bound regime: a few queries against a 2M-row index, where the
+    full-level filter's time is the database stream. For each index shape,
+    precision and batch size: median device time of IndexBF.search, of its
+    full-level filter launch (HIP events on the search stream), and the bytes
+    that launch must read -- the candidate copy of the rows it scores plus
+    their norms (BF16: 32-row packed tiles of dpad elements; FP32: the
+    row-major rows) -- as GB/s against the 8 TB/s HBM peak."""
+    res = ca.DeviceResources(0)
+    shapes = [tuple(int(v) for v in sh.split("x")) for sh in args.sb_shapes.split(",") if sh]
+    nqs = [int(v) for v in args.sb_nq.split(",") if v]
+    forms = [f for f in args.sb_forms.split(",") if f]
+    k = 10
+    for n, d in shapes:
+        log(f"[smallbatch] {n}x{d} U[-1,1)")
+        x = ca.gen_uniform(n, d, 9000 + d, -1.0, 1.0)
+        idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=forms[0]))
+        idx.add(x)
+        del x
+        points = []
+        for form in forms:
+            idx.set_precision(form, 4)
+            elem = {"bf16": 2, "fp8": 1, "fp32": 4}[form]
+            if form == "bf16":
+                dpad = next(s_ for s_ in (32, 64, 96, 128, 160, 192, 256) if d <= s_)
+            elif form == "fp8":
+                dpad = next(s_ for s_ in (64, 128, 256) if d <= s_)
+            else:
+                dpad = -(-d // 4) * 4  # the row-major FP32 rows, padded to 16 bytes
+            for nq in nqs:
+                q = ca.gen_uniform(nq, d, 9100 + nq, -1.0, 1.0)
+                dq = ca.DeviceArray.from_host(q)
+                dI = ca.DeviceArray((nq, k), np.int32)
+                dD = ca.DeviceArray((nq, k), np.float32)
+                for _ in range(3):
+                    idx.search(dq, k, dI, dD)
+                tot, full, rows, wall = [], [], 0, []
+                for _ in range(args.sb_steps):
+                    t0 = time.perf_counter()
+                    idx.search(dq, k, dI, dD)
+                    wall.append((time.perf_counter() - t0) * 1000.0)
+                    st = idx.last_stats()
+                    tot.append(st["ms_total"])
+                    full.append(st["ms_filter_full"])
+                    rows = int(st["rows_filter_full"])
+                fms = float(np.median(full))
+                bytes_full = rows * (dpad * elem + 4)
+                gbs = bytes_full / (fms / 1000.0) / 1e9
+                pt = {"precision": form, "nq": nq, "k": k, "ms_search": round(float(np.median(tot)), 4),
+                      "ms_wall": round(float(np.median(wall)), 4), "ms_filter_full": round(fms, 4),
+                      "rows_filter_full": rows, "bytes_filter_full": bytes_full,
+                      "filter_gbs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                      "qps": round(nq / (float(np.median(tot)) / 1000.0), 1)}
+                points.append(pt)
+                log(f"[smallbatch] {n}x{d} {form} nq={nq}: search {pt['ms_search']} ms, full filter "
+                    f"{fms:.4f} ms = {gbs:.0f} GB/s ({gbs / HBM_PEAK_GBS:.3f} of peak)")
+        print(json.dumps({"metric": "database stream GB/s of the full-level filter, small batches",
+                          "unit": "GB/s", "n_gpus": 1, "higher_is_better": True,
+                          "config": {"workload": f"{n}x{d} U[-1,1) index, k={k}", "n": n, "d": d},
+                          "roofline": {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s"},
+                          "points": points}), flush=True)
+        idx.close()
+    return 0
 
 
 def mfma_peak(kind: str, random_operands: bool) -> float:
