@@ -427,7 +427,9 @@ def small_batch(args, ca) -> int:
     full-level filter launch (HIP events on the search stream), and the bytes
     that launch must read -- the candidate copy of the rows it scores plus
     their norms (BF16: 32-row packed tiles of dpad elements; FP32: the
-    row-major rows) -- as GB/s against the 8 TB/s HBM peak."""
+    row-major rows) -- as GB/s against the 8 TB/s HBM peak. BF16/FP8 run both
+    paths: the launched pipeline and the persistent kernel (one launch per
+    step; its full level is one step, timed the same way)."""
     res = ca.DeviceResources(0)
     shapes = [tuple(int(v) for v in sh.split("x")) for sh in args.sb_shapes.split(",") if sh]
     nqs = [int(v) for v in args.sb_nq.split(",") if v]
@@ -449,7 +451,9 @@ def small_batch(args, ca) -> int:
                 dpad = next(s_ for s_ in (64, 128, 256) if d <= s_)
             else:
                 dpad = -(-d // 4) * 4  # the row-major FP32 rows, padded to 16 bytes
-            for nq in nqs:
+            paths = ["launched", "persistent"] if form != "fp32" else ["launched"]
+            for path, nq in [(pa, nq_) for pa in paths for nq_ in nqs]:
+                idx.set_persistent(max(nqs) if path == "persistent" else 0)
                 q = ca.gen_uniform(nq, d, 9100 + nq, -1.0, 1.0)
                 dq = ca.DeviceArray.from_host(q)
                 dI = ca.DeviceArray((nq, k), np.int32)
@@ -468,13 +472,15 @@ def small_batch(args, ca) -> int:
                 fms = float(np.median(full))
                 bytes_full = rows * (dpad * elem + 4)
                 gbs = bytes_full / (fms / 1000.0) / 1e9
-                pt = {"precision": form, "nq": nq, "k": k, "ms_search": round(float(np.median(tot)), 4),
+                assert bool(st["persistent"]) == (path == "persistent")
+                pt = {"precision": form, "path": path, "nq": nq, "k": k,
+                      "ms_search": round(float(np.median(tot)), 4),
                       "ms_wall": round(float(np.median(wall)), 4), "ms_filter_full": round(fms, 4),
                       "rows_filter_full": rows, "bytes_filter_full": bytes_full,
                       "filter_gbs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
                       "qps": round(nq / (float(np.median(tot)) / 1000.0), 1)}
                 points.append(pt)
-                log(f"[smallbatch] {n}x{d} {form} nq={nq}: search {pt['ms_search']} ms, full filter "
+                log(f"[smallbatch] {n}x{d} {form} {path} nq={nq}: search {pt['ms_search']} ms, full filter "
                     f"{fms:.4f} ms = {gbs:.0f} GB/s ({gbs / HBM_PEAK_GBS:.3f} of peak)")
         print(json.dumps({"metric": "database stream GB/s of the full-level filter, small batches",
                           "unit": "GB/s", "n_gpus": 1, "higher_is_better": True,

@@ -181,6 +181,7 @@ class IndexBFConfig:
     precision: str | int = "fp32"
     kprime_factor: int = 4
     fp64_boundary: bool = True
+    persistent_max_queries: int = 256
 
     def to_c(self) -> _lib.BfConfig:
         c = _lib.BfConfig()
@@ -191,6 +192,7 @@ class IndexBFConfig:
         c.precision = _PREC[self.precision]
         c.kprime_factor = self.kprime_factor
         c.fp64_boundary = int(self.fp64_boundary)
+        c.persistent_max_queries = int(self.persistent_max_queries)
         return c
 
 
@@ -406,6 +408,10 @@ class IndexBF:
         x, n = C.c_void_p(), C.c_void_p()
         call("rplsh_bf_data_dev", self._h, C.byref(x), C.byref(n))
         return x.value or 0, n.value or 0
+
+    def set_persistent(self, max_queries: int) -> None:
+        """Largest batch search() runs in the persistent kernel (0: never)."""
+        call("rplsh_bf_set_persistent", self._h, int(max_queries))
 
     def autotune(self, sample, k: int, batch_nq: int, target_recall: float = 0.999) -> dict:
         """rplsh_bf_autotune: tunes (k' factor, grid, survivor target) for k

@@ -49,6 +49,7 @@ class BfConfig(C.Structure):
         ("precision", C.c_int),
         ("kprime_factor", C.c_int),
         ("fp64_boundary", C.c_int),
+        ("persistent_max_queries", C.c_int),
     ]
 
 
@@ -69,6 +70,7 @@ class SearchStats(C.Structure):
         ("rows_filter_full", C.c_int64),
         ("filter_variant", C.c_int32),
         ("target_wgs", C.c_int32),
+        ("persistent", C.c_int32),
     ]
 
 
@@ -235,6 +237,7 @@ _SIGS = {
     "rplsh_bf_num_data": [_vp, C.POINTER(_i64)],
     "rplsh_bf_dimension": [_vp, C.POINTER(_i32)],
     "rplsh_bf_device": [_vp, C.POINTER(_i32)],
+    "rplsh_bf_set_persistent": [_vp, _i32],
     "rplsh_bf_set_precision": [_vp, _i32, _i32],
     "rplsh_bf_data_dev": [_vp, C.POINTER(_vp), C.POINTER(_vp)],
     "rplsh_bf_last_stats": [_vp, C.POINTER(SearchStats)],

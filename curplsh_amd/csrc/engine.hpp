@@ -264,7 +264,10 @@ struct PersistBatch {
   float* out_d;
   Workspace* ws;
 };
-void persist_run(PersistState& state, const std::vector<PersistBatch>& batches, hipStream_t st);
+// pe (single batch only): phase events as search_batch records them (sample
+// levels, full level, its select, re-rank).
+void persist_run(PersistState& state, const std::vector<PersistBatch>& batches, hipStream_t st,
+                 PhaseEvents* pe = nullptr);
 
 // Device bytes search_batch allocates for one batch (depth-0 workspace).
 size_t search_workspace_bytes(const DbView& db, int nq, const SearchParams& p);
@@ -326,6 +329,7 @@ struct rplsh_bf {
   rplsh::DevBuf qdev, idev, ddev;
   rplsh_search_stats last{};
   std::vector<rplsh_tune_result> tunes;  // autotuned parameters per (k, batch size)
+  rplsh::PersistState* pst = nullptr;     // persistent-kernel program buffers (small searches)
   // Servers holding the index (rplsh_server_add_index until the server is
   // destroyed): while non-zero every call that changes the rows, their
   // encoding or the tuning fails with RPLSH_ERR_STATE.

@@ -311,7 +311,8 @@ bool persist_supported(const DbView& db, const SearchParams& p) {
          persist_kernel(db.elem, db.dpad) != nullptr && std::min<int64_t>(p.kp, db.n) <= kPersistMaxKp;
 }
 
-void persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipStream_t st) {
+void persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipStream_t st,
+                 PhaseEvents* pe) {
   RPLSH_REQUIRE(!batches.empty(), "persistent kernel: no batches");
   const Elem e = batches[0].db->elem;
   const int dpad = batches[0].db->dpad;
@@ -400,8 +401,16 @@ void persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipS
   int* claims = S.sync.as<int>(nsteps);
   RPLSH_HIP(hipMemcpyAsync(dprog, hp, off, hipMemcpyHostToDevice, st));
   launch_fill_i32(claims, nsteps, 0, st);
+  // One batch: steps 1, 3, .., 2L-1 filter, 2, .., 2L select, 2L+1 re-rank.
+  const int last_filter = nsteps - 3;
+  auto rec = [&](int i) {
+    if (pe && batches.size() == 1) pe->rec(i, st);
+  };
   for (int sidx = 0; sidx < nsteps; ++sidx) {
     const int ub = step_unit[sidx], ue = step_unit[sidx + 1];
+    if (sidx == 1) rec(0);
+    if (sidx == last_filter) rec(1);
+    if (sidx == nsteps - 1) rec(4);
     if (ue <= ub) continue;
     const char* cprog = dprog;
     int pb = step_phase[sidx], pe = step_phase[sidx + 1];
@@ -409,6 +418,9 @@ void persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipS
     void* kargs[] = {&cprog, &pb, &pe, const_cast<int*>(&ub), const_cast<int*>(&ue), &claim};
     const int grid = std::min(ue - ub, S.resident);
     RPLSH_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kern), dim3(grid), dim3(kPBlock), kargs, lds, st));
+    if (sidx == last_filter) rec(2);
+    if (sidx == last_filter + 1) rec(3);
+    if (sidx == nsteps - 1) rec(5);
   }
   // Every batch's counters come back with the caller's next synchronisation.
   for (const PersistBatch& pb : batches) {

@@ -813,6 +813,7 @@ void rplsh_bf_config_default(rplsh_bf_config* cfg) {
   cfg->precision = RPLSH_PREC_FP32;
   cfg->kprime_factor = 4;
   cfg->fp64_boundary = 1;
+  cfg->persistent_max_queries = 256;
 }
 
 rplsh_status rplsh_bf_create(rplsh_resources* res, int dim, const rplsh_bf_config* cfg,
@@ -854,6 +855,7 @@ rplsh_status rplsh_bf_destroy(rplsh_bf* b) {
     DeviceScope scope(b->cfg.device);
     (void)hipDeviceSynchronize();
     bf_free_data(b);
+    if (b->pst) persist_state_destroy(b->pst);
     delete b;
   });
 }
@@ -942,6 +944,14 @@ rplsh_status rplsh_bf_set_precision(rplsh_bf* b, int precision, int kprime_facto
   });
 }
 
+rplsh_status rplsh_bf_set_persistent(rplsh_bf* b, int max_queries) {
+  return guard([&] {
+    RPLSH_REQUIRE(b, "index is NULL");
+    RPLSH_REQUIRE(max_queries >= 0, "max_queries must be >= 0");
+    b->cfg.persistent_max_queries = max_queries;
+  });
+}
+
 rplsh_status rplsh_bf_search(rplsh_bf* b, int64_t nq, const float* queries, int k,
                              int32_t* indices, float* distances) {
   return guard([&] {
@@ -980,7 +990,22 @@ rplsh_status rplsh_bf_search(rplsh_bf* b, int64_t nq, const float* queries, int 
       const DbView v = bf_view(b);
       const SearchParams p = bf_search_params(b, k, nq);
       RPLSH_REQUIRE(std::min<int64_t>(p.kp, b->n) <= 4096, "k' = %d exceeds 4096", p.kp);
-      search_batch(v, qd, (int)nq, p, id, dd, b->ws, 0, st, stats);
+      if (b->cfg.persistent_max_queries > 0 && nq <= b->cfg.persistent_max_queries &&
+          persist_supported(v, p)) {
+        // Small batch: the persistent kernel (same results; every row of a
+        // level scored, no carry between levels).
+        if (!b->pst) b->pst = persist_state_create();
+        PhaseEvents pe;
+        pe.create();
+        persist_run(*b->pst, {PersistBatch{&v, qd, (int)nq, &p, id, dd, &b->ws[0]}}, st, &pe);
+        RPLSH_HIP(hipStreamSynchronize(st));
+        finish_batch(v, qd, (int)nq, p, id, dd, b->ws, 0, st, stats, &pe,
+                     static_cast<const BatchCounters*>(b->ws[0].hctr.p));
+        stats.rows_filter_full = v.n;
+        stats.persistent = 1;
+      } else {
+        search_batch(v, qd, (int)nq, p, id, dd, b->ws, 0, st, stats);
+      }
     }
     if (!i_dev) RPLSH_HIP(hipMemcpyAsync(indices, id, (size_t)nq * k * sizeof(int32_t), hipMemcpyDefault, st));
     if (!d_dev) RPLSH_HIP(hipMemcpyAsync(distances, dd, (size_t)nq * k * sizeof(float), hipMemcpyDefault, st));
@@ -1391,7 +1416,7 @@ rplsh_status rplsh_bf_clear_tuning(rplsh_bf* b) {
 // load (they are functions of the rows).
 namespace {
 constexpr char kIndexMagic[8] = {'R', 'P', 'L', 'S', 'H', 'B', 'F', '1'};
-constexpr uint32_t kIndexVersion = 1;
+constexpr uint32_t kIndexVersion = 2;  // 2: configuration with persistent_max_queries
 }  // namespace
 
 rplsh_status rplsh_bf_save(rplsh_bf* b, const char* path) {

@@ -93,6 +93,7 @@ IndexBF::IndexBF(DeviceResources* resources, int dim, IndexBFConfig config)
   c.precision = (int)config.precision;
   c.kprime_factor = config.kprimeFactor;
   c.fp64_boundary = config.fp64Boundary;
+  c.persistent_max_queries = config.persistentMaxQueries;
   checkStatus(rplsh_bf_create(resources->handle(), dim, &c, &handle_), "IndexBF");
   isTrained_ = true;
 }

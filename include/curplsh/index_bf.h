@@ -31,7 +31,8 @@ struct IndexBFConfig : public IndexConfig {
         computeExactDistances(false),
         precision(Precision::FP32),
         kprimeFactor(4),
-        fp64Boundary(true) {}
+        fp64Boundary(true),
+        persistentMaxQueries(256) {}
 
   /// Accepted for compatibility; the resident layout is always row-major
   /// FP32 plus the candidate copy of the configured precision.
@@ -44,6 +45,9 @@ struct IndexBFConfig : public IndexConfig {
   Precision precision;
   int kprimeFactor;
   bool fp64Boundary;
+  /// BF16/FP8 searches of up to this many queries run in the persistent
+  /// kernel (same results); 0: never.
+  int persistentMaxQueries;
 };
 
 class IndexBF : public Index {

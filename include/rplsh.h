@@ -183,9 +183,14 @@ typedef struct rplsh_bf_config {
   int precision;               /* rplsh_precision */
   int kprime_factor;           /* over-fetch k' = kprime_factor * k for BF16/FP8 (0 -> 4) */
   int fp64_boundary;           /* FP64 re-rank of the boundary band (BF16/FP8), default 1 */
+  int persistent_max_queries;  /* BF16/FP8 searches of up to this many queries (k' <= 512)
+                                  run in the persistent kernel (one launch per step: prep,
+                                  each filter level and its select, re-rank; the same
+                                  results); 0: never. Default 256 */
 } rplsh_bf_config;
 
-/* Fills the defaults: device 0, Device memory, FP32, k' factor 4, boundary on. */
+/* Fills the defaults: device 0, Device memory, FP32, k' factor 4, boundary on,
+ * persistent kernel up to 256 queries. */
 void rplsh_bf_config_default(rplsh_bf_config* cfg);
 
 typedef struct rplsh_bf rplsh_bf;
@@ -221,6 +226,9 @@ rplsh_status rplsh_bf_num_data(const rplsh_bf* index, int64_t* n);
 rplsh_status rplsh_bf_dimension(const rplsh_bf* index, int* d);
 rplsh_status rplsh_bf_device(const rplsh_bf* index, int* device);
 rplsh_status rplsh_bf_set_precision(rplsh_bf* index, int precision, int kprime_factor);
+/* Largest batch IndexBF::search runs in the persistent kernel (0: never);
+ * results do not depend on it. */
+rplsh_status rplsh_bf_set_persistent(rplsh_bf* index, int max_queries);
 /* Device pointers of the resident data (row-major [n][d] f32, norms [n]). */
 rplsh_status rplsh_bf_data_dev(const rplsh_bf* index, const float** data, const float** norms);
 
@@ -246,6 +254,8 @@ typedef struct rplsh_search_stats {
                                level (packed copies, LDS filter variants) */
   int32_t filter_variant;   /* packed copies: filter variant used (rplsh_tune_result) */
   int32_t target_wgs;       /* workgroups the filter grid aimed for */
+  int32_t persistent;       /* 1: the search ran in the persistent kernel (its filter
+                               scores every row of a level: rows_filter_full = n) */
 } rplsh_search_stats;
 rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out);
 
