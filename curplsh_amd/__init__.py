@@ -181,7 +181,7 @@ class IndexBFConfig:
     precision: str | int = "fp32"
     kprime_factor: int = 4
     fp64_boundary: bool = True
-    persistent_max_queries: int = 256
+    persistent_max_queries: int = 32
 
     def to_c(self) -> _lib.BfConfig:
         c = _lib.BfConfig()

@@ -49,6 +49,18 @@ __device__ inline Frag<LB> load_frag(const char* p) {
   return f;
 }
 
+// The same load through a global (address space 1) pointer: for callers whose
+// pointers were read from memory, where the compiler would otherwise emit flat
+// loads that wait on the LDS counter too.
+template <int LB>
+__device__ inline Frag<LB> load_frag_global(const char* p) {
+  typedef const __attribute__((address_space(1))) i32x4* gp;
+  Frag<LB> f;
+#pragma unroll
+  for (int i = 0; i < LB / 16; ++i) f.v[i] = *(gp)(p + 16 * i);
+  return f;
+}
+
 template <int LB>
 __device__ inline Frag<LB> zero_frag() {
   Frag<LB> f;

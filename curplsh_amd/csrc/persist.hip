@@ -55,7 +55,7 @@ inline size_t persist_lds_bytes() {
 
 // ---- phase bodies --------------------------------------------------------
 
-__device__ void prep_unit(const PrepArgs& a, int local) {
+__device__ __forceinline__ void prep_unit(const PrepArgs& a, int local) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int q = local * 4 + w;
   if (local == 0 && threadIdx.x < 4) a.ctr[threadIdx.x] = 0;
@@ -85,11 +85,26 @@ __device__ void prep_unit(const PrepArgs& a, int local) {
   }
 }
 
-template <Elem E, int DPAD>
-__device__ void filter_unit(const FilterArgs& a, int local) {
+// Pointers read from the program are generic to the compiler; the filter's
+// loads go through global (address space 1) pointers, so they are counted on
+// vmcnt alone and can stay in flight together.
+template <class T>
+__device__ __forceinline__ T gload(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
+// WIDE: several tiles per wave and iteration, about 16 KB of row operands in
+// flight (the filter-only kernel); otherwise one tile (the general kernel,
+// whose other bodies need the registers).
+template <Elem E, int DPAD, bool WIDE>
+__device__ __forceinline__ void filter_unit(const FilterArgs& args, int local) {
+  // A private copy: the candidate stores below cannot alias it, so its fields
+  // stay in registers instead of being re-read after every store.
+  const FilterArgs a = args;
   using G = Geometry<E, DPAD>;
   constexpr int LB = G::LB;
   constexpr int NC = G::NC;
+  constexpr int TPW = !WIDE || G::TILE >= 12288 ? 1 : (G::TILE >= 6144 ? 2 : 4);
   const int qtile = local % a.num_qtiles;
   const int slice = local / a.num_qtiles;
   const int64_t r_begin = (int64_t)slice * a.rows_per_slice;
@@ -105,9 +120,9 @@ __device__ void filter_unit(const FilterArgs& a, int local) {
   if (qr < nq) {
     const char* qp = reinterpret_cast<const char*>(a.q) + (int64_t)qr * G::ROWB;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) qf[c] = load_frag<LB>(qp + 2 * LB * c + LB * h);
-    thr = fmaxf(-0.5f * a.tau[qr], -FLT_MAX);
-    if (E == Elem::FP8) qsc = e8m0_of(a.qscale[qr]);
+    for (int c = 0; c < NC; ++c) qf[c] = load_frag_global<LB>(qp + 2 * LB * c + LB * h);
+    thr = fmaxf(-0.5f * gload(a.tau + qr), -FLT_MAX);
+    if (E == Elem::FP8) qsc = e8m0_of(gload(a.qscale + qr));
   } else {
 #pragma unroll
     for (int c = 0; c < NC; ++c) qf[c] = zero_frag<LB>();
@@ -118,62 +133,80 @@ __device__ void filter_unit(const FilterArgs& a, int local) {
   uint64_t* dst = a.cand + (int64_t)qr * a.cap;
 
 #pragma unroll 1
-  for (int64_t t0 = r_begin + 32 * wave; t0 < r_end; t0 += 128) {
-    // Operand A: this lane's row of the tile (the same 32 rows for both
-    // halves of the wave, each half its part of every K-chunk).
-    const int64_t j = t0 + col;
-    const bool valid = j < r_end;
-    const int64_t p = valid ? j * stride : 0;
-    const char* ap = xb + (p >> 5) * G::TILE + ((p & 31) + 32 * h) * LB;
-    // C = -||x||^2 / 2 of rows t0 + 8g + 4h + i (-inf past the end).
-    f32x16 ci;
-    if (stride == 1 && t0 + 32 <= r_end) {
-      // Whole tile at stride 1: four aligned 16-byte loads (t0 is a multiple of 32).
+  for (int64_t t0 = r_begin + 32 * TPW * wave; t0 < r_end; t0 += 128 * TPW) {
+    // Operand A of tiles t0 + 32u: this lane's row (the same 32 rows for both
+    // halves of the wave, each half its part of every K-chunk). Lanes past the
+    // end load the last row instead: their output rows carry C = -inf and are
+    // never emitted, and unconditional loads stay in flight together.
+    Frag<LB> af[TPW][NC];
+    int xsc[TPW];
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const f32x4 v = *reinterpret_cast<const f32x4*>(a.xnorm + t0 + 8 * g + 4 * h);
+    for (int u = 0; u < TPW; ++u) {
+      int64_t j = t0 + 32 * u + col;
+      if (j >= r_end) j = r_end - 1;
+      const int64_t p = j * stride;
+      const char* ap = xb + (p >> 5) * G::TILE + ((p & 31) + 32 * h) * LB;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) ci[4 * g + i] = -0.5f * v[i];
-      }
+      for (int c = 0; c < NC; ++c) af[u][c] = load_frag_global<LB>(ap + c * G::CHUNK);
+      xsc[u] = E == Elem::FP8 ? e8m0_of(gload(a.xscale + p)) : 127;
+    }
+    // C = -||x||^2 / 2 of rows t0 + 32u + 8g + 4h + i (-inf past the end),
+    // loaded while the row operands are in flight.
+    f32x16 ci[TPW];
+    if (stride == 1 && t0 + 32 * TPW <= r_end) {
+      // Whole tiles at stride 1: aligned 16-byte loads (t0 is a multiple of 32).
+#pragma unroll
+      for (int u = 0; u < TPW; ++u)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const f32x4 v = gload(reinterpret_cast<const f32x4*>(a.xnorm + t0 + 32 * u + 8 * g + 4 * h));
+#pragma unroll
+          for (int i = 0; i < 4; ++i) ci[u][4 * g + i] = -0.5f * v[i];
+        }
     } else {
 #pragma unroll
-      for (int g = 0; g < 4; ++g)
+      for (int u = 0; u < TPW; ++u)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int64_t r = t0 + 8 * g + 4 * h + i;
-          ci[4 * g + i] = r < r_end ? -0.5f * a.xnorm[r * stride] : -INFINITY;
-        }
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int64_t r = t0 + 32 * u + 8 * g + 4 * h + i;
+            const float v = gload(a.xnorm + (r < r_end ? r : r_end - 1) * stride);
+            ci[u][4 * g + i] = r < r_end ? -0.5f * v : -INFINITY;
+          }
     }
-    const int xsc = (E == Elem::FP8 && valid) ? e8m0_of(a.xscale[p]) : 127;
-    f32x16 acc;
+    // Every load of the tiles is issued before the first product.
+    if constexpr (WIDE) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const Frag<LB> af = valid ? load_frag<LB>(ap + c * G::CHUNK) : zero_frag<LB>();
-      acc = mfma_chunk<E>(af, qf[c], c == 0 ? ci : acc, xsc, qsc);
-    }
-    if (a.dense) {
-      // Every row survives (tau = +inf): row r's word goes to slot r.
-      if (qr < nq) {
+    for (int u = 0; u < TPW; ++u) {
+      const int64_t tu = t0 + 32 * u;
+      f32x16 acc;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int64_t r = t0 + 8 * (i >> 2) + 4 * h + (i & 3);
-          if (r < r_end) dst[r] = pack_cand(-2.0f * acc[i], (int32_t)(r * stride));
+      for (int c = 0; c < NC; ++c) acc = mfma_chunk<E>(af[u][c], qf[c], c == 0 ? ci[u] : acc, xsc[u], qsc);
+      if (a.dense) {
+        // Every row survives (tau = +inf): row r's word goes to slot r.
+        if (qr < nq) {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const int64_t r = tu + 8 * (i >> 2) + 4 * h + (i & 3);
+            if (r < r_end) dst[r] = pack_cand(-2.0f * acc[i], (int32_t)(r * stride));
+          }
         }
+        continue;
       }
-      continue;
-    }
-    float m = acc[0];
+      float m = acc[0];
 #pragma unroll
-    for (int i = 1; i < 16; ++i) m = fmaxf(m, acc[i]);
-    if (__any(m >= thr)) {
-      if (m >= thr) {
+      for (int i = 1; i < 16; ++i) m = fmaxf(m, acc[i]);
+      if (__any(m >= thr)) {
+        if (m >= thr) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const float v = acc[i];
-          if (v >= thr) {
-            const int64_t r = t0 + 8 * (i >> 2) + 4 * h + (i & 3);
-            const int pos = atomicAdd(cnt, 1);
-            if (pos < a.cap) dst[pos] = pack_cand(-2.0f * v, (int32_t)(r * stride));
+          for (int i = 0; i < 16; ++i) {
+            const float v = acc[i];
+            if (v >= thr) {
+              const int64_t r = tu + 8 * (i >> 2) + 4 * h + (i & 3);
+              const int pos = atomicAdd(cnt, 1);
+              if (pos < a.cap) dst[pos] = pack_cand(-2.0f * v, (int32_t)(r * stride));
+            }
           }
         }
       }
@@ -181,7 +214,7 @@ __device__ void filter_unit(const FilterArgs& a, int local) {
   }
 }
 
-__device__ void select_unit(const SelectArgs& a, int local, int count_reset, char* lds) {
+__device__ __forceinline__ void select_unit(const SelectArgs& a, int local, int count_reset, char* lds) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int q = local * 4 + w;
   if (q >= a.nq) return;
@@ -194,7 +227,7 @@ __device__ void select_unit(const SelectArgs& a, int local, int count_reset, cha
 }
 
 template <int NP>
-__device__ void rerank_unit(const RerankTailArgs& t, int q, uint64_t* lds, float* sq, float* sxn, int* sh) {
+__device__ __forceinline__ void rerank_unit(const RerankTailArgs& t, int q, uint64_t* lds, float* sq, float* sxn, int* sh) {
   rerank_query<NP>(t.ra, q, lds, sq, sxn, sh);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -209,9 +242,41 @@ __device__ void rerank_unit(const RerankTailArgs& t, int q, uint64_t* lds, float
 
 // ---- the kernel ----------------------------------------------------------
 
-// One step: units [unit_begin, unit_end) of phases [phase_begin, phase_end),
-// taken from *claim (zeroed before the launch).
+// Any phase kind's unit (the general step kernel).
 template <Elem E, int DPAD>
+__device__ __forceinline__ void run_unit(const PersistPhase& P, const char* arg, int local, uint64_t* plds, float* sq_fixed,
+                         float* sxn, int* sh) {
+  {
+    switch (P.kind) {
+      case kPkPrep:
+        prep_unit(*reinterpret_cast<const PrepArgs*>(arg), local);
+        break;
+      case kPkFilter:
+        filter_unit<E, DPAD, false>(*reinterpret_cast<const FilterArgs*>(arg), local);
+        break;
+      case kPkSelect:
+        select_unit(*reinterpret_cast<const SelectArgs*>(arg), local, P.count_reset,
+                    reinterpret_cast<char*>(plds));
+        break;
+      default: {
+        const RerankTailArgs& t = *reinterpret_cast<const RerankTailArgs*>(arg);
+        if (rerank_staged(t.ra.x, t.ra.ldx, t.ra.d)) {
+          if (t.ra.d <= 128) rerank_unit<1>(t, local, plds, sq_fixed, sxn, sh);
+          else rerank_unit<2>(t, local, plds, sq_fixed, sxn, sh);
+        } else {
+          rerank_unit<0>(t, local, plds, sq_fixed, sxn, sh);
+        }
+        break;
+      }
+    }
+  }
+}
+
+// One step: units [unit_begin, unit_end) of phases [phase_begin, phase_end),
+// taken from *claim (zeroed before the launch). FILTER: a step of filter
+// phases only, compiled without the other bodies (fewer registers, so more
+// waves per SIMD hide the row loads); otherwise every kind.
+template <Elem E, int DPAD, bool FILTER>
 __global__ __launch_bounds__(kPBlock, 2) void persistent_step_kernel(const char* __restrict__ prog,
                                                                      int phase_begin, int phase_end,
                                                                      int unit_begin, int unit_end,
@@ -238,27 +303,10 @@ __global__ __launch_bounds__(kPBlock, 2) void persistent_step_kernel(const char*
     const PersistPhase P = ph[lo];
     const int local = u - P.begin;
     const char* arg = prog + P.arg;
-    switch (P.kind) {
-      case kPkPrep:
-        prep_unit(*reinterpret_cast<const PrepArgs*>(arg), local);
-        break;
-      case kPkFilter:
-        filter_unit<E, DPAD>(*reinterpret_cast<const FilterArgs*>(arg), local);
-        break;
-      case kPkSelect:
-        select_unit(*reinterpret_cast<const SelectArgs*>(arg), local, P.count_reset,
-                    reinterpret_cast<char*>(plds));
-        break;
-      default: {
-        const RerankTailArgs& t = *reinterpret_cast<const RerankTailArgs*>(arg);
-        if (rerank_staged(t.ra.x, t.ra.ldx, t.ra.d)) {
-          if (t.ra.d <= 128) rerank_unit<1>(t, local, plds, sq_fixed, sxn, sh);
-          else rerank_unit<2>(t, local, plds, sq_fixed, sxn, sh);
-        } else {
-          rerank_unit<0>(t, local, plds, sq_fixed, sxn, sh);
-        }
-        break;
-      }
+    if constexpr (FILTER) {
+      filter_unit<E, DPAD, true>(*reinterpret_cast<const FilterArgs*>(arg), local);
+    } else {
+      run_unit<E, DPAD>(P, arg, local, plds, sq_fixed, sxn, sh);
     }
     __syncthreads();  // s_unit is rewritten by the next claim
   }
@@ -267,28 +315,28 @@ __global__ __launch_bounds__(kPBlock, 2) void persistent_step_kernel(const char*
 typedef void (*PersistKernel)(const char*, int, int, int, int, int*);
 
 template <Elem E, int DPAD>
-PersistKernel kernel_for() {
-  return persistent_step_kernel<E, DPAD>;
+PersistKernel kernel_for(bool filter) {
+  return filter ? persistent_step_kernel<E, DPAD, true> : persistent_step_kernel<E, DPAD, false>;
 }
 
-PersistKernel persist_kernel(Elem e, int dpad) {
+PersistKernel persist_kernel(Elem e, int dpad, bool filter) {
   if (e == Elem::BF16) {
     switch (dpad) {
-      case 32: return kernel_for<Elem::BF16, 32>();
-      case 64: return kernel_for<Elem::BF16, 64>();
-      case 96: return kernel_for<Elem::BF16, 96>();
-      case 128: return kernel_for<Elem::BF16, 128>();
-      case 160: return kernel_for<Elem::BF16, 160>();
-      case 192: return kernel_for<Elem::BF16, 192>();
-      case 256: return kernel_for<Elem::BF16, 256>();
+      case 32: return kernel_for<Elem::BF16, 32>(filter);
+      case 64: return kernel_for<Elem::BF16, 64>(filter);
+      case 96: return kernel_for<Elem::BF16, 96>(filter);
+      case 128: return kernel_for<Elem::BF16, 128>(filter);
+      case 160: return kernel_for<Elem::BF16, 160>(filter);
+      case 192: return kernel_for<Elem::BF16, 192>(filter);
+      case 256: return kernel_for<Elem::BF16, 256>(filter);
       default: return nullptr;
     }
   }
   if (e == Elem::FP8) {
     switch (dpad) {
-      case 64: return kernel_for<Elem::FP8, 64>();
-      case 128: return kernel_for<Elem::FP8, 128>();
-      case 256: return kernel_for<Elem::FP8, 256>();
+      case 64: return kernel_for<Elem::FP8, 64>(filter);
+      case 128: return kernel_for<Elem::FP8, 128>(filter);
+      case 256: return kernel_for<Elem::FP8, 256>(filter);
       default: return nullptr;
     }
   }
@@ -298,9 +346,10 @@ PersistKernel persist_kernel(Elem e, int dpad) {
 }  // namespace
 
 struct PersistState {
-  DevBuf prog, sync;  // program; one unit counter per step
-  HostBuf hprog;      // program staging
-  int resident = 0;   // workgroups the device holds at once (per kernel)
+  DevBuf prog, sync;   // program; one unit counter per step
+  HostBuf hprog;       // program staging
+  int resident = 0;    // workgroups the device holds at once: general kernel
+  int resident_f = 0;  // filter-only kernel
 };
 
 PersistState* persist_state_create() { return new PersistState(); }
@@ -308,7 +357,7 @@ void persist_state_destroy(PersistState* s) { delete s; }
 
 bool persist_supported(const DbView& db, const SearchParams& p) {
   return db.packed && db.n > 0 && db.elem != Elem::F32 && db.d <= 256 &&
-         persist_kernel(db.elem, db.dpad) != nullptr && std::min<int64_t>(p.kp, db.n) <= kPersistMaxKp;
+         persist_kernel(db.elem, db.dpad, false) != nullptr && std::min<int64_t>(p.kp, db.n) <= kPersistMaxKp;
 }
 
 void persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipStream_t st,
@@ -316,18 +365,22 @@ void persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipS
   RPLSH_REQUIRE(!batches.empty(), "persistent kernel: no batches");
   const Elem e = batches[0].db->elem;
   const int dpad = batches[0].db->dpad;
-  PersistKernel kern = persist_kernel(e, dpad);
-  RPLSH_REQUIRE(kern, "persistent kernel: element kind / dimension not instantiated");
+  PersistKernel kern = persist_kernel(e, dpad, false);
+  PersistKernel kern_f = persist_kernel(e, dpad, true);
+  RPLSH_REQUIRE(kern && kern_f, "persistent kernel: element kind / dimension not instantiated");
   const size_t lds = persist_lds_bytes();
   if (S.resident == 0) {
-    int dev = 0, cus = 0, per_cu = 0;
+    int dev = 0, cus = 0, per_cu = 0, per_cu_f = 0;
     RPLSH_HIP(hipGetDevice(&dev));
     RPLSH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     RPLSH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern),
                                                            kPBlock, lds));
+    RPLSH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_f, reinterpret_cast<const void*>(kern_f),
+                                                           kPBlock, 0));
     S.resident = std::max(1, cus * std::max(1, per_cu));
+    S.resident_f = std::max(1, cus * std::max(1, per_cu_f));
   }
-  const int64_t target_units = S.resident;
+  const int64_t target_units = S.resident_f;
 
   // Plans, then the program: phases ordered by position (all batches' preps,
   // then their first filters, ...); position s is step s.
@@ -357,14 +410,21 @@ void persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipS
     }
     per[b].push_back({kPkRerank, nq, -1, &pl.rerank, sizeof(RerankTailArgs)});
   }
+  // Batches are aligned at their last phase (the re-rank): with fewer levels
+  // a batch starts later, so every step holds one kind of phase or the
+  // prep / select mix (filters at odd steps, re-ranks at the last).
   std::vector<PersistPhase> phases;
   std::vector<std::pair<const void*, size_t>> args;
   std::vector<int> step_phase{0}, step_unit{0};  // boundaries of the steps
+  std::vector<char> step_filter;                 // step of filter phases only
   int units = 0;
   for (size_t pos = 0; pos < max_phases; ++pos) {
+    bool only_filter = true;
     for (size_t b = 0; b < batches.size(); ++b) {
-      if (pos >= per[b].size()) continue;
-      const Rec& r = per[b][pos];
+      const size_t shift = max_phases - per[b].size();
+      if (pos < shift) continue;
+      const Rec& r = per[b][pos - shift];
+      only_filter = only_filter && r.kind == kPkFilter;
       PersistPhase P{};
       P.kind = r.kind;
       P.begin = units;
@@ -377,6 +437,7 @@ void persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipS
     }
     step_phase.push_back((int)phases.size());
     step_unit.push_back(units);
+    step_filter.push_back(only_filter ? 1 : 0);
   }
   size_t off = round_up(sizeof(PersistHeader) + phases.size() * sizeof(PersistPhase), 16);
   std::vector<size_t> arg_off(args.size());
@@ -413,11 +474,16 @@ void persist_run(PersistState& S, const std::vector<PersistBatch>& batches, hipS
     if (sidx == nsteps - 1) rec(4);
     if (ue <= ub) continue;
     const char* cprog = dprog;
-    int pb = step_phase[sidx], pe = step_phase[sidx + 1];
+    int ph0 = step_phase[sidx], ph1 = step_phase[sidx + 1];
     int* claim = claims + sidx;
-    void* kargs[] = {&cprog, &pb, &pe, const_cast<int*>(&ub), const_cast<int*>(&ue), &claim};
-    const int grid = std::min(ue - ub, S.resident);
-    RPLSH_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kern), dim3(grid), dim3(kPBlock), kargs, lds, st));
+    void* kargs[] = {&cprog, &ph0, &ph1, const_cast<int*>(&ub), const_cast<int*>(&ue), &claim};
+    if (step_filter[sidx]) {
+      const int grid = std::min(ue - ub, S.resident_f);
+      RPLSH_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kern_f), dim3(grid), dim3(kPBlock), kargs, 0, st));
+    } else {
+      const int grid = std::min(ue - ub, S.resident);
+      RPLSH_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kern), dim3(grid), dim3(kPBlock), kargs, lds, st));
+    }
     if (sidx == last_filter) rec(2);
     if (sidx == last_filter + 1) rec(3);
     if (sidx == nsteps - 1) rec(5);

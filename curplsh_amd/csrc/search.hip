@@ -813,7 +813,7 @@ void rplsh_bf_config_default(rplsh_bf_config* cfg) {
   cfg->precision = RPLSH_PREC_FP32;
   cfg->kprime_factor = 4;
   cfg->fp64_boundary = 1;
-  cfg->persistent_max_queries = 256;
+  cfg->persistent_max_queries = 32;
 }
 
 rplsh_status rplsh_bf_create(rplsh_resources* res, int dim, const rplsh_bf_config* cfg,

@@ -32,7 +32,7 @@ struct IndexBFConfig : public IndexConfig {
         precision(Precision::FP32),
         kprimeFactor(4),
         fp64Boundary(true),
-        persistentMaxQueries(256) {}
+        persistentMaxQueries(32) {}
 
   /// Accepted for compatibility; the resident layout is always row-major
   /// FP32 plus the candidate copy of the configured precision.

@@ -186,7 +186,8 @@ typedef struct rplsh_bf_config {
   int persistent_max_queries;  /* BF16/FP8 searches of up to this many queries (k' <= 512)
                                   run in the persistent kernel (one launch per step: prep,
                                   each filter level and its select, re-rank; the same
-                                  results); 0: never. Default 256 */
+                                  results); 0: never. Default 32 (one query tile: there
+                                  the persistent filter streams the copy fastest) */
 } rplsh_bf_config;
 
 /* Fills the defaults: device 0, Device memory, FP32, k' factor 4, boundary on,

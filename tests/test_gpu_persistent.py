@@ -104,3 +104,26 @@ def test_persistent_retry_path(ca, pindexes):
         srv.close()
     finally:
         idx.clear_tuning()
+
+
+@pytest.mark.parametrize("which", [0, 1, 3, 5, 6])
+def test_indexbf_persistent_equals_launched(ca, pindexes, which):
+    """IndexBF.search on the persistent kernel (small batches, one launch per
+    step) and on the launched pipeline return the same indices and distance
+    bits, at one query tile, a ragged one and several."""
+    idx = pindexes[which]
+    n, d, prec = SPECS[which]
+    try:
+        for nq, k in ((1, 10), (7, 100), (32, 10), (33, 1), (100, 100)):
+            q = ca.gen_uniform(nq, d, 90_000 + 31 * which + nq, -1, 1)
+            print(f"[persistent] IndexBF {SPECS[which]} nq {nq} k {k}", flush=True)
+            idx.set_persistent(0)
+            Il, Dl = idx.search(q, k)
+            assert idx.last_stats()["persistent"] == 0
+            idx.set_persistent(128)
+            Ip, Dp = idx.search(q, k)
+            assert idx.last_stats()["persistent"] == 1
+            assert np.array_equal(Ip, Il), f"{SPECS[which]} nq {nq} k {k}"
+            assert np.array_equal(Dp.view(np.uint32), Dl.view(np.uint32)), f"{SPECS[which]} nq {nq} k {k}"
+    finally:
+        idx.set_persistent(32)
