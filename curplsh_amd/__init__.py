@@ -172,6 +172,9 @@ class IndexBFConfig:
 
     precision: "fp32" (plain path), "bf16" or "fp8" (precision forms).
     kprime_factor: over-fetch k' = kprime_factor * k for the precision forms.
+    residual_clusters: BF16/FP8 on clustered rows: > 0 adds the cluster-
+    residual copy (rows minus the nearest of this many centroids) that the
+    full filter level reads; 0 (default) off.
     """
 
     device: int = 0
@@ -182,6 +185,7 @@ class IndexBFConfig:
     kprime_factor: int = 4
     fp64_boundary: bool = True
     persistent_max_queries: int = 32
+    residual_clusters: int = 0
 
     def to_c(self) -> _lib.BfConfig:
         c = _lib.BfConfig()
@@ -193,6 +197,7 @@ class IndexBFConfig:
         c.kprime_factor = self.kprime_factor
         c.fp64_boundary = int(self.fp64_boundary)
         c.persistent_max_queries = int(self.persistent_max_queries)
+        c.residual_clusters = int(self.residual_clusters)
         return c
 
 

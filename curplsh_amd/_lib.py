@@ -50,6 +50,7 @@ class BfConfig(C.Structure):
         ("kprime_factor", C.c_int),
         ("fp64_boundary", C.c_int),
         ("persistent_max_queries", C.c_int),
+        ("residual_clusters", C.c_int),
     ]
 
 

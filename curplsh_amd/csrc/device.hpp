@@ -85,6 +85,29 @@ __host__ __device__ inline int64_t sample_source_row(int64_t r, int64_t step, in
   return (int64_t)(((uint64_t)(r >> 5) * (uint64_t)step) % (uint64_t)tiles) * 32 + (r & 31);
 }
 
+// Full level over the cluster-residual copy (resid.hip): 32-row tiles of
+// one centroid each, grouped into slices of one centroid.
+struct ResidArgs {
+  const void* x;            // residual rows r = x - c, packed tiles
+  const float* xscale;      // [npad] FP8 per-row scales of r
+  const float* xnh;         // [npad] -||r||^2 / 2 (-inf: padding row)
+  const int32_t* rid;       // [npad] original row (-1: padding)
+  const int32_t* slice_t0;  // [num_slices + 1] first tile of each slice
+  const int32_t* slice_cl;  // [num_slices] centroid of each slice
+  int num_slices;
+  const float* cent;        // [K][d] centroids (FP32)
+  int d;
+  const float* q;           // FP32 queries [nq][ldq]
+  int ldq;
+  const float* tau;         // [nq] threshold on the approximate relative distance
+  int nq;
+  const int* nq_dev;        // if set: the number of live queries (<= nq)
+  int num_qtiles;
+  int cap;
+  int* counts;
+  uint64_t* cand;
+};
+
 // Carry of a sample level's survivors into the next level's lists.
 struct CarryArgs {
   uint64_t* cand;        // [nq][cap]: the sample level's lists, compacted in place

@@ -199,6 +199,17 @@ struct DbView {
   const float* snh = nullptr;
   int64_t snh_inf = 0;
   bool packed = false;            // xenc is in 32-row MFMA tiles (filter_packed.hip)
+  // Cluster-residual copy (resid.hip) for the full level, when built: rpad
+  // padded rows in tiles of one centroid each, rslices slices.
+  const void* renc = nullptr;
+  const float* rscale = nullptr;
+  const float* rnh = nullptr;
+  const int32_t* rid = nullptr;
+  const int32_t* rslice_t0 = nullptr;
+  const int32_t* rslice_cl = nullptr;
+  const float* rcent = nullptr;
+  int64_t rslices = 0;
+  int64_t rpad = 0;
 };
 
 struct SearchParams {
@@ -325,6 +336,22 @@ struct rplsh_bf {
   int sprec = -1;  // precision the sample buffers were allocated for
   int enc_precision = -1;  // precision xenc currently holds
   int dpad = 0;
+  // Cluster-residual copy (cfg.residual_clusters > 0, BF16 / FP8): rk
+  // centroids rcent [rk][dim], built for the first rn rows; rpad padded rows
+  // (rid: original row or -1), rtcl: centroid per tile, rslices slices
+  // (rslice_t0 [rslices + 1], rslice_cl); renc / rscale / rnh: the residual
+  // rows for precision rprec.
+  float* rcent = nullptr;
+  int32_t* rid = nullptr;
+  int32_t* rtcl = nullptr;
+  int32_t* rslice_t0 = nullptr;
+  int32_t* rslice_cl = nullptr;
+  void* renc = nullptr;
+  float* rscale = nullptr;
+  float* rnh = nullptr;
+  int rk = 0;
+  int64_t rn = 0, rpad = 0, rslices = 0;
+  int rprec = -1;
   std::vector<rplsh::Workspace> ws;
   rplsh::DevBuf qdev, idev, ddev;
   rplsh_search_stats last{};
@@ -340,6 +367,12 @@ struct rplsh_bf {
 };
 
 namespace rplsh {
+// Cluster-residual copy of b's rows (resid.hip): (re)builds the centroids and
+// layout when the rows changed since the last build, and the residual rows
+// when the precision changed; synchronises st. resid_free releases it.
+void resid_build(rplsh_bf* b, hipStream_t st);
+void resid_free(rplsh_bf* b);
+
 // Fails with RPLSH_ERR_STATE while a server holds the index; otherwise bumps
 // its generation (the caller is about to change it).
 void bf_begin_change(rplsh_bf* b);

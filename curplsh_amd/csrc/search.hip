@@ -214,6 +214,33 @@ static void run_filter(const DbView& db, const Level& lv, bool dense, const void
   }
 }
 
+// The full level over the cluster-residual copy: every padded row, one
+// workgroup per (query block, slice); the lists start empty.
+static void run_filter_resid(const DbView& db, const float* q, const float* tau, int nq, const int* nq_dev,
+                             int cap, int* counts, uint64_t* cand, hipStream_t st) {
+  ResidArgs ra{};
+  ra.x = db.renc;
+  ra.xscale = db.rscale;
+  ra.xnh = db.rnh;
+  ra.rid = db.rid;
+  ra.slice_t0 = db.rslice_t0;
+  ra.slice_cl = db.rslice_cl;
+  ra.num_slices = (int)db.rslices;
+  ra.cent = db.rcent;
+  ra.d = db.d;
+  ra.q = q;
+  ra.ldq = db.d;
+  ra.tau = tau;
+  ra.nq = nq;
+  ra.nq_dev = nq_dev;
+  ra.num_qtiles = (int)div_up(nq, resid_block_queries(db.elem, db.dpad));
+  ra.cap = cap;
+  ra.counts = counts;
+  ra.cand = cand;
+  launch_fill_i32(counts, nq, 0, st);
+  launch_filter_resid(db.elem, db.dpad, ra, st);
+}
+
 // Carry: a level whose predecessor read whole tiles of the sample copy starts
 // from the predecessor's survivors that pass its own threshold and does not
 // score those rows again (LDS variants): a sample level starts at the
@@ -224,6 +251,7 @@ static int64_t carried_tiles(const DbView& db, const std::vector<Level>& L, size
   if (li == 0 || li >= L.size()) return 0;
   const Level& lp = L[li - 1];
   const Level& lv = L[li];
+  if (db.renc && lv.stride == 1) return 0;  // the residual full level scores every row
   const bool lds = variant == kFilterLdsWide || variant == kFilterLdsNarrow || variant == kFilterLdsWide8;
   if (!lds || !db.packed || !db.trank || lp.stride <= 1 || db.srows < lp.m) return 0;
   if (lv.stride == 1) return lp.m / 32;                       // into the full level
@@ -284,8 +312,11 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
     const bool last = li + 1 == L.size();
     if (last) rec(1);
     const int64_t carried = carried_tiles(db, L, li, p.variant);
-    run_filter(db, lv, li == 0, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs,
-               p.variant, last ? carried : 0, last ? 0 : 32 * carried, st);
+    if (last && li > 0 && db.renc)
+      run_filter_resid(db, q, tau, nq, nq_dev, cap, counts, cand, st);
+    else
+      run_filter(db, lv, li == 0, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs,
+                 p.variant, last ? carried : 0, last ? 0 : 32 * carried, st);
     if (last) rec(2);
     SelectArgs sa{};
     sa.cand = cand;
@@ -517,7 +548,9 @@ void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& 
   if (depth == 0) {
     const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
     stats.levels = (int32_t)L.size();
-    stats.rows_filter_full = L.back().m - 32 * carried_tiles(db, L, L.size() - 1, p.variant);
+    stats.rows_filter_full = db.renc && L.size() > 1
+                                 ? db.rpad
+                                 : L.back().m - 32 * carried_tiles(db, L, L.size() - 1, p.variant);
     stats.filter_variant = p.variant;
     stats.target_wgs = (int32_t)p.target_wgs;
     if (pe && pe->on) {
@@ -666,6 +699,7 @@ static void bf_encode(rplsh_bf* b, int64_t from, hipStream_t st) {
     b->enc_precision = b->cfg.precision;
     b->dpad = filter_dpad(e, b->dim);
     bf_free_sample(b);
+    resid_free(b);
     return;
   }
   const int dpad = filter_dpad(e, b->dim);
@@ -687,10 +721,17 @@ static void bf_encode(rplsh_bf* b, int64_t from, hipStream_t st) {
     b->dpad = dpad;
     from = 0;
   }
-  if (b->n - from <= 0) return;
-  launch_encode_packed(e, dpad, b->x, b->n, b->dim, b->ldx, from, b->xenc, b->xscale, st);
-  launch_neg_half_norms(b->xn, from, b->n, b->xnh, div_up(b->capacity, 32) * 32, st);
-  bf_encode_sample(b, e, dpad, st);
+  if (b->n - from > 0) {
+    launch_encode_packed(e, dpad, b->x, b->n, b->dim, b->ldx, from, b->xenc, b->xscale, st);
+    launch_neg_half_norms(b->xn, from, b->n, b->xnh, div_up(b->capacity, 32) * 32, st);
+    bf_encode_sample(b, e, dpad, st);
+  }
+  // The residual copy serves the full level of a multi-level search (more
+  // than 2,048 rows; below that one dense level reads the regular copy).
+  if (b->cfg.residual_clusters > 0 && b->n > 2048)
+    resid_build(b, st);
+  else
+    resid_free(b);
 }
 
 static DbView bf_view(const rplsh_bf* b) {
@@ -725,6 +766,17 @@ static DbView bf_view(const rplsh_bf* b) {
     v.trank = b->senc ? b->trank : nullptr;
     v.rank_tiles = div_up(b->n, 32);
     v.packed = true;
+    if (b->renc && b->rprec == b->cfg.precision && b->rn == b->n) {
+      v.renc = b->renc;
+      v.rscale = b->rscale;
+      v.rnh = b->rnh;
+      v.rid = b->rid;
+      v.rslice_t0 = b->rslice_t0;
+      v.rslice_cl = b->rslice_cl;
+      v.rcent = b->rcent;
+      v.rslices = b->rslices;
+      v.rpad = b->rpad;
+    }
   }
   return v;
 }
@@ -735,6 +787,7 @@ static void bf_free_data(rplsh_bf* b) {
   if (b->xenc) (void)hipFree(b->xenc);
   if (b->xscale) (void)hipFree(b->xscale);
   bf_free_sample(b);
+  resid_free(b);
   if (b->xnh) (void)hipFree(b->xnh);
   b->xnh = nullptr;
   b->x = nullptr;
@@ -814,6 +867,7 @@ void rplsh_bf_config_default(rplsh_bf_config* cfg) {
   cfg->kprime_factor = 4;
   cfg->fp64_boundary = 1;
   cfg->persistent_max_queries = 32;
+  cfg->residual_clusters = 0;
 }
 
 rplsh_status rplsh_bf_create(rplsh_resources* res, int dim, const rplsh_bf_config* cfg,
@@ -834,6 +888,7 @@ rplsh_status rplsh_bf_create(rplsh_resources* res, int dim, const rplsh_bf_confi
                   "dimension %d is not supported by precision %d (FP32: up to %d; BF16/FP8: up to 256)",
                   dim, c.precision, kMaxKLoopDim);
     if (c.kprime_factor <= 0) c.kprime_factor = 4;
+    RPLSH_REQUIRE(c.residual_clusters >= 0, "residual_clusters must be >= 0");
     if (rplsh_resources_init_device(res, c.device) != RPLSH_OK)
       fail(RPLSH_ERR_STATE, "%s", rplsh_last_error());
     auto* b = new rplsh_bf();
@@ -1416,7 +1471,7 @@ rplsh_status rplsh_bf_clear_tuning(rplsh_bf* b) {
 // load (they are functions of the rows).
 namespace {
 constexpr char kIndexMagic[8] = {'R', 'P', 'L', 'S', 'H', 'B', 'F', '1'};
-constexpr uint32_t kIndexVersion = 2;  // 2: configuration with persistent_max_queries
+constexpr uint32_t kIndexVersion = 3;  // 3: configuration with residual_clusters
 }  // namespace
 
 rplsh_status rplsh_bf_save(rplsh_bf* b, const char* path) {

@@ -94,6 +94,7 @@ IndexBF::IndexBF(DeviceResources* resources, int dim, IndexBFConfig config)
   c.kprime_factor = config.kprimeFactor;
   c.fp64_boundary = config.fp64Boundary;
   c.persistent_max_queries = config.persistentMaxQueries;
+  c.residual_clusters = config.residualClusters;
   checkStatus(rplsh_bf_create(resources->handle(), dim, &c, &handle_), "IndexBF");
   isTrained_ = true;
 }

@@ -32,7 +32,8 @@ struct IndexBFConfig : public IndexConfig {
         precision(Precision::FP32),
         kprimeFactor(4),
         fp64Boundary(true),
-        persistentMaxQueries(32) {}
+        persistentMaxQueries(32),
+        residualClusters(0) {}
 
   /// Accepted for compatibility; the resident layout is always row-major
   /// FP32 plus the candidate copy of the configured precision.
@@ -48,6 +49,10 @@ struct IndexBFConfig : public IndexConfig {
   /// BF16/FP8 searches of up to this many queries run in the persistent
   /// kernel (same results); 0: never.
   int persistentMaxQueries;
+  /// BF16/FP8 on clustered rows: > 0 adds a candidate copy of the rows minus
+  /// the nearest of this many centroids, read by the full filter level
+  /// (keeps the low-precision error at the scale of a cluster's spread).
+  int residualClusters;
 };
 
 class IndexBF : public Index {

@@ -188,6 +188,15 @@ typedef struct rplsh_bf_config {
                                   each filter level and its select, re-rank; the same
                                   results); 0: never. Default 32 (one query tile: there
                                   the persistent filter streams the copy fastest) */
+  int residual_clusters;       /* BF16/FP8 on clustered rows: > 0 builds a second candidate
+                                  copy holding each row minus the nearest of this many
+                                  centroids (farthest-point seeding + Lloyd on a sample),
+                                  in tiles of one centroid; the full filter level then
+                                  contracts (q - c).(x - c) with the exact FP32 terms
+                                  ||x - c||^2 and q.c - ||c||^2/2 in the C operand, which
+                                  keeps the low-precision error at the scale of a
+                                  cluster's spread. 0: off (default). Searches with it
+                                  run on the launched pipeline. */
 } rplsh_bf_config;
 
 /* Fills the defaults: device 0, Device memory, FP32, k' factor 4, boundary on,
