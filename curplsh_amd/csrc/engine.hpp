@@ -199,17 +199,25 @@ struct DbView {
   const float* snh = nullptr;
   int64_t snh_inf = 0;
   bool packed = false;            // xenc is in 32-row MFMA tiles (filter_packed.hip)
-  // Cluster-residual copy (resid.hip) for the full level, when built: rpad
-  // padded rows in tiles of one centroid each, rslices slices.
-  const void* renc = nullptr;
-  const float* rscale = nullptr;
-  const float* rnh = nullptr;
-  const int32_t* rid = nullptr;
-  const int32_t* rslice_t0 = nullptr;
-  const int32_t* rslice_cl = nullptr;
+  // Cluster-residual copies (resid.hip), when built: rcopy[0] holds every
+  // row, rcopy[j] (j > 0) a systematic sample of every rstride[j]-th row in
+  // centroid order; each in tiles of one centroid. Every filter level then
+  // reads one of them (resid_levels in search.hip).
+  static constexpr int kMaxResidCopies = 8;
+  int nrcopies = 0;
   const float* rcent = nullptr;
-  int64_t rslices = 0;
-  int64_t rpad = 0;
+  struct ResidView {
+    const void* enc = nullptr;
+    const float* scale = nullptr;
+    const float* nh = nullptr;
+    const int32_t* rid = nullptr;
+    const int32_t* slice_t0 = nullptr;
+    const int32_t* slice_cl = nullptr;
+    int64_t slices = 0;
+    int64_t npad = 0;    // padded rows
+    int64_t rows = 0;    // real rows
+    int64_t stride = 1;  // sampling stride
+  } rcopy[kMaxResidCopies];
 };
 
 struct SearchParams {
@@ -336,21 +344,25 @@ struct rplsh_bf {
   int sprec = -1;  // precision the sample buffers were allocated for
   int enc_precision = -1;  // precision xenc currently holds
   int dpad = 0;
-  // Cluster-residual copy (cfg.residual_clusters > 0, BF16 / FP8): rk
-  // centroids rcent [rk][dim], built for the first rn rows; rpad padded rows
-  // (rid: original row or -1), rtcl: centroid per tile, rslices slices
-  // (rslice_t0 [rslices + 1], rslice_cl); renc / rscale / rnh: the residual
-  // rows for precision rprec.
+  // Cluster-residual copies (cfg.residual_clusters > 0, BF16 / FP8): rk
+  // centroids rcent [rk][dim], built for the first rn rows; copies of
+  // sampling stride 1, 8, 64, ... (rows: original row or -1 for padding,
+  // tcl: centroid per tile, slices of one centroid), their residual rows for
+  // precision rprec.
+  struct ResidCopy {
+    int64_t stride = 1, rows = 0, npad = 0, slices = 0;
+    int32_t* rid = nullptr;
+    int32_t* tcl = nullptr;
+    int32_t* slice_t0 = nullptr;
+    int32_t* slice_cl = nullptr;
+    void* enc = nullptr;
+    float* scale = nullptr;
+    float* nh = nullptr;
+  };
+  std::vector<ResidCopy> rcopies;
   float* rcent = nullptr;
-  int32_t* rid = nullptr;
-  int32_t* rtcl = nullptr;
-  int32_t* rslice_t0 = nullptr;
-  int32_t* rslice_cl = nullptr;
-  void* renc = nullptr;
-  float* rscale = nullptr;
-  float* rnh = nullptr;
   int rk = 0;
-  int64_t rn = 0, rpad = 0, rslices = 0;
+  int64_t rn = 0;
   int rprec = -1;
   std::vector<rplsh::Workspace> ws;
   rplsh::DevBuf qdev, idev, ddev;

@@ -357,7 +357,7 @@ void persist_state_destroy(PersistState* s) { delete s; }
 
 bool persist_supported(const DbView& db, const SearchParams& p) {
   // The cluster-residual copy's full level runs on the launched pipeline.
-  return db.packed && !db.renc && db.n > 0 && db.elem != Elem::F32 && db.d <= 256 &&
+  return db.packed && db.nrcopies == 0 && db.n > 0 && db.elem != Elem::F32 && db.d <= 256 &&
          persist_kernel(db.elem, db.dpad, false) != nullptr && std::min<int64_t>(p.kp, db.n) <= kPersistMaxKp;
 }
 

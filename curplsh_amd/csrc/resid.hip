@@ -24,11 +24,15 @@
 // kSliceTiles. The copy holds the residuals in the packed tile layout
 // (filter_packed.hip) with their own per-row scales (FP8) and -||r||^2/2.
 //
-// Search: the sample levels run on the regular copy (they only set
-// thresholds); the full level runs filter_resid_kernel, one workgroup per
-// (query block, slice): it forms s = q - c for its queries in registers (FP8:
-// a power-of-two scale per query and centroid), adds q.c - ||c||^2/2 to the
-// C operand and emits the candidates with their original row numbers. The
+// Besides the copy of every row, copies of every 8th, 64th, ... row of the
+// centroid order (systematic samples: each centroid keeps its share) serve
+// the sample levels, so that every level's keys -- and so the thresholds
+// handed from level to level -- come from the same residual arithmetic.
+//
+// Search: every level runs filter_resid_kernel over one copy, one workgroup
+// per (query block, slice): it forms s = q - c for its queries (FP8: a
+// power-of-two scale per query and centroid), adds q.c - ||c||^2/2 to the C
+// operand and emits the candidates with their original row numbers. The
 // exact re-rank then reads the FP32 rows as for every other form.
 #include <hip/hip_runtime.h>
 
@@ -431,13 +435,14 @@ struct ResidLaunch {
 
 struct ResidPack {
   const rplsh_bf* b;
-  int64_t tiles;
+  const rplsh_bf::ResidCopy* c;
   hipStream_t st;
   template <Elem E, int DPAD>
   int operator()() const {
     using G = Geometry<E, DPAD>;
+    const int64_t tiles = c->npad / 32;
     resid_pack_kernel<E, DPAD><<<grid_of(tiles * (G::TILE / 16), 256), 256, 0, st>>>(
-        b->x, b->ldx, b->dim, b->rcent, b->rid, b->rtcl, tiles, b->rscale, static_cast<char*>(b->renc));
+        b->x, b->ldx, b->dim, b->rcent, c->rid, c->tcl, tiles, c->scale, static_cast<char*>(c->enc));
     RPLSH_HIP(hipGetLastError());
     return 0;
   }
@@ -455,15 +460,16 @@ void launch_filter_resid(Elem e, int dpad, const ResidArgs& a, hipStream_t st) {
 // ---- host build ----------------------------------------------------------
 
 void resid_free(rplsh_bf* b) {
-  void* bufs[] = {b->rcent, b->rid, b->rtcl, b->rslice_t0, b->rslice_cl, b->renc, b->rscale, b->rnh};
-  for (void* p : bufs)
-    if (p) (void)hipFree(p);
+  for (auto& c : b->rcopies) {
+    void* bufs[] = {c.rid, c.tcl, c.slice_t0, c.slice_cl, c.enc, c.scale, c.nh};
+    for (void* p : bufs)
+      if (p) (void)hipFree(p);
+  }
+  b->rcopies.clear();
+  if (b->rcent) (void)hipFree(b->rcent);
   b->rcent = nullptr;
-  b->rid = b->rtcl = b->rslice_t0 = b->rslice_cl = nullptr;
-  b->renc = nullptr;
-  b->rscale = b->rnh = nullptr;
   b->rk = 0;
-  b->rpad = b->rslices = b->rn = 0;
+  b->rn = 0;
   b->rprec = -1;
 }
 
@@ -532,6 +538,11 @@ void build_centroids(rplsh_bf* b, int K, float* cent, hipStream_t st, L2Scratch&
 
 }  // namespace
 
+// Sampling strides of the copies: 1 (every row), then 8, 64, ... while a
+// copy keeps at least kMinSampleRows rows.
+constexpr int64_t kResidStrideStep = 8;
+constexpr int64_t kMinSampleRows = 2048;
+
 void resid_build(rplsh_bf* b, hipStream_t st) {
   const Elem e = b->cfg.precision == RPLSH_PREC_FP8 ? Elem::FP8 : Elem::BF16;
   const int d = b->dim;
@@ -541,86 +552,98 @@ void resid_build(rplsh_bf* b, hipStream_t st) {
   RPLSH_REQUIRE(K > 0 && n > 0, "residual copy: no rows or no centroids");
 
   if (b->rn != n || b->rk != K) {
-    // Layout: centroids, assignment, cluster-pure tiles and slices.
+    // Centroids and the assignment of every row.
     resid_free(b);
     RPLSH_HIP(hipMalloc((void**)&b->rcent, (size_t)K * d * sizeof(float)));
     L2Scratch sc;
     sc.wss.resize(4);
     build_centroids(b, K, b->rcent, st, sc);
-
-    // Nearest centroid of every row (plain FP32 search in batches).
     constexpr int64_t kBatch = 65536;
     std::vector<int32_t> assign((size_t)n);
-    DevBuf qb, ai, ad;
-    int32_t* dai = ai.as<int32_t>((size_t)kBatch);
-    float* dad = ad.as<float>((size_t)kBatch);
-    for (int64_t r0 = 0; r0 < n; r0 += kBatch) {
-      const int64_t m = std::min(kBatch, n - r0);
-      const float* qp = b->x + r0 * b->ldx;
-      if (b->ldx != d) {
-        float* t = qb.as<float>((size_t)kBatch * d);
-        launch_pad_rows(qp, m, d, b->ldx, d, t, st);
-        qp = t;
+    {
+      DevBuf qb, ai, ad;
+      int32_t* dai = ai.as<int32_t>((size_t)kBatch);
+      float* dad = ad.as<float>((size_t)kBatch);
+      for (int64_t r0 = 0; r0 < n; r0 += kBatch) {
+        const int64_t m = std::min(kBatch, n - r0);
+        const float* qp = b->x + r0 * b->ldx;
+        if (b->ldx != d) {
+          float* t = qb.as<float>((size_t)kBatch * d);
+          launch_pad_rows(qp, m, d, b->ldx, d, t, st);
+          qp = t;
+        }
+        search_l2_device(b->rcent, K, d, nullptr, qp, m, 1, dai, dad, 0, st, sc);
+        RPLSH_HIP(hipMemcpy(assign.data() + r0, dai, (size_t)m * 4, hipMemcpyDeviceToHost));
       }
-      search_l2_device(b->rcent, K, d, nullptr, qp, m, 1, dai, dad, 0, st, sc);
-      RPLSH_HIP(hipMemcpy(assign.data() + r0, dai, (size_t)m * 4, hipMemcpyDeviceToHost));
     }
-
-    // Stable counting sort by centroid into whole tiles.
-    std::vector<int64_t> cnt((size_t)K, 0), tile0((size_t)K + 1, 0);
+    // Rows in centroid order (stable: original order within a centroid).
+    std::vector<int64_t> first((size_t)K + 1, 0);
     for (int64_t r = 0; r < n; ++r) {
       const int c = assign[(size_t)r];
       RPLSH_REQUIRE(c >= 0 && c < K, "residual copy: row %lld has no centroid", (long long)r);
-      cnt[(size_t)c] += 1;
+      first[(size_t)c + 1] += 1;
     }
-    for (int c = 0; c < K; ++c) tile0[(size_t)c + 1] = tile0[(size_t)c] + div_up(cnt[(size_t)c], 32);
-    const int64_t tiles = tile0[(size_t)K];
-    const int64_t npad = tiles * 32;
-    RPLSH_REQUIRE(npad < (1ll << 31), "residual copy: too many rows");
-    std::vector<int32_t> rid((size_t)npad, -1), tcl((size_t)tiles);
-    std::vector<int64_t> fill((size_t)K, 0);
-    for (int64_t r = 0; r < n; ++r) {
-      const int c = assign[(size_t)r];
-      rid[(size_t)(tile0[(size_t)c] * 32 + fill[(size_t)c]++)] = (int32_t)r;
+    for (int c = 0; c < K; ++c) first[(size_t)c + 1] += first[(size_t)c];
+    std::vector<int32_t> order((size_t)n);
+    {
+      std::vector<int64_t> fill(first.begin(), first.end() - 1);
+      for (int64_t r = 0; r < n; ++r) order[(size_t)fill[(size_t)assign[(size_t)r]]++] = (int32_t)r;
     }
-    std::vector<int32_t> s_t0, s_cl;
-    for (int c = 0; c < K; ++c) {
-      for (int64_t t = tile0[(size_t)c]; t < tile0[(size_t)c + 1]; t += kSliceTiles) {
-        s_t0.push_back((int32_t)t);
-        s_cl.push_back(c);
+    // One copy per stride: positions p = 0, S, 2S, ... of the centroid order
+    // (a systematic sample, so each centroid keeps its share of the rows),
+    // packed into whole tiles of one centroid, slices of <= kSliceTiles.
+    for (int64_t S = 1; b->rcopies.size() < (size_t)DbView::kMaxResidCopies; S *= kResidStrideStep) {
+      if (S > 1 && n / S < kMinSampleRows) break;
+      rplsh_bf::ResidCopy cp;
+      cp.stride = S;
+      std::vector<int32_t> rid, tcl, s_t0, s_cl;
+      for (int c = 0; c < K; ++c) {
+        const int64_t p0 = div_up(first[(size_t)c], S) * S;  // first sampled position in this centroid
+        const size_t tile_begin = tcl.size();
+        for (int64_t p = p0; p < first[(size_t)c + 1]; p += S) rid.push_back(order[(size_t)p]);
+        while (rid.size() % 32) rid.push_back(-1);
+        while (tcl.size() * 32 < rid.size()) tcl.push_back(c);
+        for (size_t t = tile_begin; t < tcl.size(); t += kSliceTiles) {
+          s_t0.push_back((int32_t)t);
+          s_cl.push_back(c);
+        }
       }
-      for (int64_t t = tile0[(size_t)c]; t < tile0[(size_t)c + 1]; ++t) tcl[(size_t)t] = c;
+      s_t0.push_back((int32_t)tcl.size());
+      cp.rows = 0;
+      for (int32_t v : rid) cp.rows += v >= 0;
+      cp.npad = (int64_t)rid.size();
+      cp.slices = (int64_t)s_cl.size();
+      RPLSH_REQUIRE(cp.npad > 0 && cp.npad < (1ll << 31), "residual copy: bad size");
+      RPLSH_HIP(hipMalloc((void**)&cp.rid, rid.size() * 4));
+      RPLSH_HIP(hipMalloc((void**)&cp.tcl, tcl.size() * 4));
+      RPLSH_HIP(hipMalloc((void**)&cp.slice_t0, s_t0.size() * 4));
+      RPLSH_HIP(hipMalloc((void**)&cp.slice_cl, s_cl.size() * 4));
+      RPLSH_HIP(hipMalloc((void**)&cp.nh, rid.size() * 4));
+      b->rcopies.push_back(cp);  // owned from here on (freed by resid_free)
+      RPLSH_HIP(hipMemcpy(cp.rid, rid.data(), rid.size() * 4, hipMemcpyHostToDevice));
+      RPLSH_HIP(hipMemcpy(cp.tcl, tcl.data(), tcl.size() * 4, hipMemcpyHostToDevice));
+      RPLSH_HIP(hipMemcpy(cp.slice_t0, s_t0.data(), s_t0.size() * 4, hipMemcpyHostToDevice));
+      RPLSH_HIP(hipMemcpy(cp.slice_cl, s_cl.data(), s_cl.size() * 4, hipMemcpyHostToDevice));
     }
-    s_t0.push_back((int32_t)tiles);
-    RPLSH_HIP(hipMalloc((void**)&b->rid, (size_t)npad * 4));
-    RPLSH_HIP(hipMalloc((void**)&b->rtcl, (size_t)std::max<int64_t>(tiles, 1) * 4));
-    RPLSH_HIP(hipMalloc((void**)&b->rslice_t0, s_t0.size() * 4));
-    RPLSH_HIP(hipMalloc((void**)&b->rslice_cl, std::max<size_t>(s_cl.size(), 1) * 4));
-    RPLSH_HIP(hipMemcpy(b->rid, rid.data(), (size_t)npad * 4, hipMemcpyHostToDevice));
-    RPLSH_HIP(hipMemcpy(b->rtcl, tcl.data(), (size_t)tiles * 4, hipMemcpyHostToDevice));
-    RPLSH_HIP(hipMemcpy(b->rslice_t0, s_t0.data(), s_t0.size() * 4, hipMemcpyHostToDevice));
-    RPLSH_HIP(hipMemcpy(b->rslice_cl, s_cl.data(), s_cl.size() * 4, hipMemcpyHostToDevice));
-    RPLSH_HIP(hipMalloc((void**)&b->rnh, (size_t)npad * 4));
     b->rk = K;
     b->rn = n;
-    b->rpad = npad;
-    b->rslices = (int64_t)s_cl.size();
     b->rprec = -1;
   }
 
   if (b->rprec != b->cfg.precision) {
-    // Residual rows for the current element kind.
-    if (b->renc) (void)hipFree(b->renc);
-    if (b->rscale) (void)hipFree(b->rscale);
-    b->renc = nullptr;
-    b->rscale = nullptr;
-    const int64_t tiles = b->rpad / 32;
-    RPLSH_HIP(hipMalloc(&b->renc, (size_t)(tiles * packed_tile_bytes(e, dpad))));
-    if (e == Elem::FP8) RPLSH_HIP(hipMalloc((void**)&b->rscale, (size_t)b->rpad * 4));
-    resid_rows_kernel<<<grid_of(b->rpad, 4), 256, 0, st>>>(b->x, b->ldx, d, b->rcent, b->rid, b->rtcl, b->rpad,
-                                                             b->rnh, b->rscale);
-    RPLSH_HIP(hipGetLastError());
-    resid_dispatch(e, dpad, ResidPack{b, tiles, st});
+    // Residual rows of every copy for the current element kind.
+    for (auto& cp : b->rcopies) {
+      if (cp.enc) (void)hipFree(cp.enc);
+      if (cp.scale) (void)hipFree(cp.scale);
+      cp.enc = nullptr;
+      cp.scale = nullptr;
+      RPLSH_HIP(hipMalloc(&cp.enc, (size_t)(cp.npad / 32 * packed_tile_bytes(e, dpad))));
+      if (e == Elem::FP8) RPLSH_HIP(hipMalloc((void**)&cp.scale, (size_t)cp.npad * 4));
+      resid_rows_kernel<<<grid_of(cp.npad, 4), 256, 0, st>>>(b->x, b->ldx, d, b->rcent, cp.rid, cp.tcl, cp.npad,
+                                                               cp.nh, cp.scale);
+      RPLSH_HIP(hipGetLastError());
+      resid_dispatch(e, dpad, ResidPack{b, &cp, st});
+    }
     RPLSH_HIP(hipStreamSynchronize(st));
     b->rprec = b->cfg.precision;
   }

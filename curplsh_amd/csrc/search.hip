@@ -142,6 +142,49 @@ static int64_t level_cap(const std::vector<Level>& L, int kp, int c_factor) {
   return cap;
 }
 
+// Levels over the cluster-residual copies: the sparsest copy first (every
+// row kept: the threshold is +inf), then each denser sample copy down to the
+// last one, the sparsest whose threshold rank target / stride is at least the
+// final rank, then the copy of every row. Intermediate thresholds take rank
+// kRhoMid (copies 8 times denser: about 128 survivors), the last sample
+// level's the rank that leaves about the survivor target at the full level.
+struct ResidLevel {
+  int copy;
+  int rank_next;  // rank of the threshold for the next level; 0 on the last
+};
+static std::vector<ResidLevel> resid_levels(const DbView& db, int kp, int c_factor, int rho_final) {
+  constexpr int kRhoMid = 16;
+  const int64_t target = survivor_target(kp, c_factor);
+  const int rf = final_rank(kp, c_factor, rho_final);
+  int last = 0;
+  for (int j = db.nrcopies - 1; j >= 1; --j)
+    if (db.rcopy[j].stride * rf <= target) {
+      last = j;
+      break;
+    }
+  std::vector<ResidLevel> L;
+  if (last > 0) {
+    for (int j = db.nrcopies - 1; j >= last; --j) {
+      const auto& c = db.rcopy[j];
+      const int rank = j > last ? kRhoMid : (int)std::min<int64_t>(c.rows, div_up(target, c.stride));
+      L.push_back({j, std::max(1, rank)});
+    }
+  }
+  L.push_back({0, 0});
+  return L;
+}
+
+// Candidate slots per query of a batch (every level's survivors fit, the
+// first level's all of its rows).
+static int64_t batch_cap(const DbView& db, int kp, const SearchParams& p) {
+  if (db.nrcopies > 0) {
+    const std::vector<ResidLevel> L = resid_levels(db, kp, p.c_factor, p.rho);
+    int64_t cap = std::max<int64_t>(4096, 4 * survivor_target(kp, p.c_factor));
+    return std::max<int64_t>(cap, db.rcopy[L.front().copy].rows);
+  }
+  return level_cap(plan_levels(db.n, kp, p.c_factor, p.rho), kp, p.c_factor);
+}
+
 // One filter pass. A sample level (stride > 1) of a packed index whose sample
 // copy holds at least lv.m rows reads the first lv.m rows of that copy
 // instead of every stride-th row (its candidates only set a threshold, their
@@ -214,18 +257,19 @@ static void run_filter(const DbView& db, const Level& lv, bool dense, const void
   }
 }
 
-// The full level over the cluster-residual copy: every padded row, one
-// workgroup per (query block, slice); the lists start empty.
-static void run_filter_resid(const DbView& db, const float* q, const float* tau, int nq, const int* nq_dev,
-                             int cap, int* counts, uint64_t* cand, hipStream_t st) {
+// One level over a cluster-residual copy: every padded row, one workgroup
+// per (query block, slice); the lists start empty.
+static void run_filter_resid(const DbView& db, int copy, const float* q, const float* tau, int nq,
+                             const int* nq_dev, int cap, int* counts, uint64_t* cand, hipStream_t st) {
+  const DbView::ResidView& c = db.rcopy[copy];
   ResidArgs ra{};
-  ra.x = db.renc;
-  ra.xscale = db.rscale;
-  ra.xnh = db.rnh;
-  ra.rid = db.rid;
-  ra.slice_t0 = db.rslice_t0;
-  ra.slice_cl = db.rslice_cl;
-  ra.num_slices = (int)db.rslices;
+  ra.x = c.enc;
+  ra.xscale = c.scale;
+  ra.xnh = c.nh;
+  ra.rid = c.rid;
+  ra.slice_t0 = c.slice_t0;
+  ra.slice_cl = c.slice_cl;
+  ra.num_slices = (int)c.slices;
   ra.cent = db.rcent;
   ra.d = db.d;
   ra.q = q;
@@ -251,7 +295,6 @@ static int64_t carried_tiles(const DbView& db, const std::vector<Level>& L, size
   if (li == 0 || li >= L.size()) return 0;
   const Level& lp = L[li - 1];
   const Level& lv = L[li];
-  if (db.renc && lv.stride == 1) return 0;  // the residual full level scores every row
   const bool lds = variant == kFilterLdsWide || variant == kFilterLdsNarrow || variant == kFilterLdsWide8;
   if (!lds || !db.packed || !db.trank || lp.stride <= 1 || db.srows < lp.m) return 0;
   if (lv.stride == 1) return lp.m / 32;                       // into the full level
@@ -269,10 +312,13 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
                    hipStream_t st, PhaseEvents* pe) {
   const int d = db.d;
   const int kp = (int)std::min<int64_t>(p.kp, db.n);
-  const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
-  const int64_t cap64 = level_cap(L, kp, p.c_factor);
+  const bool resid = db.nrcopies > 0;
+  const std::vector<Level> L = resid ? std::vector<Level>{} : plan_levels(db.n, kp, p.c_factor, p.rho);
+  const std::vector<ResidLevel> RL = resid ? resid_levels(db, kp, p.c_factor, p.rho) : std::vector<ResidLevel>{};
+  const int64_t cap64 = batch_cap(db, kp, p);
   RPLSH_REQUIRE(cap64 < (1ll << 31), "candidate capacity overflow");
   const int cap = (int)cap64;
+  const size_t nlevels = resid ? RL.size() : L.size();
   auto rec = [&](int i) {
     if (pe) pe->rec(i, st);
   };
@@ -307,16 +353,19 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
   launch_fill_f32(tau, nq, INFINITY, st);
   launch_fill_i32(status, nq, 0, st);
   rec(0);
-  for (size_t li = 0; li < L.size(); ++li) {
-    const Level& lv = L[li];
-    const bool last = li + 1 == L.size();
+  for (size_t li = 0; li < nlevels; ++li) {
+    const bool last = li + 1 == nlevels;
     if (last) rec(1);
-    const int64_t carried = carried_tiles(db, L, li, p.variant);
-    if (last && li > 0 && db.renc)
-      run_filter_resid(db, q, tau, nq, nq_dev, cap, counts, cand, st);
-    else
-      run_filter(db, lv, li == 0, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs,
+    int rank_next;
+    if (resid) {
+      run_filter_resid(db, RL[li].copy, q, tau, nq, nq_dev, cap, counts, cand, st);
+      rank_next = RL[li].rank_next;
+    } else {
+      const int64_t carried = carried_tiles(db, L, li, p.variant);
+      run_filter(db, L[li], li == 0, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs,
                  p.variant, last ? carried : 0, last ? 0 : 32 * carried, st);
+      rank_next = L[li].rank_next;
+    }
     if (last) rec(2);
     SelectArgs sa{};
     sa.cand = cand;
@@ -325,14 +374,14 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
     sa.nq = nq;
     sa.nq_dev = nq_dev;
     sa.tau_in = tau;
-    if (lv.rank_next > 0) {
-      sa.K = lv.rank_next;
+    if (rank_next > 0) {
+      sa.K = rank_next;
       sa.topk_mode = 0;
       sa.tau_out = tau2;
       sa.status = nullptr;  // a loose intermediate threshold only costs time
       launch_select(sa, st);
       std::swap(tau, tau2);
-      const int64_t carry_next = carried_tiles(db, L, li + 1, p.variant);
+      const int64_t carry_next = resid ? 0 : carried_tiles(db, L, li + 1, p.variant);
       if (carry_next > 0) {
         CarryArgs ca{};
         ca.cand = cand;
@@ -342,7 +391,7 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
         ca.nq_dev = nq_dev;
         ca.tau = tau;
         ca.row_limit = 32 * carry_next;
-        const bool into_full = li + 2 == L.size();
+        const bool into_full = li + 2 == nlevels;
         ca.step = into_full ? db.sstep : 0;
         ca.tiles = db.stiles;
         ca.status = status;
@@ -546,11 +595,14 @@ void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& 
   const long long nsurv = hc->survivors;
   stats.boundary_fp64 += nfp64;
   if (depth == 0) {
-    const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
-    stats.levels = (int32_t)L.size();
-    stats.rows_filter_full = db.renc && L.size() > 1
-                                 ? db.rpad
-                                 : L.back().m - 32 * carried_tiles(db, L, L.size() - 1, p.variant);
+    if (db.nrcopies > 0) {
+      stats.levels = (int32_t)resid_levels(db, kp, p.c_factor, p.rho).size();
+      stats.rows_filter_full = db.rcopy[0].npad;
+    } else {
+      const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
+      stats.levels = (int32_t)L.size();
+      stats.rows_filter_full = L.back().m - 32 * carried_tiles(db, L, L.size() - 1, p.variant);
+    }
     stats.filter_variant = p.variant;
     stats.target_wgs = (int32_t)p.target_wgs;
     if (pe && pe->on) {
@@ -598,8 +650,7 @@ static int plain_kprime(int k) { return std::max(2 * k, k + 32); }
 
 size_t search_workspace_bytes(const DbView& db, int nq, const SearchParams& p) {
   const int kp = (int)std::min<int64_t>(p.kp, db.n);
-  const std::vector<Level> L = plan_levels(db.n, kp, p.c_factor, p.rho);
-  const size_t cap = (size_t)level_cap(L, kp, p.c_factor);
+  const size_t cap = (size_t)batch_cap(db, kp, p);
   const size_t eb = db.elem == Elem::F32 ? 4 : (db.elem == Elem::BF16 ? 2 : 1);
   size_t b = 0;
   b += (size_t)nq * 4 * 9;                 // norms, scales, tau x2, counts, bound, status, relk, list
@@ -766,16 +817,23 @@ static DbView bf_view(const rplsh_bf* b) {
     v.trank = b->senc ? b->trank : nullptr;
     v.rank_tiles = div_up(b->n, 32);
     v.packed = true;
-    if (b->renc && b->rprec == b->cfg.precision && b->rn == b->n) {
-      v.renc = b->renc;
-      v.rscale = b->rscale;
-      v.rnh = b->rnh;
-      v.rid = b->rid;
-      v.rslice_t0 = b->rslice_t0;
-      v.rslice_cl = b->rslice_cl;
+    if (!b->rcopies.empty() && b->rprec == b->cfg.precision && b->rn == b->n) {
+      v.nrcopies = (int)b->rcopies.size();
       v.rcent = b->rcent;
-      v.rslices = b->rslices;
-      v.rpad = b->rpad;
+      for (int j = 0; j < v.nrcopies; ++j) {
+        const auto& c = b->rcopies[(size_t)j];
+        auto& r = v.rcopy[j];
+        r.enc = c.enc;
+        r.scale = c.scale;
+        r.nh = c.nh;
+        r.rid = c.rid;
+        r.slice_t0 = c.slice_t0;
+        r.slice_cl = c.slice_cl;
+        r.slices = c.slices;
+        r.npad = c.npad;
+        r.rows = c.rows;
+        r.stride = c.stride;
+      }
     }
   }
   return v;
