@@ -63,6 +63,8 @@ def main() -> int:
                          "256 centres U[-1,1) + N(0, 0.1^2) (seed 2, queries seed 30), or 1024 centres "
                          "+ N(0, 0.05^2) with 5%% exact duplicate rows (seed 6, queries seed 7)")
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp8", "fp32"])
+    ap.add_argument("--residual-clusters", type=int, default=0,
+                    help="BF16/FP8: centroids of the cluster-residual copy (0: off)")
     ap.add_argument("--forms", default="bf16,fp32,fp8",
                     help="forms to time (the --precision form is the headline value)")
     ap.add_argument("--plain-steps", type=int, default=3)
@@ -131,7 +133,8 @@ def main() -> int:
         forms.insert(0, args.precision)
     forms.sort(key=lambda f: f != args.precision)
 
-    idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=args.precision, kprime_factor=args.kprime_factor))
+    idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=args.precision, kprime_factor=args.kprime_factor,
+                                              residual_clusters=args.residual_clusters))
     t0 = time.time()
     idx.add(x)
     log(f"[bench] index built (add incl. H2D, norms, encode) in {time.time() - t0:.1f}s")
@@ -287,6 +290,7 @@ def main() -> int:
             "workload": f"{n}x{d} database ({args.data}), {nq} queries, k={k}, {form} candidates k'={args.kprime_factor}k + FP32 re-rank"
                         + (" + FP64 boundary" if form != "fp32" else ""),
             "n": n, "d": d, "nq": nq, "k": k, "precision": form, "kprime": args.kprime_factor * k if form != "fp32" else None,
+            "residual_clusters": args.residual_clusters,
         },
         "roofline": {
             "bound": "mfma",

@@ -75,6 +75,19 @@ struct FilterArgs {
   // First logical row (a multiple of 32): a sample level after a carried one
   // scores rows [row_begin, m) only; the slices cover m - row_begin rows.
   int64_t row_begin;
+  // Cluster-residual copies (LDS kernels with the residual flag, resid.hip):
+  // x holds r = x - c in tiles of one centroid, xnorm holds -||r||^2 / 2
+  // (-inf: padding row), xscale the scales of r; slice s covers tiles
+  // [slice_t0[s], slice_t0[s + 1]) of centroid slice_cl[s]; the queries are
+  // FP32 (qf32, ld ldq, d elements) and are encoded relative to the slice's
+  // centroid in the kernel; rid maps a row to the index row it holds.
+  const int32_t* rid;
+  const int32_t* slice_t0;
+  const int32_t* slice_cl;
+  const float* cent;
+  const float* qf32;
+  int ldq;
+  int d;
 };
 
 // Sample copy of a packed index: whole 32-row tiles, sample tile s holding
@@ -84,29 +97,6 @@ struct FilterArgs {
 __host__ __device__ inline int64_t sample_source_row(int64_t r, int64_t step, int64_t tiles) {
   return (int64_t)(((uint64_t)(r >> 5) * (uint64_t)step) % (uint64_t)tiles) * 32 + (r & 31);
 }
-
-// Full level over the cluster-residual copy (resid.hip): 32-row tiles of
-// one centroid each, grouped into slices of one centroid.
-struct ResidArgs {
-  const void* x;            // residual rows r = x - c, packed tiles
-  const float* xscale;      // [npad] FP8 per-row scales of r
-  const float* xnh;         // [npad] -||r||^2 / 2 (-inf: padding row)
-  const int32_t* rid;       // [npad] original row (-1: padding)
-  const int32_t* slice_t0;  // [num_slices + 1] first tile of each slice
-  const int32_t* slice_cl;  // [num_slices] centroid of each slice
-  int num_slices;
-  const float* cent;        // [K][d] centroids (FP32)
-  int d;
-  const float* q;           // FP32 queries [nq][ldq]
-  int ldq;
-  const float* tau;         // [nq] threshold on the approximate relative distance
-  int nq;
-  const int* nq_dev;        // if set: the number of live queries (<= nq)
-  int num_qtiles;
-  int cap;
-  int* counts;
-  uint64_t* cand;
-};
 
 // Carry of a sample level's survivors into the next level's lists.
 struct CarryArgs {

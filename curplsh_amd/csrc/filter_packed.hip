@@ -339,6 +339,105 @@ __device__ __forceinline__ void stage_hits(const f32x16 (&acc)[TN], const float 
   }
 }
 
+// Query operands of a workgroup over a cluster-residual slice of centroid
+// `cl`: s = q - c for query tile nt of each wave (queries q0 + (wave * TN +
+// nt) * 32 + col), encoded as a query row (BF16: round to nearest even; FP8:
+// a power-of-two scale per query from max |s|, which keeps every value within
+// +-448, then e4m3 by the hardware conversion, round to nearest even), and
+// cq = q.c - ||c||^2/2 (FP32, each lane its half of the row in element
+// order, the halves added in a fixed order). A lane's fragment elements are
+// exactly its half of its query's row (elements c * KC + h * KC/2 + [0,
+// KC/2) of chunk c): it loads those elements of q (all loads issued before
+// use), takes c from the LDS copy of the centroid, writes the encoded bytes
+// to its own place in an LDS row buffer and reads its fragments back from
+// there, so no lane waits for another. The LDS area (the staging buffers,
+// free before the stage loop, or a buffer of the kernel's own) holds the
+// centroid (DPAD floats) and 32 encoded rows per wave. Every thread of the
+// workgroup calls it.
+template <Elem E, int DPAD, int TN, int NW>
+__device__ __forceinline__ void resid_query_operands(const FilterArgs& a, int q0, int cl, int nq, int wave,
+                                                     int lane, char* area,
+                                                     Frag<PackTraits<E>::kLaneBytes> (&qf)[TN][Geometry<E, DPAD>::NC],
+                                                     float (&thr)[TN], int (&qsc)[TN], int (&qrow)[TN],
+                                                     float (&cq)[TN]) {
+  using G = Geometry<E, DPAD>;
+  constexpr int LB = G::LB;
+  constexpr int NC = G::NC;
+  constexpr int KC = PackTraits<E>::kChunkElems;
+  constexpr int HALF = KC / 2;
+  constexpr int NV = HALF / 4;  // 4-element groups per chunk and lane
+  const int h = lane >> 5, col = lane & 31;
+  const int d = a.d;
+  float* cen = reinterpret_cast<float*>(area);
+  char* erow = area + DPAD * 4 + (wave * 32 + col) * G::ROWB;
+  const float* cr = a.cent + (int64_t)cl * d;
+  for (int k = threadIdx.x; k < DPAD; k += NW * 64) cen[k] = k < d ? cr[k] : 0.0f;
+  __syncthreads();
+  const bool vec = (d & 3) == 0 && (a.ldq & 3) == 0;  // 16-byte aligned rows
+#pragma unroll
+  for (int nt = 0; nt < TN; ++nt) {
+    const int qr = q0 + (wave * TN + nt) * 32 + col;
+    qrow[nt] = qr;
+    const bool live = qr < nq;
+    const float* qp = a.qf32 + (int64_t)(live ? qr : 0) * a.ldq;
+    f32x4 qv[NC][NV];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const int k0 = c * KC + h * HALF + 4 * v;
+        if (vec) {
+          qv[c][v] = k0 < d ? *reinterpret_cast<const f32x4*>(qp + k0) : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) qv[c][v][u] = k0 + u < d ? qp[k0 + u] : 0.0f;
+        }
+      }
+    float mx = 0.0f, dot = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const int k0 = c * KC + h * HALF + 4 * v;
+        const f32x4 cv = *reinterpret_cast<const f32x4*>(cen + k0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          mx = fmaxf(mx, fabsf(qv[c][v][u] - cv[u]));  // zero past d (both are)
+          dot = fmaf(cv[u], qv[c][v][u] - 0.5f * cv[u], dot);
+        }
+      }
+    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+    const float other = __shfl_xor(dot, 32, 64);
+    dot = h == 0 ? dot + other : other + dot;
+    const float sc = E == Elem::FP8 ? e4m3_row_scale(mx) : 1.0f;
+    const float inv = 1.0f / sc;  // a power of two: exact
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const int k0 = c * KC + h * HALF + 4 * v;
+        const f32x4 sv = qv[c][v] - *reinterpret_cast<const f32x4*>(cen + k0);
+        if constexpr (E == Elem::FP8) {
+          int w = __builtin_amdgcn_cvt_pk_fp8_f32(sv[0] * inv, sv[1] * inv, 0, false);
+          w = __builtin_amdgcn_cvt_pk_fp8_f32(sv[2] * inv, sv[3] * inv, w, true);
+          *reinterpret_cast<int*>(erow + k0) = w;
+        } else {
+          const uint32_t w0 = (uint32_t)bf16_rne(sv[0]) | ((uint32_t)bf16_rne(sv[1]) << 16);
+          const uint32_t w1 = (uint32_t)bf16_rne(sv[2]) | ((uint32_t)bf16_rne(sv[3]) << 16);
+          *reinterpret_cast<uint32_t*>(erow + 2 * k0) = w0;
+          *reinterpret_cast<uint32_t*>(erow + 2 * k0 + 4) = w1;
+        }
+      }
+    // The lane reads back only the bytes it wrote.
+#pragma unroll
+    for (int c = 0; c < NC; ++c) qf[nt][c] = live ? load_frag<LB>(erow + 2 * LB * c + LB * h) : zero_frag<LB>();
+    thr[nt] = live ? fmaxf(-0.5f * a.tau[qr], -FLT_MAX) : INFINITY;
+    cq[nt] = live ? dot : 0.0f;
+    qsc[nt] = E == Elem::FP8 ? e8m0_of(sc) : 127;
+  }
+  __syncthreads();  // the area becomes the staging buffers
+}
+
 template <Elem E, int DPAD, bool WIDE, int NW>
 struct StageShape {
   using G = Geometry<E, DPAD>;
@@ -355,7 +454,12 @@ struct StageShape {
   static constexpr int WAVES_PER_SIMD = (NW == 4 && !WIDE && 3 * LDS <= 160 * 1024) ? 3 : 2;
 };
 
-template <Elem E, int DPAD, int TN, bool WIDE, int NW = 4>
+// RESID: a level over a cluster-residual copy (resid.hip): the slice is a
+// tile range of one centroid c, the query operands are s = q - c encoded in
+// the kernel (FP8: a power-of-two scale per query and centroid), the C
+// operand adds q.c - ||c||^2/2 to the copy's -||r||^2/2, and survivors carry
+// the index row the residual row holds.
+template <Elem E, int DPAD, int TN, bool WIDE, int NW = 4, bool RESID = false>
 __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_SIMD)) void filter_lds_kernel(
     FilterArgs a) {
   using G = Geometry<E, DPAD>;
@@ -392,9 +496,15 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
   const int slice = logical / a.num_qtiles;
   const int nq = a.nq_dev ? min(*a.nq_dev, a.nq) : a.nq;
   if (qtile * BQ >= nq) return;
-  const int64_t r_begin = a.row_begin + (int64_t)slice * a.rows_per_slice;
-  int64_t r_end = r_begin + a.rows_per_slice;
-  if (r_end > a.m) r_end = a.m;
+  int64_t r_begin, r_end;
+  if constexpr (RESID) {
+    r_begin = 32 * (int64_t)a.slice_t0[slice];
+    r_end = 32 * (int64_t)a.slice_t0[slice + 1];
+  } else {
+    r_begin = a.row_begin + (int64_t)slice * a.rows_per_slice;
+    r_end = r_begin + a.rows_per_slice;
+    if (r_end > a.m) r_end = a.m;
+  }
   if (r_begin >= r_end) return;
 
   const int tid = threadIdx.x;
@@ -406,21 +516,34 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
   float thr[TN];
   int qsc[TN];
   int qrow[TN];
+  float cq[TN];  // RESID: q.c - ||c||^2/2
+  if constexpr (RESID) {
+    // The centroid and every wave's encoded rows: in the staging buffers, or
+    // (wide rows, one-tile stages) in a buffer of their own.
+    constexpr int NEED = DPAD * 4 + NW * 32 * G::ROWB;
+    constexpr bool OWN = NEED > 2 * STAGE;
+    __shared__ __attribute__((aligned(16))) char rarea[OWN ? NEED : 16];
+    char* area = OWN ? rarea : &sx[0][0];
+    resid_query_operands<E, DPAD, TN, NW>(a, qtile * BQ, a.slice_cl[slice], nq, wave, lane, area, qf, thr,
+                                          qsc, qrow, cq);
+  } else {
 #pragma unroll
-  for (int nt = 0; nt < TN; ++nt) {
-    const int qr = qtile * BQ + (wave * TN + nt) * 32 + col;
-    qrow[nt] = qr;
-    if (qr < nq) {
-      const char* qp = reinterpret_cast<const char*>(a.q) + (int64_t)qr * G::ROWB;
+    for (int nt = 0; nt < TN; ++nt) {
+      const int qr = qtile * BQ + (wave * TN + nt) * 32 + col;
+      qrow[nt] = qr;
+      cq[nt] = 0.0f;
+      if (qr < nq) {
+        const char* qp = reinterpret_cast<const char*>(a.q) + (int64_t)qr * G::ROWB;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) qf[nt][c] = load_frag<LB>(qp + 2 * LB * c + LB * h);
-      thr[nt] = fmaxf(-0.5f * a.tau[qr], -FLT_MAX);
-      qsc[nt] = E == Elem::FP8 ? e8m0_of(a.qscale[qr]) : 127;
-    } else {
+        for (int c = 0; c < NC; ++c) qf[nt][c] = load_frag<LB>(qp + 2 * LB * c + LB * h);
+        thr[nt] = fmaxf(-0.5f * a.tau[qr], -FLT_MAX);
+        qsc[nt] = E == Elem::FP8 ? e8m0_of(a.qscale[qr]) : 127;
+      } else {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) qf[nt][c] = zero_frag<LB>();
-      thr[nt] = INFINITY;
-      qsc[nt] = 127;
+        for (int c = 0; c < NC; ++c) qf[nt][c] = zero_frag<LB>();
+        thr[nt] = INFINITY;
+        qsc[nt] = 127;
+      }
     }
   }
 
@@ -453,7 +576,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
         pre[t] = v;
       }
       if (tid < SR) {
-        pn = -0.5f * a.xnorm[j0 + tid];
+        pn = RESID ? a.xnorm[j0 + tid] : -0.5f * a.xnorm[j0 + tid];
         pe = E == Elem::FP8 ? e8m0_of(a.xscale[j0 + tid]) : 127;
       }
       return;
@@ -480,7 +603,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
     if (tid < SR) {
       const int64_t j = j0 + tid;
       if (j < r_end) {
-        pn = -0.5f * a.xnorm[j * stride];
+        pn = RESID ? a.xnorm[j * stride] : -0.5f * a.xnorm[j * stride];
         pe = E == Elem::FP8 ? e8m0_of(a.xscale[j * stride]) : 127;
       } else {
         pn = -INFINITY;
@@ -506,7 +629,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
   // Stores a survivor at queue position t (reserved on qtail[par]); past the
   // queue's capacity it goes straight to the query's list.
   auto put = [&](int par, unsigned t, int qr, float v, int64_t row) {
-    const uint64_t w = pack_cand(-2.0f * v, (int32_t)(row * stride));
+    const uint64_t w = pack_cand(-2.0f * v, RESID ? a.rid[row] : (int32_t)(row * stride));
     if (t - head[par] < (unsigned)kQueue) {
       qword[par][t & (kQueue - 1)] = w;
       qquery[par][t & (kQueue - 1)] = qr;
@@ -566,8 +689,14 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
         for (int c = 0; c < NC; ++c) {
           const Frag<LB> af = load_frag<LB>(ap + c * G::CHUNK);
 #pragma unroll
-          for (int nt = 0; nt < TN; ++nt)
-            acc[nt] = mfma_chunk<E>(af, qf[nt][c], c == 0 ? ci : acc[nt], xsc, qsc[nt]);
+          for (int nt = 0; nt < TN; ++nt) {
+            f32x16 c0 = ci;
+            if constexpr (RESID) {
+#pragma unroll
+              for (int i = 0; i < 16; ++i) c0[i] += cq[nt];
+            }
+            acc[nt] = mfma_chunk<E>(af, qf[nt][c], c == 0 ? c0 : acc[nt], xsc, qsc[nt]);
+          }
         }
         if (!DENSE && pend != 0) {
           // The previous tiles' hits, while this tile's MFMAs run.
@@ -582,7 +711,8 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
             for (int i = 0; i < 16; ++i) {
               const int64_t row = j0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
               if (row < r_end && qrow[nt] < nq)
-                a.cand[(int64_t)qrow[nt] * a.cap + row] = pack_cand(-2.0f * acc[nt][i], (int32_t)(row * stride));
+                a.cand[(int64_t)qrow[nt] * a.cap + row] =
+                    pack_cand(-2.0f * acc[nt][i], RESID ? a.rid[row] : (int32_t)(row * stride));
             }
           }
           continue;
@@ -1190,6 +1320,32 @@ struct LaunchFilter {
   }
 };
 
+struct ResidTn {
+  bool narrow;
+  template <Elem E, int DPAD>
+  int operator()() const {
+    return narrow ? lds_tn<E, DPAD, false>() : lds_tn<E, DPAD, true>();
+  }
+};
+
+struct LaunchResid {
+  bool narrow;
+  const FilterArgs& a;
+  hipStream_t st;
+  template <Elem E, int DPAD>
+  int operator()() const {
+    const int64_t total = (int64_t)a.num_qtiles * a.num_slices;
+    RPLSH_REQUIRE(total < (1ll << 31), "residual filter grid too large");
+    const unsigned grid = (unsigned)round_up(total, 8);
+    if (narrow)
+      filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, false>(), false, 4, true><<<grid, kBlock, 0, st>>>(a);
+    else
+      filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, true>(), true, 4, true><<<grid, kBlock, 0, st>>>(a);
+    RPLSH_HIP(hipGetLastError());
+    return 0;
+  }
+};
+
 struct EncodeSample {
   const float* x;
   int64_t tiles;
@@ -1228,6 +1384,17 @@ int filter_packed_block_queries(Elem e, int dpad, int variant) {
 
 void launch_filter_packed(Elem e, int dpad, int variant, const FilterArgs& a, hipStream_t st) {
   dispatch(e, dpad, LaunchFilter{variant, a, st});
+}
+
+int filter_resid_block_queries(Elem e, int dpad, int variant) {
+  const bool narrow = variant == kFilterLdsNarrow || variant == kFilterRingNarrow;
+  return 4 * 32 * dispatch(e, dpad, ResidTn{narrow});
+}
+
+void launch_filter_resid(Elem e, int dpad, int variant, const FilterArgs& a, hipStream_t st) {
+  if (a.nq <= 0 || a.num_slices <= 0) return;
+  const bool narrow = variant == kFilterLdsNarrow || variant == kFilterRingNarrow;
+  dispatch(e, dpad, LaunchResid{narrow, a, st});
 }
 
 void launch_encode_packed(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx,

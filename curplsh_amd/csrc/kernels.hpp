@@ -75,9 +75,10 @@ void launch_encode_sample(Elem e, int dpad, const float* x, int64_t tiles, int d
 void launch_encode_packed(Elem e, int dpad, const float* x, int64_t n, int d, int64_t ldx,
                           int64_t row_from, void* out, float* scales, hipStream_t st);
 
-// resid.hip: the full level over the cluster-residual copy.
-int resid_block_queries(Elem e, int dpad);
-void launch_filter_resid(Elem e, int dpad, const ResidArgs& a, hipStream_t st);
+// A level over a cluster-residual copy (the LDS variants, wide or narrow, with
+// the residual fields of FilterArgs; any other variant runs the wide one).
+int filter_resid_block_queries(Elem e, int dpad, int variant);
+void launch_filter_resid(Elem e, int dpad, int variant, const FilterArgs& a, hipStream_t st);
 
 // select.hip
 void launch_select(const SelectArgs& a, hipStream_t st);

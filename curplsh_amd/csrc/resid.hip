@@ -29,11 +29,12 @@
 // the sample levels, so that every level's keys -- and so the thresholds
 // handed from level to level -- come from the same residual arithmetic.
 //
-// Search: every level runs filter_resid_kernel over one copy, one workgroup
-// per (query block, slice): it forms s = q - c for its queries (FP8: a
-// power-of-two scale per query and centroid), adds q.c - ||c||^2/2 to the C
-// operand and emits the candidates with their original row numbers. The
-// exact re-rank then reads the FP32 rows as for every other form.
+// Search: every level runs the LDS-staged packed filter in its residual mode
+// (filter_packed.hip) over one copy, one workgroup per (query block, slice):
+// it forms s = q - c for its queries (FP8: a power-of-two scale per query and
+// centroid), adds q.c - ||c||^2/2 to the C operand and emits the candidates
+// with their original row numbers. The exact re-rank then reads the FP32
+// rows as for every other form.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,8 +55,7 @@ namespace rplsh {
 
 namespace {
 
-constexpr int kSliceTiles = 64;  // tiles per slice (one centroid each)
-constexpr int kResidBlock = 256;
+constexpr int kSliceTiles = 128;  // tiles per slice (one centroid each)
 
 // ---- build kernels ------------------------------------------------------
 
@@ -214,178 +214,6 @@ __global__ void resid_pack_kernel(const float* __restrict__ x, int64_t ldx, int 
   }
 }
 
-// ---- the residual full-level filter ------------------------------------
-
-// Query tiles per wave: the query operands within ~128 registers (one wave
-// per SIMD, accumulators in the other half of the register file), at most 6
-// (FP8: 5, whose per-tile scale and epilogue registers leave less room).
-template <Elem E, int DPAD>
-constexpr int resid_tn() {
-  using G = Geometry<E, DPAD>;
-  constexpr int per_tile = G::NC * G::LB / 4;
-  constexpr int cap = E == Elem::FP8 ? 5 : 6;
-  constexpr int tn = 128 / per_tile;
-  return tn > cap ? cap : (tn < 1 ? 1 : tn);
-}
-
-// One workgroup per (query block, slice); the four waves take different
-// queries (TN tiles of 32 each) and every tile of the slice, so a row
-// operand fetched by one wave is an L1 hit for the others. The row operand
-// of the next tile is loaded chunk by chunk as the current tile's products on
-// that chunk are issued.
-template <Elem E, int DPAD, int TN>
-__global__ __launch_bounds__(kResidBlock, 1) void filter_resid_kernel(ResidArgs a) {
-  using G = Geometry<E, DPAD>;
-  constexpr int LB = G::LB;
-  constexpr int NC = G::NC;
-  constexpr int KC = PackTraits<E>::kChunkElems;
-  constexpr int HALF = KC / 2;  // elements of a lane's piece per chunk
-  constexpr int BQ = 4 * TN * 32;
-
-  const int per_xcd = (int)(gridDim.x >> 3);
-  const int logical = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-  if (logical >= a.num_qtiles * a.num_slices) return;
-  const int qtile = logical % a.num_qtiles;
-  const int slice = logical / a.num_qtiles;
-  const int nq = a.nq_dev ? min(*a.nq_dev, a.nq) : a.nq;
-  if (qtile * BQ >= nq) return;
-  const int64_t t_begin = a.slice_t0[slice];
-  const int64_t t_end = a.slice_t0[slice + 1];
-  const float* cr = a.cent + (int64_t)a.slice_cl[slice] * a.d;
-
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int h = lane >> 5, col = lane & 31;
-  const int d = a.d;
-
-  // Query operands s = q - c of this wave's queries, their scales (FP8) and
-  // C-operand terms q.c - ||c||^2/2. Each lane encodes its own half of its
-  // query's row into the wave's LDS row buffer (plain loops, so the encoder
-  // is not unrolled TN times), then loads its fragments from there in the
-  // layout of an encoded query row.
-  __shared__ __attribute__((aligned(16))) char qenc_lds[4][32 * G::ROWB];
-  char* my = qenc_lds[wave] + col * G::ROWB;
-  Frag<LB> qf[TN][NC];
-  float thr[TN], cq[TN];
-  int qsc[TN], qrow[TN];
-#pragma unroll
-  for (int nt = 0; nt < TN; ++nt) {
-    const int qr = qtile * BQ + (wave * TN + nt) * 32 + col;
-    qrow[nt] = qr;
-    const bool live = qr < nq;
-    const float* qp = a.q + (int64_t)(live ? qr : 0) * a.ldq;
-    float mx = 0.0f, dot = 0.0f;
-#pragma unroll 1
-    for (int c = 0; c < NC; ++c)
-      for (int i = 0; i < HALF; ++i) {
-        const int k = c * KC + h * HALF + i;
-        if (k < d) {
-          const float cv = cr[k];
-          const float qv = qp[k];
-          mx = fmaxf(mx, fabsf(qv - cv));
-          dot = fmaf(cv, qv - 0.5f * cv, dot);
-        }
-      }
-    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
-    dot += __shfl_xor(dot, 32, 64);
-    const float sc = E == Elem::FP8 ? e4m3_row_scale(mx) : 1.0f;
-#pragma unroll 1
-    for (int c = 0; c < NC; ++c)
-      for (int i = 0; i < HALF; ++i) {
-        const int k = c * KC + h * HALF + i;
-        const float v = live && k < d ? qp[k] - cr[k] : 0.0f;
-        if (E == Elem::BF16)
-          reinterpret_cast<uint16_t*>(my)[k] = bf16_rne(v);
-        else
-          reinterpret_cast<uint8_t*>(my)[k] = e4m3_rne_sat(v / sc);
-      }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < NC; ++c) qf[nt][c] = load_frag<LB>(my + 2 * LB * c + LB * h);
-    __syncthreads();  // the buffer is rewritten for the next tile of queries
-    thr[nt] = live ? fmaxf(-0.5f * a.tau[qr], -FLT_MAX) : INFINITY;
-    cq[nt] = live ? dot : 0.0f;
-    qsc[nt] = E == Elem::FP8 ? e8m0_of(sc) : 127;
-  }
-
-  const char* xb = reinterpret_cast<const char*>(a.x);
-  auto load_ci = [&](int64_t t) -> f32x16 {
-    f32x16 ci;
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const f32x4 v = *reinterpret_cast<const f32x4*>(a.xnh + t * 32 + 8 * g + 4 * h);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) ci[4 * g + i] = v[i];
-    }
-    return ci;
-  };
-  auto row_scale = [&](int64_t t) -> int {
-    return E == Elem::FP8 ? e8m0_of(a.xscale[t * 32 + col]) : 127;
-  };
-
-  Frag<LB> af[NC];
-  {
-    const char* p0 = xb + t_begin * G::TILE + (col + 32 * h) * LB;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) af[c] = load_frag<LB>(p0 + c * G::CHUNK);
-  }
-  f32x16 ci = load_ci(t_begin);
-  int xs = row_scale(t_begin);
-
-  for (int64_t t = t_begin; t < t_end; ++t) {
-    const bool has_next = t + 1 < t_end;
-    const char* pn = xb + (t + 1) * G::TILE + (col + 32 * h) * LB;
-    f32x16 nci = ci;
-    int nxs = xs;
-    if (has_next) {
-      nci = load_ci(t + 1);
-      nxs = row_scale(t + 1);
-    }
-    f32x16 acc[TN];
-#pragma unroll
-    for (int nt = 0; nt < TN; ++nt) {
-#pragma unroll
-      for (int i = 0; i < 16; ++i) acc[nt][i] = ci[i] + cq[nt];
-    }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-#pragma unroll
-      for (int nt = 0; nt < TN; ++nt) acc[nt] = mfma_chunk<E>(af[c], qf[nt][c], acc[nt], xs, qsc[nt]);
-      if (has_next) af[c] = load_frag<LB>(pn + c * G::CHUNK);
-    }
-    bool hit = false;
-    float mxv[TN];
-#pragma unroll
-    for (int nt = 0; nt < TN; ++nt) {
-      float v = acc[nt][0];
-#pragma unroll
-      for (int i = 1; i < 16; ++i) v = fmaxf(v, acc[nt][i]);
-      mxv[nt] = v;
-      hit |= v >= thr[nt];
-    }
-    if (__any(hit)) {
-#pragma unroll
-      for (int nt = 0; nt < TN; ++nt) {
-        if (mxv[nt] >= thr[nt]) {
-          const int qr = qrow[nt];
-          int* cnt = a.counts + qr;
-          uint64_t* dst = a.cand + (int64_t)qr * a.cap;
-#pragma unroll
-          for (int i = 0; i < 16; ++i) {
-            const float v = acc[nt][i];
-            if (v >= thr[nt]) {
-              const int64_t row = t * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
-              const int pos = atomicAdd(cnt, 1);
-              if (pos < a.cap) dst[pos] = pack_cand(-2.0f * v, a.rid[row]);
-            }
-          }
-        }
-      }
-    }
-    ci = nci;
-    xs = nxs;
-  }
-}
-
 template <typename F>
 auto resid_dispatch(Elem e, int dpad, F&& f) {
   if (e == Elem::BF16) {
@@ -413,26 +241,6 @@ unsigned grid_of(int64_t work, int64_t per_block) {
   return (unsigned)std::min<int64_t>(g, 65536);
 }
 
-struct ResidTn {
-  template <Elem E, int DPAD>
-  int operator()() const {
-    return resid_tn<E, DPAD>();
-  }
-};
-
-struct ResidLaunch {
-  const ResidArgs& a;
-  hipStream_t st;
-  template <Elem E, int DPAD>
-  int operator()() const {
-    const int64_t total = (int64_t)a.num_qtiles * a.num_slices;
-    RPLSH_REQUIRE(total < (1ll << 31), "residual filter grid too large");
-    filter_resid_kernel<E, DPAD, resid_tn<E, DPAD>()><<<(unsigned)round_up(total, 8), kResidBlock, 0, st>>>(a);
-    RPLSH_HIP(hipGetLastError());
-    return 0;
-  }
-};
-
 struct ResidPack {
   const rplsh_bf* b;
   const rplsh_bf::ResidCopy* c;
@@ -449,13 +257,6 @@ struct ResidPack {
 };
 
 }  // namespace
-
-int resid_block_queries(Elem e, int dpad) { return 4 * 32 * resid_dispatch(e, dpad, ResidTn{}); }
-
-void launch_filter_resid(Elem e, int dpad, const ResidArgs& a, hipStream_t st) {
-  if (a.nq <= 0 || a.num_slices <= 0) return;
-  resid_dispatch(e, dpad, ResidLaunch{a, st});
-}
 
 // ---- host build ----------------------------------------------------------
 

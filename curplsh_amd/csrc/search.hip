@@ -143,17 +143,21 @@ static int64_t level_cap(const std::vector<Level>& L, int kp, int c_factor) {
 }
 
 // Levels over the cluster-residual copies: the sparsest copy first (every
-// row kept: the threshold is +inf), then each denser sample copy down to the
-// last one, the sparsest whose threshold rank target / stride is at least the
-// final rank, then the copy of every row. Intermediate thresholds take rank
-// kRhoMid (copies 8 times denser: about 128 survivors), the last sample
-// level's the rank that leaves about the survivor target at the full level.
+// row kept: the threshold is +inf), then denser sample copies down to the
+// last one -- the sparsest whose threshold rank target / stride is at least
+// the final rank -- skipping copies while a threshold of rank kRhoMid still
+// leaves at most kRhoMid * kResidMaxRatio survivors at the next level taken,
+// then the copy of every row. (Each level costs an encoding of the queries per
+// centroid besides its rows; a wider step costs survivor emission instead:
+// measured on 20M x 128, 1,024 centres, a step of 8 between copies is the
+// cheaper of the two.)
 struct ResidLevel {
   int copy;
   int rank_next;  // rank of the threshold for the next level; 0 on the last
 };
+constexpr int kResidRhoMid = 16;
+constexpr int64_t kResidMaxRatio = 8;
 static std::vector<ResidLevel> resid_levels(const DbView& db, int kp, int c_factor, int rho_final) {
-  constexpr int kRhoMid = 16;
   const int64_t target = survivor_target(kp, c_factor);
   const int rf = final_rank(kp, c_factor, rho_final);
   int last = 0;
@@ -164,10 +168,21 @@ static std::vector<ResidLevel> resid_levels(const DbView& db, int kp, int c_fact
     }
   std::vector<ResidLevel> L;
   if (last > 0) {
-    for (int j = db.nrcopies - 1; j >= last; --j) {
-      const auto& c = db.rcopy[j];
-      const int rank = j > last ? kRhoMid : (int)std::min<int64_t>(c.rows, div_up(target, c.stride));
-      L.push_back({j, std::max(1, rank)});
+    std::vector<int> chain{db.nrcopies - 1};
+    while (chain.back() > last) {
+      // The sparsest copy within kResidMaxRatio of the current one, at least `last`.
+      int nxt = last;
+      for (int j = last; j < chain.back(); ++j)
+        if (db.rcopy[chain.back()].stride / db.rcopy[j].stride <= kResidMaxRatio) {
+          nxt = j;
+          break;
+        }
+      chain.push_back(nxt);
+    }
+    for (size_t i = 0; i < chain.size(); ++i) {
+      const auto& c = db.rcopy[chain[i]];
+      const int rank = chain[i] > last ? kResidRhoMid : (int)std::min<int64_t>(c.rows, div_up(target, c.stride));
+      L.push_back({chain[i], std::max(1, rank)});
     }
   }
   L.push_back({0, 0});
@@ -180,6 +195,7 @@ static int64_t batch_cap(const DbView& db, int kp, const SearchParams& p) {
   if (db.nrcopies > 0) {
     const std::vector<ResidLevel> L = resid_levels(db, kp, p.c_factor, p.rho);
     int64_t cap = std::max<int64_t>(4096, 4 * survivor_target(kp, p.c_factor));
+    cap = std::max<int64_t>(cap, 2 * kResidRhoMid * kResidMaxRatio);
     return std::max<int64_t>(cap, db.rcopy[L.front().copy].rows);
   }
   return level_cap(plan_levels(db.n, kp, p.c_factor, p.rho), kp, p.c_factor);
@@ -260,29 +276,34 @@ static void run_filter(const DbView& db, const Level& lv, bool dense, const void
 // One level over a cluster-residual copy: every padded row, one workgroup
 // per (query block, slice); the lists start empty.
 static void run_filter_resid(const DbView& db, int copy, const float* q, const float* tau, int nq,
-                             const int* nq_dev, int cap, int* counts, uint64_t* cand, hipStream_t st) {
+                             const int* nq_dev, int cap, int* counts, uint64_t* cand, int variant,
+                             hipStream_t st) {
   const DbView::ResidView& c = db.rcopy[copy];
-  ResidArgs ra{};
-  ra.x = c.enc;
-  ra.xscale = c.scale;
-  ra.xnh = c.nh;
-  ra.rid = c.rid;
-  ra.slice_t0 = c.slice_t0;
-  ra.slice_cl = c.slice_cl;
-  ra.num_slices = (int)c.slices;
-  ra.cent = db.rcent;
-  ra.d = db.d;
-  ra.q = q;
-  ra.ldq = db.d;
-  ra.tau = tau;
-  ra.nq = nq;
-  ra.nq_dev = nq_dev;
-  ra.num_qtiles = (int)div_up(nq, resid_block_queries(db.elem, db.dpad));
-  ra.cap = cap;
-  ra.counts = counts;
-  ra.cand = cand;
+  FilterArgs fa{};
+  fa.x = c.enc;
+  fa.ldx = db.dpad;
+  fa.dvalid_bytes = db.dvalid_bytes;
+  fa.xnorm = c.nh;
+  fa.xscale = c.scale;
+  fa.m = c.npad;
+  fa.stride = 1;
+  fa.rid = c.rid;
+  fa.slice_t0 = c.slice_t0;
+  fa.slice_cl = c.slice_cl;
+  fa.num_slices = (int)c.slices;
+  fa.cent = db.rcent;
+  fa.qf32 = q;
+  fa.ldq = db.d;
+  fa.d = db.d;
+  fa.tau = tau;
+  fa.nq = nq;
+  fa.nq_dev = nq_dev;
+  fa.num_qtiles = (int)div_up(nq, filter_resid_block_queries(db.elem, db.dpad, variant));
+  fa.cap = cap;
+  fa.counts = counts;
+  fa.cand = cand;
   launch_fill_i32(counts, nq, 0, st);
-  launch_filter_resid(db.elem, db.dpad, ra, st);
+  launch_filter_resid(db.elem, db.dpad, variant, fa, st);
 }
 
 // Carry: a level whose predecessor read whole tiles of the sample copy starts
@@ -358,7 +379,7 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
     if (last) rec(1);
     int rank_next;
     if (resid) {
-      run_filter_resid(db, RL[li].copy, q, tau, nq, nq_dev, cap, counts, cand, st);
+      run_filter_resid(db, RL[li].copy, q, tau, nq, nq_dev, cap, counts, cand, p.variant, st);
       rank_next = RL[li].rank_next;
     } else {
       const int64_t carried = carried_tiles(db, L, li, p.variant);
