@@ -33,6 +33,7 @@
 //          accumulator is already in real units.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdint>
 
@@ -530,9 +531,112 @@ void launch_kloop(int dpad, const FilterArgs& a, hipStream_t st) {
   RPLSH_HIP(hipGetLastError());
 }
 
+// Plain path, small batches (at most kSmallMaxQ queries): a streaming FP32
+// contraction on the vector ALUs. The FP32 MFMA computes 32 query columns per
+// instruction, so for a handful of queries it spends 32x the needed work at a
+// quarter of the BF16 rate and the kernel is not HBM-bound; here every
+// product is one the batch needs. Eight lanes share a row (lane part p takes
+// columns 4p + 32i, 16-byte loads: one load instruction covers 8 rows x 128
+// contiguous bytes), the queries (zero padded to dpad) sit in LDS and are read
+// as broadcasts, the eight partial sums are added across the lanes, and
+// lane p tests and emits queries j = p, p + 8. The key is an approximation in
+// a different summation order than the MFMA kernels' (-||x||^2/2 added last),
+// within the same forward-error bound the certificate uses, so the re-rank
+// and certificate keep the results exact.
+constexpr int kSmallMaxQ = 16;
+
+template <int NQ>
+__global__ __launch_bounds__(256) void filter_small_kernel(FilterArgs a, int dpad) {
+  extern __shared__ __attribute__((aligned(16))) float sq[];  // [NQ][dpad]
+  const int nq = a.nq_dev ? min(*a.nq_dev, a.nq) : a.nq;    // <= NQ
+  const float* qg = static_cast<const float*>(a.q);
+  for (int e = threadIdx.x; e < NQ * dpad; e += blockDim.x) sq[e] = e / dpad < nq ? qg[e] : 0.0f;
+  __syncthreads();
+  float thr[NQ];
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) thr[j] = j < nq ? fmaxf(-0.5f * a.tau[j], -FLT_MAX) : INFINITY;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = lane & 7, rr = lane >> 3;
+  const int ni = dpad / 32;
+  const float* xb = static_cast<const float*>(a.x);
+  const int64_t groups = (a.m - a.row_begin + 7) / 8;
+  for (int64_t g = (int64_t)blockIdx.x * 4 + wave; g < groups; g += (int64_t)gridDim.x * 4) {
+    const int64_t r = a.row_begin + g * 8 + rr;  // logical row
+    const bool valid = r < a.m;
+    const int64_t rp = (valid ? r : a.m - 1) * a.stride;  // physical row (any valid one past the end)
+    const float* xr = xb + rp * a.ldx;
+    float acc[NQ];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) acc[j] = 0.0f;
+    for (int i0 = 0; i0 < ni; i0 += 8) {
+      f32x4 xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int col = 4 * p + 32 * (i0 + u);
+        xv[u] = i0 + u < ni && col < a.ldx ? *reinterpret_cast<const f32x4*>(xr + col) : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (i0 + u >= ni) break;  // uniform
+        const int col = 4 * p + 32 * (i0 + u);
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+          const f32x4 qv = *reinterpret_cast<const f32x4*>(sq + j * dpad + col);
+          acc[j] = fmaf(xv[u][0], qv[0], acc[j]);
+          acc[j] = fmaf(xv[u][1], qv[1], acc[j]);
+          acc[j] = fmaf(xv[u][2], qv[2], acc[j]);
+          acc[j] = fmaf(xv[u][3], qv[3], acc[j]);
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+      acc[j] += __shfl_xor(acc[j], 1, 64);
+      acc[j] += __shfl_xor(acc[j], 2, 64);
+      acc[j] += __shfl_xor(acc[j], 4, 64);
+    }
+    if (!valid) continue;
+    const float nh = -0.5f * a.xnorm[rp];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+      if (j % 8 != p || j >= nq) continue;
+      const float v = nh + acc[j];
+      if (v >= thr[j]) {
+        const int pos = atomicAdd(a.counts + j, 1);
+        if (pos < a.cap) a.cand[(int64_t)j * a.cap + pos] = pack_cand(-2.0f * v, (int32_t)rp);
+      }
+    }
+  }
+}
+
+template <int NQ>
+void launch_small_one(int dpad, const FilterArgs& a, hipStream_t st) {
+  const int64_t groups = div_up(a.m - a.row_begin, 8);
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(div_up(groups, 4), 2048));
+  filter_small_kernel<NQ><<<grid, 256, (size_t)NQ * dpad * 4, st>>>(a, dpad);
+  RPLSH_HIP(hipGetLastError());
+}
+
+// The small-batch kernel takes the launch when the batch has at most
+// kSmallMaxQ queries and their rows fit in 64 KB of LDS.
+bool launch_small(int dpad, const FilterArgs& a, hipStream_t st) {
+  if (a.nq > kSmallMaxQ || a.dense || a.tile_rank) return false;
+  const int nq_cap = a.nq <= 1 ? 1 : a.nq <= 2 ? 2 : a.nq <= 4 ? 4 : a.nq <= 8 ? 8 : 16;
+  if ((size_t)nq_cap * dpad * 4 > 65536) return false;
+  switch (nq_cap) {
+    case 1: launch_small_one<1>(dpad, a, st); break;
+    case 2: launch_small_one<2>(dpad, a, st); break;
+    case 4: launch_small_one<4>(dpad, a, st); break;
+    case 8: launch_small_one<8>(dpad, a, st); break;
+    default: launch_small_one<16>(dpad, a, st); break;
+  }
+  return true;
+}
+
 template <Elem E>
 void launch_elem(int dpad, const FilterArgs& a, hipStream_t st) {
   if constexpr (E == Elem::F32) {
+    if (launch_small(dpad, a, st)) return;
     if (dpad > 256) return launch_kloop(dpad, a, st);
   }
   switch (dpad) {
