@@ -151,12 +151,19 @@ static int64_t level_cap(const std::vector<Level>& L, int kp, int c_factor) {
 // centroid besides its rows; a wider step costs survivor emission instead:
 // measured on 20M x 128, 1,024 centres, a step of 8 between copies is the
 // cheaper of the two.)
+// Sample levels of at most kF32SampleRows rows score every stride-th FP32 row
+// exactly (the plain path's filter) instead of a residual copy: their keys are
+// as consistent with the residual ones as the residual copies' (both close to
+// the exact key), and a level that small costs its per-centroid query
+// encoding, not its rows, on a residual copy.
 struct ResidLevel {
-  int copy;
-  int rank_next;  // rank of the threshold for the next level; 0 on the last
+  int copy;            // residual copy (f32_stride == 0)
+  int64_t f32_stride;  // > 0: every f32_stride-th FP32 row instead
+  int rank_next;       // rank of the threshold for the next level; 0 on the last
 };
 constexpr int kResidRhoMid = 16;
 constexpr int64_t kResidMaxRatio = 8;
+constexpr int64_t kF32SampleRows = 65536;
 static std::vector<ResidLevel> resid_levels(const DbView& db, int kp, int c_factor, int rho_final) {
   const int64_t target = survivor_target(kp, c_factor);
   const int rf = final_rank(kp, c_factor, rho_final);
@@ -182,10 +189,11 @@ static std::vector<ResidLevel> resid_levels(const DbView& db, int kp, int c_fact
     for (size_t i = 0; i < chain.size(); ++i) {
       const auto& c = db.rcopy[chain[i]];
       const int rank = chain[i] > last ? kResidRhoMid : (int)std::min<int64_t>(c.rows, div_up(target, c.stride));
-      L.push_back({chain[i], std::max(1, rank)});
+      const int64_t f32 = div_up(db.n, c.stride) <= kF32SampleRows ? c.stride : 0;
+      L.push_back({chain[i], f32, std::max(1, rank)});
     }
   }
-  L.push_back({0, 0});
+  L.push_back({0, 0, 0});
   return L;
 }
 
@@ -196,7 +204,8 @@ static int64_t batch_cap(const DbView& db, int kp, const SearchParams& p) {
     const std::vector<ResidLevel> L = resid_levels(db, kp, p.c_factor, p.rho);
     int64_t cap = std::max<int64_t>(4096, 4 * survivor_target(kp, p.c_factor));
     cap = std::max<int64_t>(cap, 2 * kResidRhoMid * kResidMaxRatio);
-    return std::max<int64_t>(cap, db.rcopy[L.front().copy].rows);
+    const ResidLevel& f = L.front();
+    return std::max<int64_t>(cap, f.f32_stride > 0 ? div_up(db.n, f.f32_stride) : db.rcopy[f.copy].rows);
   }
   return level_cap(plan_levels(db.n, kp, p.c_factor, p.rho), kp, p.c_factor);
 }
@@ -359,6 +368,26 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
       break;
   }
 
+  // Residual search: the exact FP32 view and padded FP32 queries for its
+  // smallest sample levels.
+  DbView db32;
+  float* qpad32 = nullptr;
+  if (resid && RL.front().f32_stride > 0) {
+    db32.x = db.x;
+    db32.ldx = db.ldx;
+    db32.n = db.n;
+    db32.d = d;
+    db32.xnorm = db.xnorm;
+    db32.xn_max = db.xn_max;
+    db32.elem = Elem::F32;
+    db32.dpad = filter_dpad(Elem::F32, d);
+    db32.xenc = db.x;
+    db32.ldenc = db.ldx;
+    db32.dvalid_bytes = (int)(db.ldx * 4);
+    qpad32 = ws.qpad.as<float>((size_t)nq * db32.dpad);
+    launch_pad_rows(q, nq, d, d, db32.dpad, qpad32, st);
+  }
+
   // 2. filter levels
   float* tau = ws.tau.as<float>(nq);
   float* tau2 = ws.tau2.as<float>(nq);
@@ -378,7 +407,12 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
     const bool last = li + 1 == nlevels;
     if (last) rec(1);
     int rank_next;
-    if (resid) {
+    if (resid && RL[li].f32_stride > 0) {
+      const Level lv{RL[li].f32_stride, div_up(db.n, RL[li].f32_stride), RL[li].rank_next};
+      run_filter(db32, lv, false, qpad32, nullptr, tau, nq, nq_dev, cap, counts, cand, p.target_wgs, 0, 0, 0,
+                 st);
+      rank_next = RL[li].rank_next;
+    } else if (resid) {
       run_filter_resid(db, RL[li].copy, q, tau, nq, nq_dev, cap, counts, cand, p.variant, st);
       rank_next = RL[li].rank_next;
     } else {
@@ -676,6 +710,7 @@ size_t search_workspace_bytes(const DbView& db, int nq, const SearchParams& p) {
   size_t b = 0;
   b += (size_t)nq * 4 * 9;                 // norms, scales, tau x2, counts, bound, status, relk, list
   b += (size_t)nq * db.dpad * eb;          // encoded queries
+  if (db.nrcopies > 0) b += (size_t)nq * filter_dpad(Elem::F32, db.d) * 4;  // padded FP32 queries
   b += (size_t)nq * cap * 8;               // candidate words
   b += (size_t)nq * kp * 8;                // top-k' words
   return b + 4096;
