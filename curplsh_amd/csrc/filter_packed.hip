@@ -628,14 +628,22 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
   const PendSlots ps{pend_v[wave], pend_thr[wave], pend_qr[wave], pend_off[wave]};
   // Stores a survivor at queue position t (reserved on qtail[par]); past the
   // queue's capacity it goes straight to the query's list.
+  // RESID: queue words carry the residual copy's row; the index row it holds
+  // (rid) is looked up when the word leaves for the global list, so the emit
+  // path inside the tile loop issues no global load (a load there would make
+  // the next wait for memory cover the next stage's prefetch too).
+  auto to_index_row = [&](uint64_t w) -> uint64_t {
+    if constexpr (RESID) return (w & 0xffffffff00000000ull) | (uint32_t)a.rid[cand_idx(w)];
+    return w;
+  };
   auto put = [&](int par, unsigned t, int qr, float v, int64_t row) {
-    const uint64_t w = pack_cand(-2.0f * v, RESID ? a.rid[row] : (int32_t)(row * stride));
+    const uint64_t w = pack_cand(-2.0f * v, (int32_t)(row * stride));
     if (t - head[par] < (unsigned)kQueue) {
       qword[par][t & (kQueue - 1)] = w;
       qquery[par][t & (kQueue - 1)] = qr;
     } else {
       const int pos = atomicAdd(a.counts + qr, 1);
-      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = w;
+      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = to_index_row(w);
     }
   };
   // Whole workgroup: moves the entries of queue par that were stored in it
@@ -646,7 +654,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
       const unsigned t = (head[par] + i) & (kQueue - 1);
       const int qr = qquery[par][t];
       const int pos = atomicAdd(a.counts + qr, 1);
-      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = qword[par][t];
+      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = to_index_row(qword[par][t]);
     }
     head[par] = tail;
   };
