@@ -55,7 +55,7 @@ namespace rplsh {
 
 namespace {
 
-constexpr int kSliceTiles = 128;  // tiles per slice (one centroid each)
+constexpr int kSliceTiles = 256;  // tiles per slice (one centroid each)
 
 // ---- build kernels ------------------------------------------------------
 
