@@ -26,6 +26,10 @@ typedef int i32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kChains = 4;  // independent accumulators per wave
 constexpr int kOps = 8;     // operand registers cycled through
+// Launches of ~0.6 ms each (BF16 / FP8 at the random-operand rate; FP32 about
+// as long) before and inside the timed region.
+constexpr int kWarmLaunches = 40;
+constexpr int kTimedLaunches = 40;
 
 __device__ inline uint32_t mix32(uint32_t x) {
   x ^= x >> 16;
@@ -139,9 +143,12 @@ extern "C" rplsh_status rplsh_bench_mfma_peak(int kind, int random_operands, dou
         else mfma_peak_kernel<2><<<blocks, 256>>>(n_iters, random_operands, out);
         RPLSH_HIP(hipGetLastError());
       };
-      launch(iters);  // warm-up (clocks ramp)
+      // The clock settles only after ~25 ms of sustained MFMA work (a first
+      // 25 ms window runs ~8 % below the later ones): warm up that long, then
+      // time as long again.
+      for (int w = 0; w < kWarmLaunches; ++w) launch(iters);
       RPLSH_HIP(hipEventRecord(e0, nullptr));
-      for (int r = 0; r < 3; ++r) launch(iters);
+      for (int r = 0; r < kTimedLaunches; ++r) launch(iters);
       RPLSH_HIP(hipEventRecord(e1, nullptr));
       RPLSH_HIP(hipEventSynchronize(e1));
       RPLSH_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -154,7 +161,7 @@ extern "C" rplsh_status rplsh_bench_mfma_peak(int kind, int random_operands, dou
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     RPLSH_HIP(hipFree(out));
-    const double mfmas = 3.0 * blocks * 4.0 * iters * kOps * kChains;  // 4 waves per block
+    const double mfmas = (double)kTimedLaunches * blocks * 4.0 * iters * kOps * kChains;  // 4 waves per block
     *tflops = mfmas * mfma_flops(kind) / (ms / 1e3) / 1e12;
   });
 }

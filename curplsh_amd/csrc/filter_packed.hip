@@ -549,20 +549,36 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
 
   const char* xb = reinterpret_cast<const char*>(a.x);
   const int64_t stride = a.stride;
+  // Stage loads: the data (pre), the row norm and FP8 scale of stage row tid
+  // (pn, pe: raw, turned into the C operand and the E8M0 scale by sstore)
+  // and, in lane t < SM of every wave, the rank of stage tile t (grk; sstore
+  // turns the ranks into a wave-uniform skip mask). The norm, scale and rank
+  // loads are unconditional and their values untouched until sstore: an
+  // operation on a loaded value, or a branch merging it, in gload would wait
+  // for every load of the stage right after their issue.
   i32x4 pre[UPT];
   float pn = 0.0f;
-  int pe = 127;
-  // Tile skipping: lane t < SM of every wave loads the rank of stage tile t
-  // with the stage (a vector load, retired in order with the data loads);
-  // sstore turns the ranks into a wave-uniform mask for that stage.
+  float pe = 1.0f;
+  int64_t pj0 = 0;  // first row of the loaded stage
   int32_t grk = INT32_MAX;
   unsigned gskip = 0u;  // bit t: tile t of the stored stage is skipped
   static_assert(SM <= 32, "one skip bit per stage tile");
+  const int32_t* rank_src = a.tile_rank ? a.tile_rank : a.counts;  // read only with tile_rank
+  const int64_t rank_last = a.tile_rank ? a.rank_tiles - 1 : 0;
 
   auto gload = [&](int64_t j0) {
-    if (a.tile_rank) {  // uniform; tiles past rank_tiles hold no rows (clamped)
-      const int64_t u = (j0 >> 5) + (lane < SM ? lane : 0);
-      grk = a.tile_rank[u < a.rank_tiles ? u : a.rank_tiles - 1];
+    pj0 = j0;
+    {
+      // Tiles past rank_tiles hold no rows (clamped).
+      int64_t u = (j0 >> 5) + (lane < SM ? lane : 0);
+      u = u < rank_last ? u : rank_last;
+      grk = rank_src[u];
+    }
+    {
+      int64_t j = j0 + (tid < SR ? tid : 0);
+      j = (j < r_end ? j : r_end - 1) * stride;
+      pn = a.xnorm[j];
+      if (E == Elem::FP8) pe = a.xscale[j];
     }
     if (stride == 1 && j0 + SR <= r_end) {
       // Whole stage at stride 1: j0 is a multiple of 32, so the stage is SM
@@ -574,10 +590,6 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
         i32x4 v = {0, 0, 0, 0};
         if (UNITS % NT == 0 || u < UNITS) v = *reinterpret_cast<const i32x4*>(src + u * 16);
         pre[t] = v;
-      }
-      if (tid < SR) {
-        pn = RESID ? a.xnorm[j0 + tid] : -0.5f * a.xnorm[j0 + tid];
-        pe = E == Elem::FP8 ? e8m0_of(a.xscale[j0 + tid]) : 127;
       }
       return;
     }
@@ -600,16 +612,6 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
       }
       pre[t] = v;
     }
-    if (tid < SR) {
-      const int64_t j = j0 + tid;
-      if (j < r_end) {
-        pn = RESID ? a.xnorm[j * stride] : -0.5f * a.xnorm[j * stride];
-        pe = E == Elem::FP8 ? e8m0_of(a.xscale[j * stride]) : 127;
-      } else {
-        pn = -INFINITY;
-        pe = 127;
-      }
-    }
   };
   auto sstore = [&](int buf) {
 #pragma unroll
@@ -618,10 +620,12 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_
       if (UNITS % NT == 0 || u < UNITS) *reinterpret_cast<i32x4*>(&sx[buf][u * 16]) = pre[t];
     }
     if (tid < SR) {
-      sn[buf][tid] = pn;
-      se[buf][tid] = pe;
+      // -||x||^2/2 (residual copies hold it already); -inf past the end
+      const bool in = pj0 + tid < r_end;
+      sn[buf][tid] = in ? (RESID ? pn : -0.5f * pn) : -INFINITY;
+      se[buf][tid] = E == Elem::FP8 && in ? e8m0_of(pe) : 127;
     }
-    gskip = (unsigned)__ballot(lane < SM && grk < a.skip_tiles);
+    gskip = (unsigned)__ballot(lane < SM && a.tile_rank != nullptr && grk < a.skip_tiles);
   };
 
   unsigned head[2] = {0u, 0u};  // flushed up to here (same in every thread)
