@@ -299,7 +299,8 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": round(ach / peak, 4),
             "traffic": traffic.get("bytes"),
-            "kernel": f"{KERNEL_OF_VARIANT.get(variant, 'filter')}<{form}> full-database launch",
+            # the tuned variant's low byte is the full level's kernel (bits 8-15: the sample levels')
+            "kernel": f"{KERNEL_OF_VARIANT.get(variant & 0xFF, 'filter')}<{form}> full-database launch",
             "algorithmic_flops_per_launch": 2.0 * nq * head["rows_filter_full"] * d,
             "rows_per_launch": head["rows_filter_full"],
             "frac_of_measured_mfma": round(ach / measured["mfma_tflops_random"][form], 4),
@@ -334,7 +335,7 @@ def main() -> int:
 
 KERNEL_OF_VARIANT = {0: "filter_lds_kernel(wide)", 1: "filter_lds_kernel(narrow)",
                      2: "filter_packed_kernel", 3: "filter_ring_kernel(wide)",
-                     4: "filter_ring_kernel(narrow)"}
+                     4: "filter_ring_kernel(narrow)", 5: "filter_lds_kernel(wide, 8 waves)"}
 
 
 def pmc_pass(args, ca) -> int:

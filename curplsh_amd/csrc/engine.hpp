@@ -228,9 +228,17 @@ struct SearchParams {
   bool certify = false;
   int c_factor = 3;    // last-level target survivors = c_factor * kp
   int target_wgs = 4096;  // filter workgroups aimed for (query tiles x database slices)
-  int variant = 0;        // packed filter variant (FilterVariant)
+  int variant = 0;        // packed filter variant (FilterVariant) of the full level
+  int sample_variant = -1;  // variant of the sample levels (-1: the same)
   int rho = 0;            // rank of the last threshold in its sample (0: automatic)
 };
+
+// Tuning entries (and the graph cache keys) carry both variants in one int:
+// the full level's in bits 0-7, 1 + the sample levels' in bits 8-15 (0: the
+// same kernel for every level).
+inline int encode_variants(int full, int sample) { return full | (sample >= 0 ? (sample + 1) << 8 : 0); }
+inline int full_variant_of(int v) { return v & 0xff; }
+inline int sample_variant_of(int v) { return ((v >> 8) & 0xff) - 1; }
 
 
 // The capturable part of a batch (no host synchronisation; allocates only

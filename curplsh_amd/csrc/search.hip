@@ -332,6 +332,12 @@ static int64_t carried_tiles(const DbView& db, const std::vector<Level>& L, size
   return 0;
 }
 
+// Filter variant of level li of nlevels: the sample levels may run another
+// kernel than the full level (same arithmetic and survivors).
+static int level_variant(const SearchParams& p, size_t li, size_t nlevels) {
+  return li + 1 < nlevels && p.sample_variant >= 0 ? p.sample_variant : p.variant;
+}
+
 // Capturable part of a batch: query norms and encoding, filter levels,
 // selects, re-rank and the failed-query list, all on st with no host
 // synchronisation (nothing is allocated once ws has seen this size). When
@@ -416,9 +422,9 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
       run_filter_resid(db, RL[li].copy, q, tau, nq, nq_dev, cap, counts, cand, p.variant, st);
       rank_next = RL[li].rank_next;
     } else {
-      const int64_t carried = carried_tiles(db, L, li, p.variant);
+      const int64_t carried = carried_tiles(db, L, li, level_variant(p, li, nlevels));
       run_filter(db, L[li], li == 0, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs,
-                 p.variant, last ? carried : 0, last ? 0 : 32 * carried, st);
+                 level_variant(p, li, nlevels), last ? carried : 0, last ? 0 : 32 * carried, st);
       rank_next = L[li].rank_next;
     }
     if (last) rec(2);
@@ -436,7 +442,7 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
       sa.status = nullptr;  // a loose intermediate threshold only costs time
       launch_select(sa, st);
       std::swap(tau, tau2);
-      const int64_t carry_next = resid ? 0 : carried_tiles(db, L, li + 1, p.variant);
+      const int64_t carry_next = resid ? 0 : carried_tiles(db, L, li + 1, level_variant(p, li + 1, nlevels));
       if (carry_next > 0) {
         CarryArgs ca{};
         ca.cand = cand;
@@ -658,7 +664,7 @@ void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& 
       stats.levels = (int32_t)L.size();
       stats.rows_filter_full = L.back().m - 32 * carried_tiles(db, L, L.size() - 1, p.variant);
     }
-    stats.filter_variant = p.variant;
+    stats.filter_variant = encode_variants(p.variant, p.sample_variant);
     stats.target_wgs = (int32_t)p.target_wgs;
     if (pe && pe->on) {
       stats.ms_filter_samples = pe->ms(0, 1);
@@ -917,6 +923,14 @@ namespace rplsh {
 
 DbView bf_db_view(const rplsh_bf* b) { return bf_view(b); }
 
+// A tuning entry's variant field: a full-level variant 0..5, optionally with
+// a sample-level variant 0..5 in bits 8-15 (encode_variants).
+static bool valid_variants(int v) {
+  const int f = full_variant_of(v), sv = sample_variant_of(v);
+  return v >= 0 && (v >> 16) == 0 && f >= kFilterLdsWide && f <= kFilterLdsWide8 && sv >= -1 &&
+         sv <= kFilterLdsWide8;
+}
+
 SearchParams bf_search_params(const rplsh_bf* b, int k, int64_t nq) {
   SearchParams p;
   p.k = k;
@@ -946,7 +960,10 @@ SearchParams bf_search_params(const rplsh_bf* b, int k, int64_t nq) {
     if (!plain && best->kprime_factor > 0) p.kp = k * best->kprime_factor;
     if (best->target_wgs > 0) p.target_wgs = best->target_wgs;
     if (best->c_factor > 0) p.c_factor = best->c_factor;
-    if (best->variant >= kFilterLdsWide && best->variant <= kFilterLdsWide8) p.variant = best->variant;
+    if (valid_variants(best->variant)) {
+      p.variant = full_variant_of(best->variant);
+      p.sample_variant = sample_variant_of(best->variant);
+    }
     if (best->rho > 0) p.rho = best->rho;
   }
   return p;
@@ -1539,6 +1556,24 @@ rplsh_status rplsh_bf_autotune(rplsh_bf* b, const float* sample, int64_t nsample
             r.variant = vars[vi];
           }
         }
+    // 3. The sample levels with the other LDS kernel (same survivors): their
+    // loose thresholds make them emission-heavy, which the narrow kernel's
+    // third wave per SIMD hides better for FP8 (10M x 96, 10k queries: sample
+    // levels 1.41 -> 1.22 ms with the wide kernel on the full level).
+    if (v.packed && v.nrcopies == 0 && (r.variant == kFilterLdsWide || r.variant == kFilterLdsNarrow)) {
+      SearchParams p = base;
+      p.target_wgs = r.target_wgs;
+      p.c_factor = r.c_factor;
+      p.rho = r.rho;
+      p.variant = r.variant;
+      const float ms_same = time_it(p);
+      p.sample_variant = r.variant == kFilterLdsWide ? kFilterLdsNarrow : kFilterLdsWide;
+      const float ms_mixed = time_it(p);
+      if (ms_mixed < ms_same) {
+        r.ms_tuned -= ms_same - ms_mixed;
+        r.variant = encode_variants(p.variant, p.sample_variant);
+      }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->tunes.erase(std::remove_if(b->tunes.begin(), b->tunes.end(),
@@ -1556,7 +1591,7 @@ rplsh_status rplsh_bf_set_tuning(rplsh_bf* b, const rplsh_tune_result* t) {
     RPLSH_REQUIRE(b && t, "NULL argument");
     RPLSH_REQUIRE(t->k > 0 && t->batch_nq > 0, "bad tuning entry");
     RPLSH_REQUIRE(t->kprime_factor >= 0 && (int64_t)t->kprime_factor * t->k <= 4096, "k' too large");
-    RPLSH_REQUIRE(t->variant >= kFilterLdsWide && t->variant <= kFilterLdsWide8, "unknown filter variant %d", t->variant);
+    RPLSH_REQUIRE(valid_variants(t->variant), "unknown filter variant %d", t->variant);
     RPLSH_REQUIRE(t->rho >= 0 && t->rho <= 256, "bad rho %d", t->rho);
     bf_begin_change(b);
     b->tunes.erase(std::remove_if(b->tunes.begin(), b->tunes.end(),

@@ -287,6 +287,8 @@ rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out)
  *  - variant (BF16/FP8): the filter's data movement and tile shape, by time,
  *    among LDS wide (0), LDS narrow (1) and ring wide (3); direct (2), ring
  *    narrow (4) and LDS wide with 8-wave workgroups (5) can be installed.
+ *    With LDS wide or narrow chosen, the sample levels are then timed with
+ *    the other of the two (same survivors) and run with it if faster.
  * target_recall <= 0 keeps the configured k' factor and tunes the rest.
  * The result is installed on the index and used by later searches with the
  * same k (the entry with the nearest batch size). */
@@ -296,8 +298,10 @@ typedef struct rplsh_tune_result {
   int kprime_factor;  /* 0 for the plain path */
   int target_wgs;
   int c_factor;
-  int variant;        /* 0 LDS wide, 1 LDS narrow, 2 direct, 3 ring wide, 4 ring narrow,
-                         5 LDS wide with 8-wave workgroups (BF16/FP8 only) */
+  int variant;        /* bits 0-7: the full level's filter variant: 0 LDS wide, 1 LDS
+                         narrow, 2 direct, 3 ring wide, 4 ring narrow, 5 LDS wide with
+                         8-wave workgroups (BF16/FP8 only); bits 8-15: 1 + the sample
+                         levels' variant (0: the same as the full level) */
   int rho;            /* rank of the last threshold within its sample (0: automatic) */
   float recall;       /* recall@k of the chosen k' on the sample (1 for plain) */
   float ms_default;   /* batch time with the built-in defaults */

@@ -83,3 +83,37 @@ def test_carry_after_incremental_add_and_partial_tile(ca, res, precision):
             # same rows kept at the full level, the same queries re-run
             assert (st["survivors"], st["retried_queries"]) == ref[2:], (name, v)
         idx.close()
+
+
+@pytest.mark.parametrize("precision,n,d", [("bf16", 300_000, 96), ("fp8", 200_000, 128)])
+def test_sample_level_variant(ca, res, precision, n, d):
+    """A tuning entry may run the sample levels with another filter kernel
+    than the full level (variant bits 8-15 = 1 + the sample levels' variant):
+    the levels keep the same survivors, the carry still applies between LDS
+    kernels, and the results are bit-identical to one kernel throughout.
+    Malformed encodings are refused."""
+    x = ca.gen_gaussian(n, d, 81, True)
+    q = ca.gen_gaussian(600, d, 82, True)
+    k = 100
+    idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=precision, kprime_factor=4))
+    idx.add(x)
+    out = {}
+    for v in (0, 1, 0 | (2 << 8), 1 | (1 << 8), 0 | (4 << 8)):  # wide, narrow, wide+narrow, narrow+wide, wide+ring
+        idx.set_tuning(dict(k=k, batch_nq=q.shape[0], kprime_factor=4, target_wgs=16384, c_factor=2, variant=v,
+                            rho=64, recall=0.0, ms_default=0.0, ms_tuned=0.0))
+        I, D = idx.search(q, k)
+        st = idx.last_stats()
+        assert st["filter_variant"] == v
+        out[v] = (I, D, st["survivors"], st["rows_filter_full"])
+    I0, D0, s0, r0 = out[0]
+    for v, (I, D, s, r) in out.items():
+        assert np.array_equal(I, I0), v
+        assert np.array_equal(D.view(np.uint32), D0.view(np.uint32)), v
+        assert s == s0, v
+    # LDS kernels on both sides: the full level skips the carried sample tiles
+    assert out[0 | (2 << 8)][3] == r0 < n
+    for bad in (6, 0 | (7 << 8), 1 << 16, -1):
+        with pytest.raises(Exception):
+            idx.set_tuning(dict(k=k, batch_nq=q.shape[0], kprime_factor=4, target_wgs=16384, c_factor=2,
+                                variant=bad, rho=64, recall=0.0, ms_default=0.0, ms_tuned=0.0))
+    idx.close()
