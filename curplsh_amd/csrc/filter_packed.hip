@@ -438,13 +438,17 @@ __device__ __forceinline__ void resid_query_operands(const FilterArgs& a, int q0
   __syncthreads();  // the area becomes the staging buffers
 }
 
-template <Elem E, int DPAD, bool WIDE, int NW>
+template <Elem E, int DPAD, int TN, bool WIDE, int NW>
 struct StageShape {
   using G = Geometry<E, DPAD>;
   // Stage tiles: the double-buffered stage within ~48 KB (wide, two 4-wave
   // workgroups per CU), ~36 KB (narrow, three) or ~64 KB (one 8-wave
-  // workgroup per CU), at most 4 tiles.
-  static constexpr int kBudget = NW == 8 ? 64 * 1024 : (WIDE ? 48 * 1024 : 36 * 1024);
+  // workgroup per CU), at most 4 tiles. A wide kernel holding more than 80
+  // registers of query operands (BF16 at dpad 96 with 4 query tiles) stages
+  // within ~24 KB: its staging registers are what keeps its tile loop free of
+  // spills.
+  static constexpr bool kBigQ = TN * G::NC * G::LB / 4 > 80;
+  static constexpr int kBudget = NW == 8 ? 64 * 1024 : (WIDE ? (kBigQ ? 24 : 48) * 1024 : 36 * 1024);
   static constexpr int kMaxSM = 4;  // more tiles per stage need more staging registers
   static constexpr int kFit = kBudget / (2 * G::TILE);
   static constexpr int SM = kFit > kMaxSM ? kMaxSM : (kFit < 1 ? 1 : kFit);
@@ -460,10 +464,10 @@ struct StageShape {
 // operand adds q.c - ||c||^2/2 to the copy's -||r||^2/2, and survivors carry
 // the index row the residual row holds.
 template <Elem E, int DPAD, int TN, bool WIDE, int NW = 4, bool RESID = false>
-__global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, WIDE, NW>::WAVES_PER_SIMD)) void filter_lds_kernel(
+__global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_PER_SIMD)) void filter_lds_kernel(
     FilterArgs a) {
   using G = Geometry<E, DPAD>;
-  using SS = StageShape<E, DPAD, WIDE, NW>;
+  using SS = StageShape<E, DPAD, TN, WIDE, NW>;
   constexpr int NT = NW * 64;  // threads per workgroup
   constexpr int LB = G::LB;
   constexpr int NC = G::NC;
@@ -1096,12 +1100,29 @@ constexpr int lds_tn() {
   return WIDE ? wide : narrow;
 }
 
+// Query tiles per wave of the non-residual LDS wide kernel: BF16 holds up to
+// ~96 registers of query operands there and at least 2 tiles (4 at dpad <= 96,
+// 3 at 128, 2 from 160 to 256; the kernels above 80 registers stage within
+// ~24 KB), which spreads each tile's fixed instructions (operand and
+// C-operand reads, skip test, loop) over more MFMAs. Measured on one box
+// (10k queries, full-database launch): 10M x 96 15.42 -> 15.19 ms, 2M x 192
+// 9.70 -> 7.79 ms, 2M x 256 12.34 -> 9.48 ms, 2M x 128 unchanged; 3 tiles at
+// dpad 160 spill and were slower. Everything else as lds_tn.
+template <Elem E, int DPAD>
+constexpr int wide_tn() {
+  using G = Geometry<E, DPAD>;
+  constexpr int per_tile = G::NC * G::LB / 4;
+  constexpr int t96 = 96 / per_tile > 4 ? 4 : (96 / per_tile < 2 ? 2 : 96 / per_tile);
+  return E == Elem::BF16 ? t96 : lds_tn<E, DPAD, true>();
+}
+
 template <Elem E, int DPAD>
 int tn_of(int variant) {
   switch (variant) {
     case kFilterDirect: return direct_tn<E, DPAD>();
     case kFilterLdsNarrow:
     case kFilterRingNarrow: return lds_tn<E, DPAD, false>();
+    case kFilterLdsWide: return wide_tn<E, DPAD>();
     default: return lds_tn<E, DPAD, true>();
   }
 }
@@ -1129,7 +1150,7 @@ void launch_packed_one(int variant, const FilterArgs& a, hipStream_t st) {
       filter_ring_kernel<E, DPAD, lds_tn<E, DPAD, false>(), false><<<grid, kBlock, 0, st>>>(a);
       break;
     default:
-      filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, true>(), true><<<grid, kBlock, 0, st>>>(a);
+      filter_lds_kernel<E, DPAD, wide_tn<E, DPAD>(), true><<<grid, kBlock, 0, st>>>(a);
       break;
   }
   RPLSH_HIP(hipGetLastError());
