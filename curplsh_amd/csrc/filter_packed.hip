@@ -192,11 +192,17 @@ __global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) 
 // it, one barrier per stage. Each wave works one 32-row tile at a time with TN
 // query tiles, so an A operand read from LDS (consecutive lanes, consecutive
 // 16 B: conflict-free) feeds TN MFMAs.
-// Survivors are appended to one of two LDS queues (by stage parity) with LDS
-// atomics. After the barrier that ends stage s, the queue of stage s is
-// flushed to the per-query lists if more than half full: nobody appends to it
-// during stage s+1, so every thread reads the same tail and takes the same
-// branch, and it is not reused before the barrier that ends stage s+1.
+// Survivors go to a queue of the wave's own in LDS, at positions given by
+// ballot prefix counts from a wave-uniform fill level (no atomics, no LDS
+// round trip in the tile loop); the wave moves it to the per-query lists
+// (global atomics) when the next values would not fit, and at the end.
+constexpr int kSeg = 256;  // entries per wave queue (LDS variants)
+// Ring variants: survivors are appended to one of two LDS queues (by stage
+// parity) with LDS atomics. After the barrier that ends stage s, the queue of
+// stage s is flushed to the per-query lists if more than half full: nobody
+// appends to it during stage s+1, so every thread reads the same tail and
+// takes the same branch, and it is not reused before the barrier that ends
+// stage s+1.
 constexpr int kQueue = 512;  // entries per survivor queue (power of two)
 constexpr int kHitSlots = 8;  // lanes a wave stages at a time in emit_hits
 
@@ -256,85 +262,6 @@ __device__ __forceinline__ void emit_hits(const f32x16 (&acc)[TN], const float (
         }
       }
       __builtin_amdgcn_wave_barrier();
-    }
-  }
-}
-
-// Deferred form of emit_hits for the LDS kernels. In a tile's epilogue the
-// hitting lanes only stage their 16 values plus their threshold, query and
-// row offset (mt * 32 + 4 (L >> 5)) in the wave's kPend pending slots (LDS
-// stores, nothing waited on); drain_hits() emits them -- four staged lanes
-// per step as in emit_hits -- right after the next tile's MFMAs are issued,
-// so its LDS round trips overlap MFMA execution, and at the end of the
-// stage. A query tile with more hitting lanes than free slots drains first.
-constexpr int kPend = 12;
-
-struct PendSlots {
-  float* v;    // [kPend][16]
-  float* thr;  // [kPend]
-  int* qr;     // [kPend]
-  int* off;    // [kPend] row offset within the stage
-};
-
-template <typename Put>
-__device__ __forceinline__ void drain_hits(int pend, const PendSlots& ps, int64_t j0, int lane, unsigned* tail,
-                                           Put&& put) {
-  const int g = lane >> 4, j = lane & 15;
-  for (int s0 = 0; s0 < pend; s0 += 4) {
-    const int e = s0 + g;
-    const bool live = e < pend;
-    const int ec = live ? e : 0;
-    const float thr_l = ps.thr[ec];
-    const int qr_l = ps.qr[ec];
-    const int off = ps.off[ec];
-    const float v = live ? ps.v[ec * 16 + j] : -INFINITY;
-    const bool hit = v >= thr_l;
-    const uint64_t H = __ballot(hit);
-    unsigned base = 0u;
-    if (lane == 0) base = atomicAdd(tail, (unsigned)__popcll(H));
-    base = __builtin_amdgcn_readfirstlane(base);
-    if (hit) {
-      const unsigned t =
-          base + __builtin_amdgcn_mbcnt_hi((uint32_t)(H >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)H, 0u));
-      put(t, qr_l, v, j0 + off + 8 * (j >> 2) + (j & 3));
-    }
-  }
-  __builtin_amdgcn_wave_barrier();
-}
-
-// Stages the hitting lanes of the tile (first row offset mt * 32 within the
-// stage); `pend` is the wave's pending count (uniform), drained when full.
-template <int TN, typename Put>
-__device__ __forceinline__ void stage_hits(const f32x16 (&acc)[TN], const float (&mx)[TN], const float (&thr)[TN],
-                                           const int (&qrow)[TN], int mt, int64_t j0, int lane, int& pend,
-                                           const PendSlots& ps, unsigned* tail, Put&& put) {
-#pragma unroll
-  for (int nt = 0; nt < TN; ++nt) {
-    uint64_t B = __ballot(mx[nt] >= thr[nt]);
-    while (B != 0) {
-      if (pend == kPend) {
-        __builtin_amdgcn_wave_barrier();
-        drain_hits(pend, ps, j0, lane, tail, put);
-        pend = 0;
-      }
-      const int room = kPend - pend;
-      const unsigned rank =
-          __builtin_amdgcn_mbcnt_hi((uint32_t)(B >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)B, 0u));
-      const bool take = (B >> lane & 1ull) && rank < (unsigned)room;
-      if (take) {
-        const int slot = pend + (int)rank;
-        f32x4* d = reinterpret_cast<f32x4*>(ps.v + slot * 16);
-        d[0] = f32x4{acc[nt][0], acc[nt][1], acc[nt][2], acc[nt][3]};
-        d[1] = f32x4{acc[nt][4], acc[nt][5], acc[nt][6], acc[nt][7]};
-        d[2] = f32x4{acc[nt][8], acc[nt][9], acc[nt][10], acc[nt][11]};
-        d[3] = f32x4{acc[nt][12], acc[nt][13], acc[nt][14], acc[nt][15]};
-        ps.thr[slot] = thr[nt];
-        ps.qr[slot] = qrow[nt];
-        ps.off[slot] = mt * 32 + 4 * (lane >> 5);
-      }
-      const uint64_t T = __ballot(take);
-      pend += __popcll(T);
-      B &= ~T;
     }
   }
 }
@@ -454,7 +381,7 @@ struct StageShape {
   static constexpr int SM = kFit > kMaxSM ? kMaxSM : (kFit < 1 ? 1 : kFit);
   static constexpr int SR = 32 * SM;  // rows per stage
   static constexpr int STAGE = SM * G::TILE;
-  static constexpr int LDS = 2 * STAGE + 2 * SR * 8 + 2 * kQueue * 12 + NW * kPend * 76;
+  static constexpr int LDS = 2 * STAGE + 2 * SR * 8 + NW * kSeg * 12;
   static constexpr int WAVES_PER_SIMD = (NW == 4 && !WIDE && 3 * LDS <= 160 * 1024) ? 3 : 2;
 };
 
@@ -485,13 +412,9 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
   __shared__ __attribute__((aligned(16))) char sx[2][STAGE];
   __shared__ __attribute__((aligned(16))) float sn[2][SR];  // -||x||^2/2, -inf past the end
   __shared__ int se[2][SR];                                 // E8M0 row scales (FP8)
-  __shared__ uint64_t qword[2][kQueue];
-  __shared__ int qquery[2][kQueue];
-  __shared__ unsigned qtail[2];
-  __shared__ __attribute__((aligned(16))) float pend_v[NW][kPend * 16];
-  __shared__ float pend_thr[NW][kPend];
-  __shared__ int pend_qr[NW][kPend];
-  __shared__ int pend_off[NW][kPend];
+  // Survivor queue of wave w: qword/qquery[w][t], t < its fill level.
+  __shared__ uint64_t qword[NW][kSeg];
+  __shared__ int qquery[NW][kSeg];
 
   const int per_xcd = (int)(gridDim.x >> 3);
   const int logical = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
@@ -632,10 +555,6 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
     gskip = (unsigned)__ballot(lane < SM && a.tile_rank != nullptr && grk < a.skip_tiles);
   };
 
-  unsigned head[2] = {0u, 0u};  // flushed up to here (same in every thread)
-  const PendSlots ps{pend_v[wave], pend_thr[wave], pend_qr[wave], pend_off[wave]};
-  // Stores a survivor at queue position t (reserved on qtail[par]); past the
-  // queue's capacity it goes straight to the query's list.
   // RESID: queue words carry the residual copy's row; the index row it holds
   // (rid) is looked up when the word leaves for the global list, so the emit
   // path inside the tile loop issues no global load (a load there would make
@@ -644,30 +563,19 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
     if constexpr (RESID) return (w & 0xffffffff00000000ull) | (uint32_t)a.rid[cand_idx(w)];
     return w;
   };
-  auto put = [&](int par, unsigned t, int qr, float v, int64_t row) {
-    const uint64_t w = pack_cand(-2.0f * v, (int32_t)(row * stride));
-    if (t - head[par] < (unsigned)kQueue) {
-      qword[par][t & (kQueue - 1)] = w;
-      qquery[par][t & (kQueue - 1)] = qr;
-    } else {
+  // Fill level of this wave's queue (wave-uniform: a survivor's position is
+  // a ballot prefix count, no atomics).
+  unsigned fill = 0u;
+  // This wave: moves its queue to the per-query lists.
+  auto wave_flush = [&]() {
+    for (unsigned t = lane; t < fill; t += 64) {
+      const int qr = qquery[wave][t];
       const int pos = atomicAdd(a.counts + qr, 1);
-      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = to_index_row(w);
+      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = to_index_row(qword[wave][t]);
     }
-  };
-  // Whole workgroup: moves the entries of queue par that were stored in it
-  // (at most kQueue since the last flush) to the per-query lists.
-  auto flush = [&](int par, unsigned tail) {
-    const unsigned n = min(tail - head[par], (unsigned)kQueue);
-    for (unsigned i = tid; i < n; i += NT) {
-      const unsigned t = (head[par] + i) & (kQueue - 1);
-      const int qr = qquery[par][t];
-      const int pos = atomicAdd(a.counts + qr, 1);
-      if (pos < a.cap) a.cand[(int64_t)qr * a.cap + pos] = to_index_row(qword[par][t]);
-    }
-    head[par] = tail;
+    fill = 0u;
   };
 
-  if (tid < 2) qtail[tid] = 0u;
   // The stage loop, instantiated twice: the hot loop (DENSE false) stays
   // small enough for the instruction cache; the dense first pass writes every
   // row's word to its own slot.
@@ -678,13 +586,10 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
     sstore(0);
     __syncthreads();
     int buf = 0;
-    for (int s = 0; j0 < r_end; j0 += SR, ++s) {
+    for (; j0 < r_end; j0 += SR) {
       const bool has_next = j0 + SR < r_end;
       const unsigned skip = DENSE ? 0u : gskip;  // this stage's, set when it was stored
       if (has_next) gload(j0 + SR);
-      const int par = s & 1;
-      auto put_par = [&](unsigned t, int qr, float v, int64_t row) { put(par, t, qr, v, row); };
-      int pend = 0;  // staged hits of this stage not yet emitted (wave-uniform)
       // One 32-row tile per iteration; not unrolled (code size).
 #pragma unroll 1
       for (int mt = 0; mt < SM; ++mt) {
@@ -714,12 +619,6 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
             acc[nt] = mfma_chunk<E>(af, qf[nt][c], c == 0 ? c0 : acc[nt], xsc, qsc[nt]);
           }
         }
-        if (!DENSE && pend != 0) {
-          // The previous tiles' hits, while this tile's MFMAs run.
-          __builtin_amdgcn_sched_barrier(0);
-          drain_hits(pend, ps, j0, lane, &qtail[par], put_par);
-          pend = 0;
-        }
         if (DENSE) {
 #pragma unroll
           for (int nt = 0; nt < TN; ++nt) {
@@ -743,19 +642,49 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
           mx[nt] = v;
           hit |= v >= thr[nt];
         }
-        if (__any(hit)) stage_hits<TN>(acc, mx, thr, qrow, mt, j0, lane, pend, ps, &qtail[par], put_par);
-      }
-      if (!DENSE && pend != 0) {
-        drain_hits(pend, ps, j0, lane, &qtail[par], put_par);
-        pend = 0;
+        if (__any(hit)) {
+          // Emit every value >= its query's threshold, one accumulator index
+          // at a time: value i of lane L is row 8 (i >> 2) + 4 (L >> 5) +
+          // (i & 3) of the tile for the query in qrow of lane L (the 32x32
+          // MFMA accumulator layout); a ballot gives the lanes and their
+          // queue positions. Four indices (at most 4 x 64 = kSeg values) at
+          // a time, after making room for them.
+          const int32_t id0 = (int32_t)((j0 + mt * 32 + 4 * h) * stride);
+#pragma unroll
+          for (int nt = 0; nt < TN; ++nt) {
+            if (!__any(mx[nt] >= thr[nt])) continue;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+              uint64_t B[4];
+              unsigned cnt = 0u;
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                B[u] = __ballot(acc[nt][4 * g + u] >= thr[nt]);
+                cnt += (unsigned)__popcll(B[u]);
+              }
+              if (cnt == 0u) continue;
+              if (fill + cnt > (unsigned)kSeg) wave_flush();
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                if (B[u] == 0) continue;
+                if (B[u] >> lane & 1ull) {
+                  const unsigned t = fill + __builtin_amdgcn_mbcnt_hi((uint32_t)(B[u] >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)B[u], 0u));
+                  const float v = acc[nt][4 * g + u];
+                  // row 8 g + u of the lane's half: (int32) row * stride, in
+                  // wrapping 32-bit arithmetic
+                  qword[wave][t] = pack_cand(-2.0f * v, id0 + (int32_t)((8 * g + u) * stride));
+                  qquery[wave][t] = qrow[nt];
+                }
+                fill += (unsigned)__popcll(B[u]);
+              }
+            }
+          }
+        }
       }
       if (has_next) sstore(buf ^ 1);
       __syncthreads();
       buf ^= 1;
-      if (!DENSE) {
-        const unsigned tail = qtail[par];
-        if (tail - head[par] > (unsigned)(kQueue / 2)) flush(par, tail);
-      }
     }
   };
   if (a.dense) {
@@ -763,9 +692,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
     return;
   }
   run(std::false_type{});
-  // Every append is complete (the last barrier follows the last stage).
-  flush(0, qtail[0]);
-  flush(1, qtail[1]);
+  wave_flush();
 }
 
 // Ring variants (wide / narrow query tiles): the same tiles, waves, queries
