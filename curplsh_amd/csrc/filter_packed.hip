@@ -370,12 +370,12 @@ struct StageShape {
   using G = Geometry<E, DPAD>;
   // Stage tiles: the double-buffered stage within ~48 KB (wide, two 4-wave
   // workgroups per CU), ~36 KB (narrow, three) or ~64 KB (one 8-wave
-  // workgroup per CU), at most 4 tiles. A wide kernel holding more than 80
-  // registers of query operands (BF16 at dpad 96 with 4 query tiles) stages
-  // within ~24 KB: its staging registers are what keeps its tile loop free of
-  // spills.
-  static constexpr bool kBigQ = TN * G::NC * G::LB / 4 > 80;
-  static constexpr int kBudget = NW == 8 ? 64 * 1024 : (WIDE ? (kBigQ ? 24 : 48) * 1024 : 36 * 1024);
+  // workgroup per CU), at most 4 tiles. Larger stages halve the barriers and
+  // stage set-up per tile; a wide BF16 kernel at dpad 96 with 4 query tiles
+  // spills a few 64-bit addresses of the strided-stage path then, and is
+  // still faster (10M x 96: 16.3 -> 15.4 ms per full level; 2M x 128:
+  // 5.50 -> 4.97 ms).
+  static constexpr int kBudget = NW == 8 ? 64 * 1024 : (WIDE ? 48 * 1024 : 36 * 1024);
   static constexpr int kMaxSM = 4;  // more tiles per stage need more staging registers
   static constexpr int kFit = kBudget / (2 * G::TILE);
   static constexpr int SM = kFit > kMaxSM ? kMaxSM : (kFit < 1 ? 1 : kFit);
@@ -632,15 +632,18 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
           }
           continue;
         }
+        // Largest value per lane and query tile (NaN-propagating maximum:
+        // v_maximum3, no canonicalizing moves; a NaN sends the lane to the
+        // value-by-value test below, where it never passes).
         float mx[TN];
         bool hit = false;
 #pragma unroll
         for (int nt = 0; nt < TN; ++nt) {
           float v = acc[nt][0];
 #pragma unroll
-          for (int i = 1; i < 16; ++i) v = fmaxf(v, acc[nt][i]);
+          for (int i = 1; i < 16; ++i) v = __builtin_elementwise_maximum(v, acc[nt][i]);
           mx[nt] = v;
-          hit |= v >= thr[nt];
+          hit |= !(v < thr[nt]);
         }
         if (__any(hit)) {
           // Emit every value >= its query's threshold, one accumulator index
@@ -652,7 +655,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
           const int32_t id0 = (int32_t)((j0 + mt * 32 + 4 * h) * stride);
 #pragma unroll
           for (int nt = 0; nt < TN; ++nt) {
-            if (!__any(mx[nt] >= thr[nt])) continue;
+            if (!__any(!(mx[nt] < thr[nt]))) continue;
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
               uint64_t B[4];
@@ -1029,8 +1032,7 @@ constexpr int lds_tn() {
 
 // Query tiles per wave of the non-residual LDS wide kernel: BF16 holds up to
 // ~96 registers of query operands there and at least 2 tiles (4 at dpad <= 96,
-// 3 at 128, 2 from 160 to 256; the kernels above 80 registers stage within
-// ~24 KB), which spreads each tile's fixed instructions (operand and
+// 3 at 128, 2 from 160 to 256), which spreads each tile's fixed instructions (operand and
 // C-operand reads, skip test, loop) over more MFMAs. Measured on one box
 // (10k queries, full-database launch): 10M x 96 15.42 -> 15.19 ms, 2M x 192
 // 9.70 -> 7.79 ms, 2M x 256 12.34 -> 9.48 ms, 2M x 128 unchanged; 3 tiles at
