@@ -300,7 +300,7 @@ def main() -> int:
             "frac": round(ach / peak, 4),
             "traffic": traffic.get("bytes"),
             # the tuned variant's low byte is the full level's kernel (bits 8-15: the sample levels')
-            "kernel": f"{KERNEL_OF_VARIANT.get(variant & 0xFF, 'filter')}<{form}> full-database launch",
+            "kernel": f"{kernel_of(variant, args.residual_clusters > 0)}<{form}> full-database launch",
             "algorithmic_flops_per_launch": 2.0 * nq * head["rows_filter_full"] * d,
             "rows_per_launch": head["rows_filter_full"],
             "frac_of_measured_mfma": round(ach / measured["mfma_tflops_random"][form], 4),
@@ -336,6 +336,14 @@ def main() -> int:
 KERNEL_OF_VARIANT = {0: "filter_lds_kernel(wide)", 1: "filter_lds_kernel(narrow)",
                      2: "filter_packed_kernel", 3: "filter_ring_kernel(wide)",
                      4: "filter_ring_kernel(narrow)", 5: "filter_lds_kernel(wide, 8 waves)"}
+
+
+def kernel_of(variant, residual):
+    """Kernel of the full level for a tuned variant. Residual levels run the
+    LDS kernel over the residual copy: narrow for variant 1, wide otherwise."""
+    if residual:
+        return "filter_lds_kernel(residual, narrow)" if variant & 0xFF == 1 else "filter_lds_kernel(residual, wide)"
+    return KERNEL_OF_VARIANT.get(variant & 0xFF, "filter")
 
 
 def pmc_pass(args, ca) -> int:
