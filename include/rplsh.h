@@ -281,9 +281,12 @@ rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out)
  *  - (c_factor, rho): the last level keeps ~c_factor * k' rows per query (its
  *    candidate queue; capacity 4x that); its threshold is the rho-th smallest
  *    key of a sample with stride c_factor * k' / rho. Pairs from
- *    {(3,auto),(4,16),(2,32),(3,32),(2,64),(8,8)}, by time plus the expected
- *    cost of re-running queries that keep fewer than k' rows (probability
- *    P(Gamma(rho) < rho / c) per query, one extra pass per failing batch);
+ *    {(3,auto),(4,16),(2,32),(3,32),(2,48),(2,64),(8,8)}, by time plus the
+ *    expected cost of re-running queries that keep fewer than k' rows
+ *    (probability P(Gamma(rho) < rho / c) per query, one extra pass per
+ *    failing batch); the four fastest (grid, c, rho, variant) candidates of
+ *    the sweep are timed again in three interleaved rounds and the lowest
+ *    total wins (one timing cannot order candidates within a few percent);
  *  - variant (BF16/FP8): the filter's data movement and tile shape, by time,
  *    among LDS wide (0), LDS narrow (1) and ring wide (3); direct (2), ring
  *    narrow (4) and LDS wide with 8-wave workgroups (5) can be installed.
