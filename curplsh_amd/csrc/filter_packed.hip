@@ -658,6 +658,12 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
             if (!__any(!(mx[nt] < thr[nt]))) continue;
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
+              // Most hitting query tiles hold one or two values: skip a group
+              // of four on its maximum (three instructions) first.
+              const float gm = __builtin_elementwise_maximum(
+                  __builtin_elementwise_maximum(acc[nt][4 * g], acc[nt][4 * g + 1]),
+                  __builtin_elementwise_maximum(acc[nt][4 * g + 2], acc[nt][4 * g + 3]));
+              if (!__any(!(gm < thr[nt]))) continue;
               uint64_t B[4];
               unsigned cnt = 0u;
 #pragma unroll
