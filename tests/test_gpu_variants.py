@@ -46,6 +46,45 @@ def test_variants_bit_identical(ca, orc, res, precision, n, d):
     idx.clear_tuning()
 
 
+@pytest.mark.parametrize("precision,d", [("bf16", 96), ("fp8", 128)])
+def test_variants_heavy_emission(ca, res, precision, d):
+    """k = 1000 (k' = 4000, ~8,000 survivors per query at the full level) over
+    long slices (target 1,024 workgroups): every wave of the LDS variants
+    emits thousands of survivors, so its LDS queue fills and is moved to the
+    lists many times within a level. The per-wave queues (LDS wide / narrow)
+    must keep exactly the rows the ring kernel's LDS-atomic queues and the
+    direct kernel's global atomics keep: same survivor count, bit-identical
+    results (the queries that keep fewer than k' rows and are re-run included),
+    no index twice in a query's list."""
+    n, k = 300_000, 1000
+    x = ca.gen_gaussian(n, d, 51, True)
+    q = ca.gen_gaussian(512, d, 52, True)
+    idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=precision, kprime_factor=4))
+    idx.add(x)
+    ref = None
+    for v in (0, 1, 3, 2):
+        idx.set_tuning(tuned(k, q.shape[0], v, 1024, c=2, rho=64))
+        I, D = idx.search(q, k)
+        st = idx.last_stats()
+        assert st["filter_variant"] == v and st["kprime"] == 4 * k
+        srt = np.sort(I, axis=1)
+        assert (srt[:, 1:] != srt[:, :-1]).all(), f"variant {v}: an index twice in a result list"
+        if ref is None:
+            ref = (I, D, st["survivors"], st["retried_queries"])
+            print(precision, "survivors per query", st["survivors"] / q.shape[0], "levels", st["levels"],
+                  "re-run", st["retried_queries"])
+            assert st["survivors"] > 1000 * q.shape[0]
+            # the sample levels' ranks reach the last one's: few queries keep
+            # fewer than k' rows (before the planner raised the intermediate
+            # rank, 152 of these 512 did)
+            assert st["retried_queries"] <= q.shape[0] // 50
+            continue
+        assert st["survivors"] == ref[2] and st["retried_queries"] == ref[3], f"variant {v}"
+        assert np.array_equal(I, ref[0]), f"variant {v}"
+        assert np.array_equal(D.view(np.uint32), ref[1].view(np.uint32)), f"variant {v}"
+    idx.clear_tuning()
+
+
 @pytest.mark.parametrize("precision", ["bf16", "fp8"])
 def test_carry_after_incremental_add_and_partial_tile(ca, res, precision):
     """The sample copy and its tile rank map are rebuilt on every add: an index
