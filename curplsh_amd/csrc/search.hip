@@ -1578,7 +1578,7 @@ rplsh_status rplsh_bf_autotune(rplsh_bf* b, const float* sample, int64_t nsample
               [](const Cand& x, const Cand& y) { return x.ms + x.penalty < y.ms + y.penalty; });
     const size_t nfinal = std::min<size_t>(cands.size(), 4);
     std::vector<float> total(nfinal, 0.0f);
-    const int kRounds = 3;
+    const int kRounds = 5;
     for (int round = 0; round < kRounds; ++round)
       for (size_t i = 0; i < nfinal; ++i) total[i] += time_it(cands[i].p);
     size_t pick = 0;
@@ -1602,12 +1602,12 @@ rplsh_status rplsh_bf_autotune(rplsh_bf* b, const float* sample, int64_t nsample
       SearchParams pm = p;
       pm.sample_variant = r.variant == kFilterLdsWide ? kFilterLdsNarrow : kFilterLdsWide;
       float ms_same = 0.0f, ms_mixed = 0.0f;
-      for (int round = 0; round < 3; ++round) {
+      for (int round = 0; round < kRounds; ++round) {
         ms_same += time_it(p);
         ms_mixed += time_it(pm);
       }
       if (ms_mixed < ms_same) {
-        r.ms_tuned -= (ms_same - ms_mixed) / 3;
+        r.ms_tuned -= (ms_same - ms_mixed) / kRounds;
         r.variant = encode_variants(pm.variant, pm.sample_variant);
       }
     }

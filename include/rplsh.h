@@ -285,7 +285,7 @@ rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out)
  *    expected cost of re-running queries that keep fewer than k' rows
  *    (probability P(Gamma(rho) < rho / c) per query, one extra pass per
  *    failing batch); the four fastest (grid, c, rho, variant) candidates of
- *    the sweep are timed again in three interleaved rounds and the lowest
+ *    the sweep are timed again in five interleaved rounds and the lowest
  *    total wins (one timing cannot order candidates within a few percent);
  *  - variant (BF16/FP8): the filter's data movement and tile shape, by time,
  *    among LDS wide (0), LDS narrow (1) and ring wide (3); direct (2), ring
