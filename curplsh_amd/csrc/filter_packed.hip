@@ -493,17 +493,23 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
   const int32_t* rank_src = a.tile_rank ? a.tile_rank : a.counts;  // read only with tile_rank
   const int64_t rank_last = a.tile_rank ? a.rank_tiles - 1 : 0;
 
+  // Row and tile indices of the per-stage loads: 32-bit (an index holds fewer
+  // than 2^31 rows), except over residual copies, whose padded rows may not.
+  using RowT = std::conditional_t<RESID, int64_t, int32_t>;
+  const RowT r_end_t = (RowT)r_end;
+  const RowT stride_t = (RowT)stride;
+  const RowT rank_last_t = (RowT)rank_last;
   auto gload = [&](int64_t j0) {
     pj0 = j0;
     {
       // Tiles past rank_tiles hold no rows (clamped).
-      int64_t u = (j0 >> 5) + (lane < SM ? lane : 0);
-      u = u < rank_last ? u : rank_last;
+      RowT u = (RowT)(j0 >> 5) + (RowT)(lane < SM ? lane : 0);
+      u = u < rank_last_t ? u : rank_last_t;
       grk = rank_src[u];
     }
     {
-      int64_t j = j0 + (tid < SR ? tid : 0);
-      j = (j < r_end ? j : r_end - 1) * stride;
+      RowT j = (RowT)j0 + (RowT)(tid < SR ? tid : 0);
+      j = (j < r_end_t ? j : r_end_t - 1) * stride_t;
       pn = a.xnorm[j];
       if (E == Elem::FP8) pe = a.xscale[j];
     }
@@ -548,7 +554,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
     }
     if (tid < SR) {
       // -||x||^2/2 (residual copies hold it already); -inf past the end
-      const bool in = pj0 + tid < r_end;
+      const bool in = (RowT)pj0 + (RowT)tid < r_end_t;
       sn[buf][tid] = in ? (RESID ? pn : -0.5f * pn) : -INFINITY;
       se[buf][tid] = E == Elem::FP8 && in ? e8m0_of(pe) : 127;
     }
