@@ -197,6 +197,10 @@ __global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) 
 // round trip in the tile loop); the wave moves it to the per-query lists
 // (global atomics) when the next values would not fit, and at the end.
 constexpr int kSeg = 256;  // entries per wave queue (LDS variants)
+// Word 3 of a raw buffer resource on gfx9-family parts (gfx950): 32-bit
+// data format, no swizzle; with stride 0, an offset at or past num_records
+// reads as zero.
+constexpr int kBufferWord3 = 0x00020000;
 // Ring variants: survivors are appended to one of two LDS queues (by stage
 // parity) with LDS atomics. After the barrier that ends stage s, the queue of
 // stage s is flushed to the per-query lists if more than half full: nobody
@@ -499,6 +503,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
   const RowT r_end_t = (RowT)r_end;
   const RowT stride_t = (RowT)stride;
   const RowT rank_last_t = (RowT)rank_last;
+  const int tiles_total = (int)((a.m + 31) >> 5);  // the copy's tiles (the last may be partial)
   auto gload = [&](int64_t j0) {
     pj0 = j0;
     {
@@ -513,9 +518,31 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
       pn = a.xnorm[j];
       if (E == Elem::FP8) pe = a.xscale[j];
     }
-    if (stride == 1 && j0 + SR <= r_end) {
-      // Whole stage at stride 1: j0 is a multiple of 32, so the stage is SM
-      // consecutive whole tiles, one contiguous run of STAGE bytes.
+    if (DPAD <= 128 && stride == 1) {
+      // Stride 1: j0 is a multiple of 32, so the stage is SM consecutive whole
+      // tiles, one contiguous run of STAGE bytes, read through a buffer
+      // resource over the copy's bytes from the stage start: no per-load
+      // 64-bit address arithmetic, and pieces past the copy's last tile read
+      // as zeros (rows past r_end take the C operand -inf, so whatever they
+      // hold never passes a threshold; dense levels skip them). Wider rows
+      // keep the plain loads below: at dpad 256 the resource's registers
+      // cost a spill in the stage loop (2M x 256: +3 %).
+      const int t0 = (int)(j0 >> 5);
+      const int tl = tiles_total - t0;  // > 0
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(xb + (int64_t)t0 * G::TILE), (short)0, tl >= SM ? STAGE : tl * G::TILE, kBufferWord3);
+#pragma unroll
+      for (int t = 0; t < UPT; ++t) {
+        const int u = tid + NT * t;
+        i32x4 v = {0, 0, 0, 0};
+        if (UNITS % NT == 0 || u < UNITS)
+          v = __builtin_bit_cast(i32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, u * 16, 0, 0));
+        pre[t] = v;
+      }
+      return;
+    }
+    if (DPAD > 128 && stride == 1 && j0 + SR <= r_end) {
+      // Whole stage at stride 1 (wide rows): one contiguous run of STAGE bytes.
       const char* src = xb + (j0 >> 5) * G::TILE;
 #pragma unroll
       for (int t = 0; t < UPT; ++t) {
