@@ -1,22 +1,23 @@
 // Fused contraction + threshold filter over an MFMA-tile-packed database
 // (BF16 and FP8 candidate copies). Same arithmetic and filter as
-// filter_kernel (filter.hip); different data movement:
+// filter_kernel (filter.hip); the index stores its candidate copy packed by
+// 32-row tiles: within a tile, K-chunk c is a contiguous block of 64
+// lane-pieces, lane l's piece holding row (l % 32), elements
+// [c*KC + (l/32)*KC/2, +KC/2) -- exactly the A operand of
+// v_mfma_f32_32x32x16_bf16 (KC = 16) or of v_mfma_scale_f32_32x32x64_f8f6f4
+// (KC = 64), so one coalesced 1 KB (BF16) / 2 KB (FP8) read is an operand.
+// Each wave holds TN query tiles (32 queries each) as B operands in registers
+// for the whole launch, so every A operand feeds TN MFMAs. Three data
+// movements:
 //
-//   * The index stores its candidate copy packed by 32-row tiles: within a
-//     tile, K-chunk c is a contiguous block of 64 lane-pieces, lane l's piece
-//     holding row (l % 32), elements [c*KC + (l/32)*KC/2, +KC/2) -- exactly
-//     the A operand of v_mfma_f32_32x32x16_bf16 (KC = 16) or of
-//     v_mfma_scale_f32_32x32x64_f8f6f4 (KC = 64). A wave loads an operand
-//     with one fully coalesced 1 KB (BF16) / 2 KB (FP8) read, straight into
-//     registers: no LDS staging and no workgroup barrier.
-//   * One wave per SIMD, up to TN = 8 query tiles (256 queries) per wave in
-//     registers, so every A operand feeds TN MFMAs: the register file, not
-//     LDS or L1 bandwidth, carries the reuse. The four waves of a workgroup
-//     take different queries and the same database rows, so a tile fetched
-//     by one wave is an L1 hit for the others.
-//   * Chunk c of the next tile is loaded into the operand register of chunk c
-//     as soon as the current tile's MFMAs on it are issued; -||x||^2/2 of the
-//     next tile is prefetched at the top of the iteration.
+//   * filter_lds_kernel (the default, wide or narrow query tiles): the
+//     workgroup streams its slice in stages of up to 4 tiles through LDS,
+//     double buffered, one barrier per stage; survivors go to a queue of the
+//     wave's own in LDS.
+//   * filter_ring_kernel: the same tiles copied by LDS-DMA into a 3-slot ring.
+//   * filter_packed_kernel (direct): operands straight from global memory into
+//     registers, no LDS staging and no workgroup barrier; the waves of a
+//     workgroup share rows through L1.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
