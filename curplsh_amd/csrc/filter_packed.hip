@@ -37,7 +37,12 @@ namespace {
 
 constexpr int kBlock = 256;
 
-
+// Direct variant: one wave per SIMD, up to TN = 8 query tiles per wave. Chunk
+// c of the next tile is loaded into the operand register of chunk c as soon
+// as the current tile's MFMAs on it are issued; -||x||^2/2 of the next tile
+// is prefetched at the top of the iteration. The four waves of a workgroup
+// take different queries and the same rows, so a tile fetched by one wave is
+// an L1 hit for the others.
 template <Elem E, int DPAD, int TN>
 __global__ __launch_bounds__(kBlock, 1) void filter_packed_kernel(FilterArgs a) {
   using G = Geometry<E, DPAD>;
