@@ -400,7 +400,11 @@ struct StageShape {
 // the kernel (FP8: a power-of-two scale per query and centroid), the C
 // operand adds q.c - ||c||^2/2 to the copy's -||r||^2/2, and survivors carry
 // the index row the residual row holds.
-template <Elem E, int DPAD, int TN, bool WIDE, int NW = 4, bool RESID = false>
+// DENSE: the dense first pass (tau = +inf for every query, a.dense set), which
+// writes every row's word to its own slot. It is an instantiation of its own:
+// compiled into the same kernel as the filtering loop, its code made that
+// loop's setup spill registers to scratch memory (up to 184 bytes per lane).
+template <Elem E, int DPAD, int TN, bool WIDE, int NW = 4, bool RESID = false, bool DENSE = false>
 __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_PER_SIMD)) void filter_lds_kernel(
     FilterArgs a) {
   using G = Geometry<E, DPAD>;
@@ -418,6 +422,7 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
   constexpr int CHUNK_UNITS = G::CHUNK / 16;
   constexpr int LANE_UNITS = LB / 16;
   static_assert(SR <= NT, "one row norm per thread");
+  static_assert(!(DENSE && RESID), "residual levels are never dense");
 
   __shared__ __attribute__((aligned(16))) char sx[2][STAGE];
   __shared__ __attribute__((aligned(16))) float sn[2][SR];  // -||x||^2/2, -inf past the end
@@ -559,9 +564,15 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
       }
       return;
     }
+    // General rows (stride > 1, or the last stage of wide rows). The lane's
+    // unit offsets are derived from an opaque copy of tid, so that the
+    // compiler recomputes them here instead of keeping UPT 64-bit pointers
+    // live across the whole stage loop (which spilled one to scratch).
+    int tid_o = tid;
+    asm volatile("" : "+v"(tid_o));
 #pragma unroll
     for (int t = 0; t < UPT; ++t) {
-      const int u = tid + NT * t;
+      const int u = tid_o + NT * t;
       i32x4 v = {0, 0, 0, 0};
       if (UNITS % NT == 0 || u < UNITS) {
         const int mt = u / TILE_UNITS;
@@ -615,11 +626,8 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
     fill = 0u;
   };
 
-  // The stage loop, instantiated twice: the hot loop (DENSE false) stays
-  // small enough for the instruction cache; the dense first pass writes every
-  // row's word to its own slot.
-  auto run = [&](auto dense_tag) {
-    constexpr bool DENSE = decltype(dense_tag)::value;
+  // The stage loop.
+  auto run = [&]() {
     int64_t j0 = r_begin;
     gload(j0);
     sstore(0);
@@ -735,12 +743,8 @@ __global__ __launch_bounds__(NW * 64, (StageShape<E, DPAD, TN, WIDE, NW>::WAVES_
       buf ^= 1;
     }
   };
-  if (a.dense) {
-    run(std::true_type{});
-    return;
-  }
-  run(std::false_type{});
-  wave_flush();
+  run();
+  if constexpr (!DENSE) wave_flush();
 }
 
 // Ring variants (wide / narrow query tiles): the same tiles, waves, queries
@@ -779,6 +783,7 @@ __device__ inline void wait_vm_lgkm() {
 // wait for every copy in flight); the caller waits for what must be complete.
 __device__ inline void stage_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
+// Never launched for a dense pass (the LDS kernel's dense instantiation runs it).
 template <Elem E, int DPAD, int TN, bool WIDE>
 __global__ __launch_bounds__(kBlock, (RingShape<E, DPAD, WIDE>::WAVES_PER_SIMD)) void filter_ring_kernel(
     FilterArgs a) {
@@ -908,8 +913,7 @@ __global__ __launch_bounds__(kBlock, (RingShape<E, DPAD, WIDE>::WAVES_PER_SIMD))
   };
 
   if (tid < 2) qtail[tid] = 0u;
-  auto run = [&](auto dense_tag) {
-    constexpr bool DENSE = decltype(dense_tag)::value;
+  auto run = [&]() {
 #pragma unroll
     for (int i = 0; i < kRing - 1; ++i)
       if (i < n_st) issue(i, r_begin + (int64_t)i * SR);
@@ -925,7 +929,7 @@ __global__ __launch_bounds__(kBlock, (RingShape<E, DPAD, WIDE>::WAVES_PER_SIMD))
       }
       stage_barrier();
       if (s + kRing - 1 < n_st) issue(slot == 0 ? kRing - 1 : slot - 1, j0 + (int64_t)(kRing - 1) * SR);
-      if (!DENSE && s > 0) {
+      if (s > 0) {
         const int par = (s - 1) & 1;
         const unsigned tail = qtail[par];
         if (tail - head[par] > (unsigned)(kQueue / 2)) flush(par, tail);
@@ -952,18 +956,6 @@ __global__ __launch_bounds__(kBlock, (RingShape<E, DPAD, WIDE>::WAVES_PER_SIMD))
           for (int nt = 0; nt < TN; ++nt)
             acc[nt] = mfma_chunk<E>(af, qf[nt][c], c == 0 ? ci : acc[nt], xsc, qsc[nt]);
         }
-        if (DENSE) {
-#pragma unroll
-          for (int nt = 0; nt < TN; ++nt) {
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-              const int64_t row = j0 + mt * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
-              if (row < r_end && qrow[nt] < nq)
-                a.cand[(int64_t)qrow[nt] * a.cap + row] = pack_cand(-2.0f * acc[nt][i], (int32_t)(row * stride));
-            }
-          }
-          continue;
-        }
         float mx[TN];
         bool hit = false;
 #pragma unroll
@@ -981,11 +973,7 @@ __global__ __launch_bounds__(kBlock, (RingShape<E, DPAD, WIDE>::WAVES_PER_SIMD))
       slot = slot == kRing - 1 ? 0 : slot + 1;
     }
   };
-  if (a.dense) {
-    run(std::true_type{});
-    return;
-  }
-  run(std::false_type{});
+  run();
   __syncthreads();
   flush(0, qtail[0]);
   flush(1, qtail[1]);
@@ -1105,6 +1093,12 @@ template <Elem E, int DPAD>
 void launch_packed_one(int variant, const FilterArgs& a, hipStream_t st) {
   const int total = a.num_qtiles * a.num_slices;
   const unsigned grid = (unsigned)round_up(total, 8);
+  if (a.dense) {
+    RPLSH_REQUIRE(variant == kFilterLdsNarrow, "dense pass: the narrow LDS kernel's tiling expected");
+    filter_lds_kernel<E, DPAD, lds_tn<E, DPAD, false>(), false, 4, false, true><<<grid, kBlock, 0, st>>>(a);
+    RPLSH_HIP(hipGetLastError());
+    return;
+  }
   switch (variant) {
     case kFilterDirect:
       filter_packed_kernel<E, DPAD, direct_tn<E, DPAD>()><<<grid, kBlock, 0, st>>>(a);

@@ -261,6 +261,9 @@ static void run_filter(const DbView& db, const Level& lv, bool dense, const void
   FilterArgs fa{};
   filter_source(fa, db, lv);
   fa.dense = dense && db.packed && variant != kFilterDirect && lv.m <= cap ? 1 : 0;
+  // The dense pass runs in the narrow LDS kernel's dense instantiation,
+  // whatever the level's variant: its words do not depend on the tiling.
+  if (fa.dense) variant = kFilterLdsNarrow;
   fa.q = qenc;
   fa.qscale = qscale;
   fa.tau = tau;
