@@ -257,7 +257,7 @@ static void filter_source(FilterArgs& fa, const DbView& db, const Level& lv) {
 static void run_filter(const DbView& db, const Level& lv, bool dense, const void* qenc,
                        const float* qscale, const float* tau, int nq, const int* nq_dev, int cap,
                        int* counts, uint64_t* cand, int64_t target_wgs, int variant,
-                       int64_t skip_tiles, int64_t row_begin, hipStream_t st) {
+                       int64_t skip_tiles, int64_t row_begin, hipStream_t st, bool l2_slices = false) {
   FilterArgs fa{};
   filter_source(fa, db, lv);
   fa.dense = dense && db.packed && variant != kFilterDirect && lv.m <= cap ? 1 : 0;
@@ -277,6 +277,10 @@ static void run_filter(const DbView& db, const Level& lv, bool dense, const void
   fa.row_begin = row_begin;
   const int64_t rows = lv.m - row_begin;
   int64_t slices = std::max<int64_t>(1, div_up(target_wgs, fa.num_qtiles));
+  if (l2_slices && db.packed) {
+    const int64_t row_bytes = (int64_t)db.dpad * (db.elem == Elem::FP8 ? 1 : 2);
+    slices = std::max<int64_t>(slices, div_up(rows * row_bytes, kL2SliceBytes));
+  }
   slices = std::min<int64_t>(slices, div_up(rows, 256));
   slices = std::max<int64_t>(slices, 1);
   fa.rows_per_slice = round_up(div_up(rows, slices), 64);
@@ -440,7 +444,8 @@ void enqueue_batch(const DbView& db, const float* q, int nq, const int* nq_dev,
     } else {
       const int64_t carried = carried_tiles(db, L, li, level_variant(p, li, nlevels));
       run_filter(db, L[li], li == 0, qenc, qscale, tau, nq, nq_dev, cap, counts, cand, p.target_wgs,
-                 level_variant(p, li, nlevels), last ? carried : 0, last ? 0 : 32 * carried, st);
+                 level_variant(p, li, nlevels), last ? carried : 0, last ? 0 : 32 * carried, st,
+                 last && p.l2_slices);
       rank_next = L[li].rank_next;
     }
     if (last) rec(2);
@@ -680,7 +685,7 @@ void finish_batch(const DbView& db, const float* q, int nq, const SearchParams& 
       stats.levels = (int32_t)L.size();
       stats.rows_filter_full = L.back().m - 32 * carried_tiles(db, L, L.size() - 1, p.variant);
     }
-    stats.filter_variant = encode_variants(p.variant, p.sample_variant);
+    stats.filter_variant = encode_variants(p.variant, p.sample_variant, p.l2_slices);
     stats.target_wgs = (int32_t)p.target_wgs;
     if (pe && pe->on) {
       stats.ms_filter_samples = pe->ms(0, 1);
@@ -940,10 +945,11 @@ namespace rplsh {
 DbView bf_db_view(const rplsh_bf* b) { return bf_view(b); }
 
 // A tuning entry's variant field: a full-level variant 0..5, optionally with
-// a sample-level variant 0..5 in bits 8-15 (encode_variants).
+// a sample-level variant 0..5 in bits 8-15 and the L2-sized slices bit 16
+// (encode_variants).
 static bool valid_variants(int v) {
   const int f = full_variant_of(v), sv = sample_variant_of(v);
-  return v >= 0 && (v >> 16) == 0 && f >= kFilterLdsWide && f <= kFilterLdsWide8 && sv >= -1 &&
+  return v >= 0 && (v >> 17) == 0 && f >= kFilterLdsWide && f <= kFilterLdsWide8 && sv >= -1 &&
          sv <= kFilterLdsWide8;
 }
 
@@ -979,6 +985,7 @@ SearchParams bf_search_params(const rplsh_bf* b, int k, int64_t nq) {
     if (valid_variants(best->variant)) {
       p.variant = full_variant_of(best->variant);
       p.sample_variant = sample_variant_of(best->variant);
+      p.l2_slices = l2_slices_of(best->variant);
     }
     if (best->rho > 0) p.rho = best->rho;
   }
@@ -1612,6 +1619,30 @@ rplsh_status rplsh_bf_autotune(rplsh_bf* b, const float* sample, int64_t nsample
       if (ms_mixed < ms_same) {
         r.ms_tuned -= (ms_same - ms_mixed) / kRounds;
         r.variant = encode_variants(pm.variant, pm.sample_variant);
+      }
+    }
+    // 4. The full level in L2-sized slices: the same arithmetic over shorter
+    // slices, whose re-reads by the other query blocks stay in L2 (10M x 96
+    // BF16: 4.33 -> 2.86 GB over HBM per launch). Kept unless slower by more
+    // than the timing noise (0.5 %): at equal speed fewer HBM bytes leave
+    // bandwidth and power to whatever else runs on the chip.
+    if (v.packed && v.nrcopies == 0) {
+      SearchParams p = base;
+      p.target_wgs = r.target_wgs;
+      p.c_factor = r.c_factor;
+      p.rho = r.rho;
+      p.variant = full_variant_of(r.variant);
+      p.sample_variant = sample_variant_of(r.variant);
+      SearchParams pl = p;
+      pl.l2_slices = true;
+      float ms_cur = 0.0f, ms_l2 = 0.0f;
+      for (int round = 0; round < kRounds; ++round) {
+        ms_cur += time_it(p);
+        ms_l2 += time_it(pl);
+      }
+      if (ms_l2 <= 1.005f * ms_cur) {
+        r.ms_tuned += (ms_l2 - ms_cur) / kRounds;
+        r.variant = encode_variants(pl.variant, pl.sample_variant, true);
       }
     }
     (void)hipEventDestroy(e0);

@@ -95,7 +95,7 @@ GraphKey make_key(int index, const rplsh_bf* b, int k, int bucket, const SearchP
   g.kp = p.kp;
   g.c_factor = p.c_factor;
   g.target_wgs = p.target_wgs;
-  g.variant = encode_variants(p.variant, p.sample_variant);
+  g.variant = encode_variants(p.variant, p.sample_variant, p.l2_slices);
   g.rho = p.rho;
   g.fp64_boundary = p.fp64_boundary;
   g.certify = p.certify;

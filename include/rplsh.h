@@ -304,7 +304,10 @@ typedef struct rplsh_tune_result {
   int variant;        /* bits 0-7: the full level's filter variant: 0 LDS wide, 1 LDS
                          narrow, 2 direct, 3 ring wide, 4 ring narrow, 5 LDS wide with
                          8-wave workgroups (BF16/FP8 only); bits 8-15: 1 + the sample
-                         levels' variant (0: the same as the full level) */
+                         levels' variant (0: the same as the full level); bit 16: the
+                         full level in slices of at most 576 KiB of rows, which keeps
+                         the query blocks' re-reads of a slice in L2 (launched
+                         searches; the persistent kernel ignores it) */
   int rho;            /* rank of the last threshold within its sample (0: automatic) */
   float recall;       /* recall@k of the chosen k' on the sample (1 for plain) */
   float ms_default;   /* batch time with the built-in defaults */

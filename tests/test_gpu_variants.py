@@ -127,17 +127,19 @@ def test_carry_after_incremental_add_and_partial_tile(ca, res, precision):
 @pytest.mark.parametrize("precision,n,d", [("bf16", 300_000, 96), ("fp8", 200_000, 128)])
 def test_sample_level_variant(ca, res, precision, n, d):
     """A tuning entry may run the sample levels with another filter kernel
-    than the full level (variant bits 8-15 = 1 + the sample levels' variant):
-    the levels keep the same survivors, the carry still applies between LDS
-    kernels, and the results are bit-identical to one kernel throughout.
-    Malformed encodings are refused."""
+    than the full level (variant bits 8-15 = 1 + the sample levels' variant)
+    and the full level in L2-sized slices (bit 16): the levels keep the same
+    survivors, the carry still applies between LDS kernels, and the results
+    are bit-identical to one kernel throughout. Malformed encodings are
+    refused."""
     x = ca.gen_gaussian(n, d, 81, True)
     q = ca.gen_gaussian(600, d, 82, True)
     k = 100
     idx = ca.IndexBF(res, d, ca.IndexBFConfig(precision=precision, kprime_factor=4))
     idx.add(x)
     out = {}
-    for v in (0, 1, 0 | (2 << 8), 1 | (1 << 8), 0 | (4 << 8)):  # wide, narrow, wide+narrow, narrow+wide, wide+ring
+    # wide, narrow, wide+narrow, narrow+wide, wide+ring; wide and wide+narrow in L2-sized slices
+    for v in (0, 1, 0 | (2 << 8), 1 | (1 << 8), 0 | (4 << 8), 1 << 16, 0 | (2 << 8) | (1 << 16)):
         idx.set_tuning(dict(k=k, batch_nq=q.shape[0], kprime_factor=4, target_wgs=16384, c_factor=2, variant=v,
                             rho=64, recall=0.0, ms_default=0.0, ms_tuned=0.0))
         I, D = idx.search(q, k)
@@ -151,7 +153,17 @@ def test_sample_level_variant(ca, res, precision, n, d):
         assert s == s0, v
     # LDS kernels on both sides: the full level skips the carried sample tiles
     assert out[0 | (2 << 8)][3] == r0 < n
-    for bad in (6, 0 | (7 << 8), 1 << 16, -1):
+    assert out[0 | (2 << 8) | (1 << 16)][3] == r0
+    # A small grid target, below which the L2-sized slices cut the full level
+    # (n x d bytes / 576 KiB slices instead of 32)
+    for v in (0, 1 << 16):
+        idx.set_tuning(dict(k=k, batch_nq=q.shape[0], kprime_factor=4, target_wgs=64, c_factor=2, variant=v,
+                            rho=64, recall=0.0, ms_default=0.0, ms_tuned=0.0))
+        I, D = idx.search(q, k)
+        assert np.array_equal(I, I0), v
+        assert np.array_equal(D.view(np.uint32), D0.view(np.uint32)), v
+        assert idx.last_stats()["survivors"] == s0, v
+    for bad in (6, 0 | (7 << 8), 1 << 17, -1):
         with pytest.raises(Exception):
             idx.set_tuning(dict(k=k, batch_nq=q.shape[0], kprime_factor=4, target_wgs=16384, c_factor=2,
                                 variant=bad, rho=64, recall=0.0, ms_default=0.0, ms_tuned=0.0))
