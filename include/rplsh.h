@@ -291,7 +291,10 @@ rplsh_status rplsh_bf_last_stats(const rplsh_bf* index, rplsh_search_stats* out)
  *    among LDS wide (0), LDS narrow (1) and ring wide (3); direct (2), ring
  *    narrow (4) and LDS wide with 8-wave workgroups (5) can be installed.
  *    With LDS wide or narrow chosen, the sample levels are then timed with
- *    the other of the two (same survivors) and run with it if faster.
+ *    the other of the two (same survivors) and run with it if faster;
+ *  - the full level in L2-sized slices (variant bit 16): timed against the
+ *    chosen grid in five interleaved rounds and kept unless more than 0.5 %
+ *    slower (the same speed with fewer bytes over HBM).
  * target_recall <= 0 keeps the configured k' factor and tunes the rest.
  * The result is installed on the index and used by later searches with the
  * same k (the entry with the nearest batch size). */

@@ -22,9 +22,10 @@ def test_autotune_precision_form(ca, orc, res, tmp_path):
     # the tuner's grid: target workgroups {4096, 8192, 16384}, survivor multiple
     # {2, 3, 4, 8}, filter variant LDS wide / narrow / ring wide, and for the
     # LDS ones possibly the other LDS kernel on the sample levels (bits 8-15:
-    # 1 + its variant)
+    # 1 + its variant), and possibly the full level in L2-sized slices (bit 16)
     assert t["target_wgs"] in (4096, 8192, 16384) and t["c_factor"] in (2, 3, 4, 8)
-    full, sample = t["variant"] & 0xFF, (t["variant"] >> 8) - 1
+    full, sample = t["variant"] & 0xFF, ((t["variant"] >> 8) & 0xFF) - 1
+    assert t["variant"] >> 16 in (0, 1)
     assert full in (0, 1, 3)
     assert sample == -1 or (full in (0, 1) and sample == 1 - full)
     assert t["ms_tuned"] <= t["ms_default"] * 1.5
